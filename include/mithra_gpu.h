@@ -1,0 +1,202 @@
+/*
+ * mithra_gpu.h -- C-ABI of the MI355X FDTD/PIC time-march library (libmithra_gpu.so).
+ *
+ * This is the drop-in boundary under MITHRA's solver plugins.  In the reference the hot
+ * path sits behind the abstract MITHRA::Solver (reference src/solver.h:20-345): the
+ * FdTd / FdTdSC plugins override currentReset / currentUpdate / fieldUpdate /
+ * fieldShift / fieldEvaluate (src/solver.h:139-160), while bunchUpdate and powerSample
+ * are non-virtual Solver members called from Solver::solve() (src/solver.cpp:1300-1399).
+ * Every entry point below replaces one of those calls, cited on its declaration.  The
+ * host keeps the reference's job-file parser and set-up (the C++ drop-in in
+ * mithra_amd/csrc/host), computes every coefficient the reference's way and hands them
+ * over in the plain structs below; the library owns all device memory and one HIP
+ * stream per handle.
+ *
+ * Conventions
+ *  - All functions return 0 on success, a negative MITHRA_E* code on a validation error
+ *    and MITHRA_E_HIP - hipError_t when the HIP runtime fails; mithra_strerror() gives
+ *    the text.  The host wrapper turns a non-zero status into printmessage + exit(1),
+ *    which is the reference's error behaviour (e.g. src/datainput.cpp:41).
+ *  - Node index m = N1*N0*k + N1*i + j (y fastest, then x, then z), as the reference
+ *    (src/solver.cpp:674, src/solver.cpp:2301-2315).  Host-side vector fields are the
+ *    reference's FieldVector layout: 3 consecutive values per node ("AoS3").
+ *  - Particles cross the boundary as the reference's Charge record, 11 doubles each:
+ *    q, rnp[3], rnm[3], gb[3], e   (src/stdinclude.h:130-144).
+ *  - A handle is not thread safe: one host thread per handle.  Work is ordered on the
+ *    handle's stream; only the get/fetch calls and mithra_dev_synchronize wait for it.
+ */
+#ifndef MITHRA_GPU_H_
+#define MITHRA_GPU_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------- */
+#define MITHRA_OK 0
+#define MITHRA_E_ARG (-1)         /* invalid argument or shape                           */
+#define MITHRA_E_UNSUPPORTED (-2) /* a job feature this library does not implement       */
+#define MITHRA_E_STATE (-3)       /* call out of order (e.g. power sample before set-up) */
+#define MITHRA_E_NOMEM (-4)       /* device allocation failed                            */
+#define MITHRA_E_HIP (-1000)      /* MITHRA_E_HIP - hipError_t                           */
+
+/* ---- enumerations: values equal the reference's enums (src/stdinclude.h:17-41) ------- */
+enum { MITHRA_SOLVER_FD = 0, MITHRA_SOLVER_NSFD = 1 };
+enum {
+  MITHRA_SIGNAL_NEUMANN = 0, MITHRA_SIGNAL_GAUSSIAN = 1, MITHRA_SIGNAL_SECANT = 2,
+  MITHRA_SIGNAL_FLATTOP = 3, MITHRA_SIGNAL_INVGAUSSIAN = 4
+};
+enum {
+  MITHRA_SEED_PLANEWAVE = 0, MITHRA_SEED_PLANEWAVE_TRUNCATED = 1, MITHRA_SEED_GAUSSIANBEAM = 2,
+  MITHRA_SEED_SUPERGAUSSIANBEAM = 3
+  /* the standing-wave seed types 4..7 are not handled by the reference's Seed::fields
+     (src/classes.cpp:758-851) and are rejected with MITHRA_E_UNSUPPORTED */
+};
+enum { MITHRA_UNDULATOR_STATIC = 0, MITHRA_UNDULATOR_OPTICAL = 1 };
+enum { MITHRA_PRECISION_FP64 = 0, MITHRA_PRECISION_FP32_BULK = 1 };
+
+/* Mesh of one device domain after the Lorentz boost (Solver::initializeMesh,
+ * src/solver.cpp:601-689; on one domain np_ = N2_ and k0_ = 0). */
+typedef struct mithra_grid {
+  int N0, N1, N2;                 /* nodes along x, y, z                                 */
+  int solver;                     /* MITHRA_SOLVER_*  (Mesh::solver_)                   */
+  int truncation_order;           /* 1 or 2 (Mesh::truncationOrder_)                     */
+  int space_charge;               /* 0: FdTd, 1: FdTdSC (src/mithra.cpp:87-90)           */
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  double zp0, zp1;                /* owned z range, Solver::zp_                          */
+  double dx, dy, dz;              /* boosted mesh resolution                             */
+  double dt;                      /* field time step (Mesh::timeStep_)                   */
+  double Lz;                      /* Mesh::meshLength_[2], period of the pmod ownership  */
+} mithra_grid;
+
+/* Update coefficients exactly as UpdateField carries them (src/database.h:178-213),
+ * computed by the host in Solver::initializeField order (src/solver.cpp:695-824). */
+typedef struct mithra_coeffs {
+  double a[6];                    /* NSFD/FD interior coefficients, a[4] J, a[5] rho     */
+  double bB[5], cB[5], dB[5];     /* Mur faces normal to x, y, z                        */
+  double eE[5], fE[5], gE[5];     /* Mur edges along z, x, y                             */
+  double hC[17];                  /* Mur corners                                         */
+  double alpha;                   /* AdvanceField::alpha_                                */
+  double beta;                    /* AdvanceField::beta_ (= beta / alpha)                */
+} mithra_coeffs;
+
+/* Seed after Solver::setSimulationParameters scaling (src/solver.cpp:74-120) and the boost
+ * parameters copied into it (src/solver.cpp:211-212, :330). */
+typedef struct mithra_seed {
+  int seed_type;                  /* MITHRA_SEED_*                                       */
+  int signal_type;                /* MITHRA_SIGNAL_*                                     */
+  int order[2];                   /* super-gaussian orders                               */
+  unsigned nR;                    /* flat-top rising cycles                              */
+  double amplitude;               /* Seed::amplitude_ ; |amplitude| <= 1e-50 means no seed*/
+  double position[3], direction[3], polarization[3];
+  double radius[2];
+  double t0, s, f0, cep;          /* Signal members after scaling                        */
+  double sigma_invg[2];
+  double gamma, beta, dt_shift, c0;
+} mithra_seed;
+
+/* One undulator module (class Undulator, src/classes.h) after sorting and shifting. */
+typedef struct mithra_undulator {
+  int type;                       /* only MITHRA_UNDULATOR_STATIC is supported           */
+  unsigned length;                /* number of periods                                   */
+  double k, lu, rb, dist, theta;
+} mithra_undulator;
+
+/* Constants of one bunch sub-step (Solver::bunchUpdate, src/solver.cpp:1424-1549, with the
+ * UpdateBunch values of Solver::initializeBunchUpdate, src/solver.cpp:1050-1059). */
+typedef struct mithra_bunch_params {
+  double gamma, beta, c0;         /* frame Lorentz factor, velocity, speed of light       */
+  double dt_shift;                /* Solver::dt_                                         */
+  double r1, r2, dtb;             /* UpdateBunch::r1, r2, dtb                             */
+} mithra_bunch_params;
+
+/* Radiation-power sampling (Solver::initializePowerSample, src/radiation.cpp:18-121).
+ * Only the sampling planes inside [zp0, zp1) are handed over, in the reference's order. */
+typedef struct mithra_power_params {
+  unsigned N;                     /* number of sampling planes                           */
+  unsigned Nl;                    /* number of normalised frequencies                    */
+  unsigned Nf;                    /* ring length (time samples per DFT)                   */
+  double pc;                      /* SampleRadiationPower::pc                            */
+  double gamma, beta, c0;
+  const double* z;                /* N boosted plane positions                            */
+  const double* w;                /* Nl angular frequencies                               */
+} mithra_power_params;
+
+typedef struct mithra_dev mithra_dev;
+
+const char* mithra_strerror(int status);
+
+/* Allocate the device domain (A^{n+1}, A^n, A^{n-1}, phi levels when space_charge, E/B).
+ * Replaces Solver::initializeMesh's allocations (src/solver.cpp:644-658) and
+ * FdTd::FdTd / FdTdSC::FdTdSC (src/fdtd.cpp:10-17, src/fdtdSC.cpp:10-17).  All fields
+ * start at zero. precision: MITHRA_PRECISION_*. */
+int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int precision,
+                      mithra_dev** out);
+int mithra_dev_destroy(mithra_dev* dev);
+int mithra_dev_synchronize(mithra_dev* dev);
+
+/* Potentials.  level 0 = A^n (an_), 1 = A^{n-1} (anm1_), 2 = A^{n+1} (anp1_, which also
+ * holds J between currentUpdate and fieldUpdate, src/fdtd.cpp:25-27).  a is AoS3 with
+ * N0*N1*N2 nodes; phi (N0*N1*N2 values) is used only when space_charge and may be NULL.
+ * The set call replaces the seed initialisation of Solver::initializeField
+ * (src/solver.cpp:828-839) as a copy from the host. */
+int mithra_dev_set_potentials(mithra_dev* dev, int level, const double* a, const double* phi);
+int mithra_dev_get_potentials(mithra_dev* dev, int level, double* a, double* phi);
+
+/* E and B (float AoS3) as last evaluated: Solver::en_, bn_ (src/solver.h:244-245). */
+int mithra_dev_get_fields(mithra_dev* dev, float* e, float* b);
+
+/* Particle list (Solver::chargeVectorn_, src/solver.h:271) as n Charge records. */
+int mithra_dev_set_particles(mithra_dev* dev, const double* charge11, long n);
+int mithra_dev_get_particles(mithra_dev* dev, double* charge11, long n);
+long mithra_dev_particle_count(const mithra_dev* dev);
+
+/* One field step: NSFD/FD interior update of A (and phi), TF/SF seed injection, Mur faces,
+ * edges and corners, then E/B on every node a particle or sampler may read.
+ * Replaces FdTd::fieldUpdate (src/fdtd.cpp:231-800) / FdTdSC::fieldUpdate
+ * (src/fdtdSC.cpp:260-1088) together with the lazy FdTd::fieldEvaluate
+ * (src/fdtd.cpp:818-845, src/fdtdSC.cpp:1110-1145).  time = Solver::time_. */
+int mithra_dev_field_update(mithra_dev* dev, double time, const mithra_seed* seed);
+
+/* rnm = rnp for every particle (src/solver.cpp:1311-1312). */
+int mithra_dev_bunch_begin(mithra_dev* dev);
+
+/* One bunch sub-step: undulator field, gather of E/B, Boris push and drift.
+ * Replaces Solver::bunchUpdate (src/solver.cpp:1424-1576) with Solver::undulatorField
+ * (src/solver.cpp:1798-1880) and Solver::staticUndulator (src/beam.cc:14-76).
+ * time_bunch = Solver::timeBunch_ at this sub-step. */
+int mithra_dev_bunch_update(mithra_dev* dev, const mithra_bunch_params* p,
+                            const mithra_undulator* und, int n_und, double time_bunch);
+
+/* Number of particles that were in the z-range but outside the transverse domain during
+ * the bunch updates since the last call (UpdateBunch::nt, src/solver.cpp:1500-1505,
+ * :1572-1575), and the number that crossed the periodic z boundary (which the reference
+ * duplicates on one rank, src/solver.cpp:1544-1568).  Both counters are reset. */
+int mithra_dev_bunch_counters(mithra_dev* dev, long* transverse_out, long* z_crossings);
+
+/* Radiation power. setup allocates the ring of lab-frame fields (SampleRadiationPower::fdt)
+ * and the DFT phases; sample computes P for every plane and frequency at step n_time and
+ * stores them in history row `row` (row < rows given to setup).  fetch copies `count` rows
+ * starting at `first` to the host as [row][plane][frequency].  Replaces
+ * Solver::initializePowerSample / Solver::powerSample (src/radiation.cpp:18-232). */
+int mithra_dev_power_setup(mithra_dev* dev, const mithra_power_params* p, unsigned rows);
+int mithra_dev_power_sample(mithra_dev* dev, unsigned n_time, unsigned row);
+int mithra_dev_power_fetch(mithra_dev* dev, unsigned first, unsigned count, double* out);
+
+/* Rotate A^{n-1} <- A^n <- A^{n+1} (and phi): FdTd::fieldShift (src/fdtd.cpp:806-812). */
+int mithra_dev_field_shift(mithra_dev* dev);
+
+/* Zero J (held in A^{n+1}) and rho: FdTd::currentReset (src/fdtd.cpp:23-32),
+ * FdTdSC::currentReset (src/fdtdSC.cpp:23-36). */
+int mithra_dev_current_reset(mithra_dev* dev);
+
+/* Charge-conserving zigzag deposition of J (and rho at rnp) from rnm -> rnp:
+ * FdTd::currentUpdate (src/fdtd.cpp:38-185), FdTdSC::currentUpdate (src/fdtdSC.cpp:38-210).
+ * On one domain FdTd::currentCommunicate (src/fdtd.cpp:191-225) does nothing else. */
+int mithra_dev_current_update(mithra_dev* dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MITHRA_GPU_H_ */

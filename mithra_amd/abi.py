@@ -1,0 +1,236 @@
+"""ctypes mirror of include/mithra_gpu.h and a thin wrapper over the device C-ABI.
+
+The structs here are field-for-field the C structs of the header; the wrapper class
+`Domain` binds either the HIP library (libmithra_gpu.so, prefix ``mithra_dev_``) or any
+library with the same entry points under another prefix (the parity tests bind the CPU
+oracle with prefix ``oracle_``), so both can be driven by one call sequence.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+GPU_LIB = os.path.join(LIB_DIR, "libmithra_gpu.so")
+HOST_LIB = os.path.join(LIB_DIR, "libmithra.so")
+
+SOLVER_FD, SOLVER_NSFD = 0, 1
+PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+
+
+class Grid(C.Structure):
+    _fields_ = [("N0", C.c_int), ("N1", C.c_int), ("N2", C.c_int), ("solver", C.c_int),
+                ("truncation_order", C.c_int), ("space_charge", C.c_int),
+                ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
+                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
+                ("zp0", C.c_double), ("zp1", C.c_double), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double), ("Lz", C.c_double)]
+
+    @property
+    def nodes(self):
+        return self.N0 * self.N1 * self.N2
+
+
+class Coeffs(C.Structure):
+    _fields_ = [("a", C.c_double * 6), ("bB", C.c_double * 5), ("cB", C.c_double * 5),
+                ("dB", C.c_double * 5), ("eE", C.c_double * 5), ("fE", C.c_double * 5),
+                ("gE", C.c_double * 5), ("hC", C.c_double * 17), ("alpha", C.c_double),
+                ("beta", C.c_double)]
+
+
+class Seed(C.Structure):
+    _fields_ = [("seed_type", C.c_int), ("signal_type", C.c_int), ("order", C.c_int * 2),
+                ("nR", C.c_uint), ("amplitude", C.c_double), ("position", C.c_double * 3),
+                ("direction", C.c_double * 3), ("polarization", C.c_double * 3),
+                ("radius", C.c_double * 2), ("t0", C.c_double), ("s", C.c_double),
+                ("f0", C.c_double), ("cep", C.c_double), ("sigma_invg", C.c_double * 2),
+                ("gamma", C.c_double), ("beta", C.c_double), ("dt_shift", C.c_double),
+                ("c0", C.c_double)]
+
+
+class Undulator(C.Structure):
+    _fields_ = [("type", C.c_int), ("length", C.c_uint), ("k", C.c_double), ("lu", C.c_double),
+                ("rb", C.c_double), ("dist", C.c_double), ("theta", C.c_double)]
+
+
+class BunchParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("beta", C.c_double), ("c0", C.c_double),
+                ("dt_shift", C.c_double), ("r1", C.c_double), ("r2", C.c_double),
+                ("dtb", C.c_double)]
+
+
+class PowerParams(C.Structure):
+    _fields_ = [("N", C.c_uint), ("Nl", C.c_uint), ("Nf", C.c_uint), ("pc", C.c_double),
+                ("gamma", C.c_double), ("beta", C.c_double), ("c0", C.c_double),
+                ("z", _dp), ("w", _dp)]
+
+
+class MithraError(RuntimeError):
+    pass
+
+
+def _declare(lib, prefix):
+    vp = C.c_void_p
+    sig = {
+        "create": (C.c_int, [C.POINTER(Grid), C.POINTER(Coeffs), C.POINTER(vp)]),
+        "destroy": (C.c_int, [vp]),
+        "set_potentials": (C.c_int, [vp, C.c_int, _dp, _dp]),
+        "get_potentials": (C.c_int, [vp, C.c_int, _dp, _dp]),
+        "get_fields": (C.c_int, [vp, _fp, _fp]),
+        "set_particles": (C.c_int, [vp, _dp, C.c_long]),
+        "get_particles": (C.c_int, [vp, _dp, C.c_long]),
+        "particle_count": (C.c_long, [vp]),
+        "field_update": (C.c_int, [vp, C.c_double, C.POINTER(Seed)]),
+        "bunch_begin": (C.c_int, [vp]),
+        "bunch_update": (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
+                                   C.c_double]),
+        "bunch_counters": (C.c_int, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+        "power_setup": (C.c_int, [vp, C.POINTER(PowerParams), C.c_uint]),
+        "power_sample": (C.c_int, [vp, C.c_uint, C.c_uint]),
+        "power_fetch": (C.c_int, [vp, C.c_uint, C.c_uint, _dp]),
+        "field_shift": (C.c_int, [vp]),
+        "current_reset": (C.c_int, [vp]),
+        "current_update": (C.c_int, [vp]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, prefix + name)
+        if name == "create" and prefix == "mithra_dev_":
+            args = [C.POINTER(Grid), C.POINTER(Coeffs), C.c_int, C.POINTER(vp)]
+        f.restype, f.argtypes = res, args
+        fns[name] = f
+    return fns
+
+
+_GPU = None
+
+
+def load_gpu_library(path=GPU_LIB):
+    """Load libmithra_gpu.so; raises MithraError when it is not built (there is no fallback)."""
+    global _GPU
+    if _GPU is None:
+        if not os.path.exists(path):
+            raise MithraError(f"{path} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(path)
+        lib.mithra_strerror.restype = C.c_char_p
+        lib.mithra_strerror.argtypes = [C.c_int]
+        _GPU = lib
+    return _GPU
+
+
+def _ptr(a, ctype=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Domain:
+    """One simulation domain behind the device C-ABI (or a library with the same API)."""
+
+    def __init__(self, grid, coeffs, lib=None, prefix="mithra_dev_", precision=PRECISION_FP64):
+        self.lib = lib if lib is not None else load_gpu_library()
+        self.prefix = prefix
+        self.f = _declare(self.lib, prefix)
+        self.grid = grid
+        self.h = C.c_void_p()
+        if prefix == "mithra_dev_":
+            self._check(self.f["create"](C.byref(grid), C.byref(coeffs), precision, C.byref(self.h)))
+        else:
+            self._check(self.f["create"](C.byref(grid), C.byref(coeffs), C.byref(self.h)))
+        self._keep = []
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = ""
+            if hasattr(self.lib, "mithra_strerror"):
+                msg = self.lib.mithra_strerror(rc).decode()
+            raise MithraError(f"{self.prefix}: status {rc} {msg}")
+        return rc
+
+    def close(self):
+        if self.h:
+            self._check(self.f["destroy"](self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state transfer ------------------------------------------------------------
+    def set_potentials(self, level, a, phi=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        self._check(self.f["set_potentials"](self.h, level, _ptr(a), _ptr(phi)))
+
+    def get_potentials(self, level):
+        n = self.grid.nodes
+        a = np.empty(3 * n)
+        phi = np.empty(n) if self.grid.space_charge else None
+        self._check(self.f["get_potentials"](self.h, level, _ptr(a), _ptr(phi)))
+        return a, phi
+
+    def get_fields(self):
+        n = self.grid.nodes
+        e = np.empty(3 * n, dtype=np.float32)
+        b = np.empty(3 * n, dtype=np.float32)
+        self._check(self.f["get_fields"](self.h, _ptr(e, C.c_float), _ptr(b, C.c_float)))
+        return e, b
+
+    def set_particles(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        self._check(self.f["set_particles"](self.h, _ptr(q), q.size // 11))
+
+    def get_particles(self):
+        n = self.f["particle_count"](self.h)
+        q = np.empty(11 * n)
+        self._check(self.f["get_particles"](self.h, _ptr(q), n))
+        return q.reshape(n, 11)
+
+    # -- the step --------------------------------------------------------------------
+    def field_update(self, time, seed=None):
+        self._check(self.f["field_update"](self.h, time, None if seed is None else C.byref(seed)))
+
+    def bunch_begin(self):
+        self._check(self.f["bunch_begin"](self.h))
+
+    def bunch_update(self, params, undulators, time_bunch):
+        arr = (Undulator * max(1, len(undulators)))(*undulators)
+        self._check(self.f["bunch_update"](self.h, C.byref(params), arr, len(undulators),
+                                           time_bunch))
+
+    def bunch_counters(self):
+        a, b = C.c_long(), C.c_long()
+        self._check(self.f["bunch_counters"](self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def power_setup(self, params, rows):
+        self._check(self.f["power_setup"](self.h, C.byref(params), rows))
+
+    def power_sample(self, n_time, row):
+        self._check(self.f["power_sample"](self.h, n_time, row))
+
+    def power_fetch(self, first, count, n_planes, n_freq):
+        out = np.empty(count * n_planes * n_freq)
+        self._check(self.f["power_fetch"](self.h, first, count, _ptr(out)))
+        return out.reshape(count, n_planes, n_freq)
+
+    def field_shift(self):
+        self._check(self.f["field_shift"](self.h))
+
+    def current_reset(self):
+        self._check(self.f["current_reset"](self.h))
+
+    def current_update(self):
+        self._check(self.f["current_update"](self.h))
+
+
+def make_power_params(z, w, Nf, pc, gamma, beta, c0):
+    """PowerParams with its two arrays kept alive on the returned object."""
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    p = PowerParams(len(z), len(w), int(Nf), pc, gamma, beta, c0, _ptr(z), _ptr(w))
+    p._arrays = (z, w)
+    return p
