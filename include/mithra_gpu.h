@@ -168,6 +168,14 @@ int mithra_dev_bunch_begin(mithra_dev* dev);
 int mithra_dev_bunch_update(mithra_dev* dev, const mithra_bunch_params* p,
                             const mithra_undulator* und, int n_und, double time_bunch);
 
+/* rnm = rnp followed by n_sub bunch sub-steps in one launch: the loop
+ * `for (t = 0; t < nUpdateBunch_; t += 1) { bunchUpdate(); timeBunch_ += dt_b; }` of
+ * Solver::solve() (src/solver.cpp:1311-1321).  Sub-step s runs at time_bunch accumulated s
+ * times with bunch_time_step, in the reference's order.  The E/B grid is constant over a
+ * field step, so each particle is loaded once and advanced n_sub times in registers. */
+int mithra_dev_bunch_advance(mithra_dev* dev, const mithra_bunch_params* p, const mithra_undulator* und,
+                             int n_und, double time_bunch, double bunch_time_step, int n_sub);
+
 /* Number of particles that were in the z-range but outside the transverse domain during
  * the bunch updates since the last call (UpdateBunch::nt, src/solver.cpp:1500-1505,
  * :1572-1575), and the number that crossed the periodic z boundary (which the reference
@@ -194,6 +202,13 @@ int mithra_dev_current_reset(mithra_dev* dev);
  * FdTd::currentUpdate (src/fdtd.cpp:38-185), FdTdSC::currentUpdate (src/fdtdSC.cpp:38-210).
  * On one domain FdTd::currentCommunicate (src/fdtd.cpp:191-225) does nothing else. */
 int mithra_dev_current_update(mithra_dev* dev);
+
+/* One whole iteration of the Solver::solve() main loop (src/solver.cpp:1303-1399) on the
+ * device, without the host-side diagnostics: field_update, bunch_advance, power_sample (skipped
+ * when power_row < 0), field_shift, current_reset, current_update. */
+int mithra_dev_step(mithra_dev* dev, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+                    const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
+                    int n_sub, unsigned n_time, int power_row);
 
 #ifdef __cplusplus
 }

@@ -95,6 +95,13 @@ def _declare(lib, prefix):
         "current_reset": (C.c_int, [vp]),
         "current_update": (C.c_int, [vp]),
     }
+    if prefix == "mithra_dev_":
+        sig["bunch_advance"] = (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
+                                          C.c_double, C.c_double, C.c_int])
+        sig["synchronize"] = (C.c_int, [vp])
+        sig["step"] = (C.c_int, [vp, C.c_double, C.POINTER(Seed), C.POINTER(BunchParams),
+                                 C.POINTER(Undulator), C.c_int, C.c_double, C.c_double, C.c_int,
+                                 C.c_uint, C.c_int])
     fns = {}
     for name, (res, args) in sig.items():
         f = getattr(lib, prefix + name)
@@ -200,6 +207,23 @@ class Domain:
         arr = (Undulator * max(1, len(undulators)))(*undulators)
         self._check(self.f["bunch_update"](self.h, C.byref(params), arr, len(undulators),
                                            time_bunch))
+
+    def bunch_advance(self, params, undulators, time_bunch, bunch_time_step, n_sub):
+        """rnm = rnp, then n_sub sub-steps in one launch (device library only)."""
+        arr = (Undulator * max(1, len(undulators)))(*undulators)
+        self._check(self.f["bunch_advance"](self.h, C.byref(params), arr, len(undulators),
+                                            time_bunch, bunch_time_step, n_sub))
+
+    def step(self, time, seed, params, undulators, time_bunch, bunch_time_step, n_sub, n_time,
+             power_row=-1):
+        """One whole main-loop iteration on the device (device library only)."""
+        arr = (Undulator * max(1, len(undulators)))(*undulators)
+        self._check(self.f["step"](self.h, time, None if seed is None else C.byref(seed),
+                                   C.byref(params), arr, len(undulators), time_bunch,
+                                   bunch_time_step, n_sub, n_time, power_row))
+
+    def synchronize(self):
+        self._check(self.f["synchronize"](self.h))
 
     def bunch_counters(self):
         a, b = C.c_long(), C.c_long()

@@ -1,0 +1,395 @@
+// C-ABI of libmithra_gpu.so (include/mithra_gpu.h): device memory, transfers between the
+// reference's host layouts and the device layout, and the launch sequence of each call.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+using namespace mithra;
+
+struct mithra_dev : Domain {
+  mithra_undulator* und_dev = nullptr;  // device copy of the undulator table
+  int und_cap = 0;
+  std::vector<mithra_undulator> und_host;  // what und_dev holds
+};
+
+namespace {
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? MITHRA_OK : MITHRA_E_HIP - (int)e; }
+
+#define MITHRA_TRY(expr)                      \
+  do {                                        \
+    hipError_t _e = (expr);                   \
+    if (_e != hipSuccess) return hip_status(_e); \
+  } while (0)
+
+bool valid_grid(const mithra_grid* g) {
+  return g && g->N0 >= 3 && g->N1 >= 3 && g->N2 >= 3 && (g->solver == MITHRA_SOLVER_FD || g->solver == MITHRA_SOLVER_NSFD) &&
+         (g->truncation_order == 1 || g->truncation_order == 2) && g->dx > 0 && g->dy > 0 && g->dz > 0 && g->dt > 0 &&
+         (long)g->N0 * g->N1 * g->N2 < (1L << 40);
+}
+
+int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n) {
+  if (n < 0 || (n > 0 && !und)) return MITHRA_E_ARG;
+  for (int u = 0; u < n; ++u)
+    if (und[u].type != MITHRA_UNDULATOR_STATIC) return MITHRA_E_UNSUPPORTED;
+  if ((int)d->und_host.size() == n && (n == 0 || std::memcmp(d->und_host.data(), und, sizeof(*und) * n) == 0))
+    return MITHRA_OK;
+  if (n > d->und_cap) {
+    if (d->und_dev) MITHRA_TRY(hipFree(d->und_dev));
+    d->und_dev = nullptr;
+    MITHRA_TRY(hipMalloc(&d->und_dev, sizeof(mithra_undulator) * n));
+    d->und_cap = n;
+  }
+  if (n) {
+    MITHRA_TRY(hipStreamSynchronize(d->stream));
+    MITHRA_TRY(hipMemcpy(d->und_dev, und, sizeof(*und) * n, hipMemcpyHostToDevice));
+  }
+  d->und_host.assign(und, und + n);
+  return MITHRA_OK;
+}
+
+int level_of(const mithra_dev* d, int level) {
+  return level == 0 ? d->i_n : level == 1 ? d->i_nm1 : level == 2 ? d->i_np1 : -1;
+}
+
+void free_power(mithra_dev* d) {
+  hipFree(d->pz); hipFree(d->pk); hipFree(d->pdzr); hipFree(d->dft); hipFree(d->ring);
+  hipFree(d->partial); hipFree(d->history);
+  d->pz = nullptr; d->pk = nullptr; d->pdzr = nullptr; d->dft = nullptr; d->ring = nullptr;
+  d->partial = nullptr; d->history = nullptr;
+  delete[] d->plane_of;
+  d->plane_of = nullptr;
+  d->prows = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mithra_strerror(int status) {
+  switch (status) {
+    case MITHRA_OK: return "success";
+    case MITHRA_E_ARG: return "invalid argument or shape";
+    case MITHRA_E_UNSUPPORTED: return "job feature not supported by the GPU library";
+    case MITHRA_E_STATE: return "call out of order";
+    case MITHRA_E_NOMEM: return "device allocation failed";
+  }
+  if (status <= MITHRA_E_HIP) return hipGetErrorString((hipError_t)(MITHRA_E_HIP - status));
+  return "unknown status";
+}
+
+int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int precision, mithra_dev** out) {
+  if (!out || !coeffs || !valid_grid(grid)) return MITHRA_E_ARG;
+  if (precision != MITHRA_PRECISION_FP64 && precision != MITHRA_PRECISION_FP32_BULK) return MITHRA_E_ARG;
+  *out = nullptr;
+  mithra_dev* d = new (std::nothrow) mithra_dev();
+  if (!d) return MITHRA_E_NOMEM;
+  std::memset(static_cast<Domain*>(d), 0, sizeof(Domain));
+  d->g = *grid;
+  d->co = *coeffs;
+  d->precision = precision;
+  d->ncomp = grid->space_charge ? 4 : 3;
+  d->real_size = precision == MITHRA_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+  d->N1N0 = (long)grid->N1 * grid->N0;
+  d->nodes = d->N1N0 * grid->N2;
+  d->i_np1 = 0; d->i_n = 1; d->i_nm1 = 2;
+  hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete d; return hip_status(e); }
+  const size_t per = (size_t)d->nodes * d->real_size;
+  // 256-byte aligned component arrays inside one allocation
+  const size_t stride = (per + 255) / 256 * 256;
+  if (hipMalloc(&d->pot_block, stride * 3 * d->ncomp) != hipSuccess ||
+      hipMalloc(&d->eb, sizeof(float4) * 2 * (size_t)d->nodes) != hipSuccess ||
+      hipMalloc(&d->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    mithra_dev_destroy(d);
+    return MITHRA_E_NOMEM;
+  }
+  for (int l = 0; l < 3; ++l)
+    for (int c = 0; c < 4; ++c)
+      d->lvl[l].c[c] = c < d->ncomp ? (char*)d->pot_block + stride * (l * d->ncomp + c) : nullptr;
+  int rc = hip_status(launch_fill_zero(d, d->pot_block, stride * 3 * d->ncomp));
+  if (!rc) rc = hip_status(launch_fill_zero(d, d->eb, sizeof(float4) * 2 * (size_t)d->nodes));
+  if (!rc) rc = hip_status(launch_fill_zero(d, d->counters, 2 * sizeof(unsigned long long)));
+  if (!rc) rc = hip_status(hipStreamSynchronize(d->stream));
+  if (rc) { mithra_dev_destroy(d); return rc; }
+  *out = d;
+  return MITHRA_OK;
+}
+
+int mithra_dev_destroy(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  if (d->stream) hipStreamSynchronize(d->stream);
+  hipFree(d->pot_block);
+  hipFree(d->eb);
+  hipFree(d->counters);
+  hipFree(d->part);
+  hipFree(d->und_dev);
+  free_power(d);
+  if (d->stream) hipStreamDestroy(d->stream);
+  delete d;
+  return MITHRA_OK;
+}
+
+int mithra_dev_synchronize(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  return hip_status(hipStreamSynchronize(d->stream));
+}
+
+int mithra_dev_set_potentials(mithra_dev* d, int level, const double* a, const double* phi) {
+  if (!d) return MITHRA_E_ARG;
+  const int li = level_of(d, level);
+  if (li < 0) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  const size_t n = (size_t)d->nodes;
+  std::vector<double> dbl;
+  std::vector<float> flt;
+  auto put = [&](int c, const double* src, size_t stride) -> hipError_t {
+    if (d->precision == MITHRA_PRECISION_FP64) {
+      dbl.resize(n);
+      for (size_t m = 0; m < n; ++m) dbl[m] = src[stride * m];
+      return hipMemcpy(d->lvl[li].c[c], dbl.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    }
+    flt.resize(n);
+    for (size_t m = 0; m < n; ++m) flt[m] = (float)src[stride * m];
+    return hipMemcpy(d->lvl[li].c[c], flt.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  };
+  if (a)
+    for (int c = 0; c < 3; ++c) MITHRA_TRY(put(c, a + c, 3));
+  if (phi && d->ncomp == 4) MITHRA_TRY(put(3, phi, 1));
+  return MITHRA_OK;
+}
+
+int mithra_dev_get_potentials(mithra_dev* d, int level, double* a, double* phi) {
+  if (!d) return MITHRA_E_ARG;
+  const int li = level_of(d, level);
+  if (li < 0) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  const size_t n = (size_t)d->nodes;
+  std::vector<double> dbl(n);
+  std::vector<float> flt;
+  auto get = [&](int c, double* dst, size_t stride) -> hipError_t {
+    hipError_t e;
+    if (d->precision == MITHRA_PRECISION_FP64) {
+      e = hipMemcpy(dbl.data(), d->lvl[li].c[c], n * sizeof(double), hipMemcpyDeviceToHost);
+    } else {
+      flt.resize(n);
+      e = hipMemcpy(flt.data(), d->lvl[li].c[c], n * sizeof(float), hipMemcpyDeviceToHost);
+      for (size_t m = 0; m < n; ++m) dbl[m] = flt[m];
+    }
+    if (e == hipSuccess)
+      for (size_t m = 0; m < n; ++m) dst[stride * m] = dbl[m];
+    return e;
+  };
+  if (a)
+    for (int c = 0; c < 3; ++c) MITHRA_TRY(get(c, a + c, 3));
+  if (phi && d->ncomp == 4) MITHRA_TRY(get(3, phi, 1));
+  return MITHRA_OK;
+}
+
+int mithra_dev_get_fields(mithra_dev* d, float* e, float* b) {
+  if (!d) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  std::vector<float4> h(2 * (size_t)d->nodes);
+  MITHRA_TRY(hipMemcpy(h.data(), d->eb, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+  for (size_t m = 0; m < (size_t)d->nodes; ++m) {
+    if (e) { e[3 * m] = h[2 * m].x; e[3 * m + 1] = h[2 * m].y; e[3 * m + 2] = h[2 * m].z; }
+    if (b) { b[3 * m] = h[2 * m].w; b[3 * m + 1] = h[2 * m + 1].x; b[3 * m + 2] = h[2 * m + 1].y; }
+  }
+  return MITHRA_OK;
+}
+
+int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
+  if (!d || n < 0 || (n > 0 && !q)) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  if (n > d->cap) {
+    hipFree(d->part);
+    d->part = nullptr;
+    d->cap = 0;
+    const long cap = (n + 63) / 64 * 64;
+    if (hipMalloc(&d->part, sizeof(double) * PCOUNT * (size_t)cap) != hipSuccess) return MITHRA_E_NOMEM;
+    d->cap = cap;
+  }
+  d->np = n;
+  if (n == 0) return MITHRA_OK;
+  // Charge record q, rnp[3], rnm[3], gb[3], e  ->  x y z xm ym zm gx gy gz q e
+  static const int from[PCOUNT] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 0, 10};
+  std::vector<double> col(n);
+  for (int a = 0; a < PCOUNT; ++a) {
+    for (long p = 0; p < n; ++p) col[p] = q[11 * p + from[a]];
+    MITHRA_TRY(hipMemcpy(particle_array(d, a), col.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  return MITHRA_OK;
+}
+
+int mithra_dev_get_particles(mithra_dev* d, double* q, long n) {
+  if (!d || n != d->np || (n > 0 && !q)) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  static const int from[PCOUNT] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 0, 10};
+  std::vector<double> col(n);
+  for (int a = 0; a < PCOUNT; ++a) {
+    MITHRA_TRY(hipMemcpy(col.data(), particle_array(d, a), sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (long p = 0; p < n; ++p) q[11 * p + from[a]] = col[p];
+  }
+  return MITHRA_OK;
+}
+
+long mithra_dev_particle_count(const mithra_dev* d) { return d ? d->np : -1; }
+
+int mithra_dev_field_update(mithra_dev* d, double time, const mithra_seed* seed) {
+  if (!d) return MITHRA_E_ARG;
+  const bool seeded = seed && std::fabs(seed->amplitude) > 1.0e-50;
+  if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
+      seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
+    return MITHRA_E_UNSUPPORTED;
+  MITHRA_TRY(launch_interior(d));
+  if (seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4) MITHRA_TRY(launch_tfsf(d, time, seed));
+  MITHRA_TRY(launch_boundaries(d));
+  MITHRA_TRY(launch_eval_eb(d));
+  return MITHRA_OK;
+}
+
+int mithra_dev_bunch_begin(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  return hip_status(launch_bunch_begin(d));
+}
+
+int mithra_dev_bunch_update(mithra_dev* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+                            double time_bunch) {
+  if (!d || !p) return MITHRA_E_ARG;
+  int rc = upload_undulators(d, und, n_und);
+  if (rc) return rc;
+  return hip_status(launch_push(d, p, d->und_dev, n_und, time_bunch, 0.0, 1, 0));
+}
+
+int mithra_dev_bunch_advance(mithra_dev* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+                             double time_bunch, double bunch_time_step, int n_sub) {
+  if (!d || !p || n_sub < 0) return MITHRA_E_ARG;
+  int rc = upload_undulators(d, und, n_und);
+  if (rc) return rc;
+  return hip_status(launch_push(d, p, d->und_dev, n_und, time_bunch, bunch_time_step, n_sub, 1));
+}
+
+int mithra_dev_bunch_counters(mithra_dev* d, long* transverse_out, long* z_crossings) {
+  if (!d) return MITHRA_E_ARG;
+  unsigned long long h[2];
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  MITHRA_TRY(hipMemcpy(h, d->counters, sizeof h, hipMemcpyDeviceToHost));
+  if (transverse_out) *transverse_out = (long)h[0];
+  if (z_crossings) *z_crossings = (long)h[1];
+  MITHRA_TRY(launch_fill_zero(d, d->counters, sizeof h));
+  return MITHRA_OK;
+}
+
+int mithra_dev_power_setup(mithra_dev* d, const mithra_power_params* p, unsigned rows) {
+  if (!d || !p || p->Nf == 0 || rows == 0 || (p->N > 0 && !p->z) || (p->Nl > 0 && !p->w)) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  free_power(d);
+  const mithra_grid& g = d->g;
+  d->pN = p->N; d->pNl = p->Nl; d->pNf = p->Nf; d->pc = p->pc;
+  d->pgamma = p->gamma; d->pbeta = p->beta; d->pc0 = p->c0;
+  d->plane_of = new int[p->N > 0 ? p->N : 1];
+  std::vector<int> hk;
+  std::vector<double> hdzr, hz;
+  unsigned nz = 0;
+  for (unsigned k = 0; k < p->N; ++k) {
+    if (p->z[k] < g.zp1 && p->z[k] >= g.zp0) {  // src/radiation.cpp:157
+      double c;
+      double dzr = std::modf((p->z[k] - g.zmin) / g.dz, &c);
+      if ((int)c < 0 || (int)c + 1 >= g.N2) return MITHRA_E_ARG;
+      hk.push_back((int)c);
+      hdzr.push_back(dzr);
+      hz.push_back(p->z[k]);
+      d->plane_of[k] = (int)nz++;
+    } else {
+      d->plane_of[k] = -1;
+    }
+  }
+  d->pNz = nz;
+  // DFT phases exactly as the reference builds ep/em (src/radiation.cpp:110-117)
+  std::vector<double> dft((size_t)p->Nl * p->Nf * 2);
+  for (unsigned i = 0; i < p->Nl; ++i)
+    for (unsigned j = 0; j < p->Nf; ++j) {
+      dft[((size_t)i * p->Nf + j) * 2] = std::cos(p->w[i] * j * g.dt);
+      dft[((size_t)i * p->Nf + j) * 2 + 1] = std::sin(p->w[i] * j * g.dt);
+    }
+  const long ni = (long)(g.N0 - 4) * (g.N1 - 4);
+  d->pblocks = ni > 0 ? (int)((ni + 255) / 256) : 1;
+  std::vector<int> kz(hk);
+  for (unsigned k = 0; k < p->N; ++k) kz.push_back(d->plane_of[k]);
+  const size_t ring = (size_t)p->Nf * 4 * (nz ? nz : 1) * d->N1N0;
+  if (hipMalloc(&d->pk, sizeof(int) * (kz.size() ? kz.size() : 1)) != hipSuccess ||
+      hipMalloc(&d->pdzr, sizeof(double) * (nz ? nz : 1)) != hipSuccess ||
+      hipMalloc(&d->pz, sizeof(double) * (nz ? nz : 1)) != hipSuccess ||
+      hipMalloc(&d->dft, sizeof(double) * (dft.size() ? dft.size() : 1)) != hipSuccess ||
+      hipMalloc(&d->ring, sizeof(double) * ring) != hipSuccess ||
+      hipMalloc(&d->partial, sizeof(double) * (size_t)(nz ? nz : 1) * d->pblocks * (p->Nl ? p->Nl : 1)) != hipSuccess ||
+      hipMalloc(&d->history, sizeof(double) * (size_t)rows * (p->N ? p->N : 1) * (p->Nl ? p->Nl : 1)) != hipSuccess) {
+    free_power(d);
+    return MITHRA_E_NOMEM;
+  }
+  if (!kz.empty()) MITHRA_TRY(hipMemcpy(d->pk, kz.data(), sizeof(int) * kz.size(), hipMemcpyHostToDevice));
+  if (nz) {
+    MITHRA_TRY(hipMemcpy(d->pdzr, hdzr.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    MITHRA_TRY(hipMemcpy(d->pz, hz.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+  }
+  if (!dft.empty()) MITHRA_TRY(hipMemcpy(d->dft, dft.data(), sizeof(double) * dft.size(), hipMemcpyHostToDevice));
+  MITHRA_TRY(launch_fill_zero(d, d->ring, sizeof(double) * ring));
+  MITHRA_TRY(launch_fill_zero(d, d->history, sizeof(double) * (size_t)rows * (p->N ? p->N : 1) * (p->Nl ? p->Nl : 1)));
+  d->prows = rows;
+  return hip_status(hipStreamSynchronize(d->stream));
+}
+
+int mithra_dev_power_sample(mithra_dev* d, unsigned n_time, unsigned row) {
+  if (!d) return MITHRA_E_ARG;
+  if (!d->prows || row >= d->prows) return MITHRA_E_STATE;
+  return hip_status(launch_power(d, n_time, row));
+}
+
+int mithra_dev_power_fetch(mithra_dev* d, unsigned first, unsigned count, double* out) {
+  if (!d || (count && !out)) return MITHRA_E_ARG;
+  if (!d->prows || first + count > d->prows) return MITHRA_E_STATE;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  const size_t w = (size_t)d->pN * d->pNl;
+  if (count && w)
+    MITHRA_TRY(hipMemcpy(out, d->history + first * w, sizeof(double) * w * count, hipMemcpyDeviceToHost));
+  return MITHRA_OK;
+}
+
+int mithra_dev_field_shift(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  const int t = d->i_nm1;
+  d->i_nm1 = d->i_n;
+  d->i_n = d->i_np1;
+  d->i_np1 = t;
+  return MITHRA_OK;
+}
+
+int mithra_dev_current_reset(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  for (int c = 0; c < d->ncomp; ++c)
+    MITHRA_TRY(launch_fill_zero(d, d->lvl[d->i_np1].c[c], (size_t)d->nodes * d->real_size));
+  return MITHRA_OK;
+}
+
+int mithra_dev_current_update(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  return hip_status(launch_deposit(d));
+}
+
+int mithra_dev_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+                    const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step, int n_sub,
+                    unsigned n_time, int power_row) {
+  int rc = mithra_dev_field_update(d, time, seed);
+  if (!rc) rc = mithra_dev_bunch_advance(d, p, und, n_und, time_bunch, bunch_time_step, n_sub);
+  if (!rc && power_row >= 0) rc = mithra_dev_power_sample(d, n_time, (unsigned)power_row);
+  if (!rc) rc = mithra_dev_field_shift(d);
+  if (!rc) rc = mithra_dev_current_reset(d);
+  if (!rc) rc = mithra_dev_current_update(d);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Shared declarations of the MI355X (gfx950) FDTD/PIC library.
+//
+// Device layout of one domain (one GPU holds the whole mesh):
+//   potentials  SoA per component, three time levels rotated by index:  A[level][c][m],
+//               phi[level][m]; m = N1*N0*k + N1*i + j (y fastest, then x, then z), the
+//               reference's node order, so every z-march reads contiguous y rows.
+//   E/B         two float4 per node: {Ex, Ey, Ez, Bx}, {By, Bz, 0, 0}; the trilinear gather
+//               reads a node's six values with two 16-byte loads.
+//   particles   SoA doubles: x y z | xm ym zm | gx gy gz | q | e  (the Charge record split by
+//               member so that every kernel streams only what it uses).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../../include/mithra_gpu.h"
+
+namespace mithra {
+
+// Constants of the reference (src/stdinclude.h:43-50); PI is truncated on purpose.
+constexpr double kPI = 3.1415926535;
+constexpr double kEC = 1.602e-19;
+constexpr double kEM = 9.109e-31;
+
+enum Particle { PX, PY, PZ, PXM, PYM, PZM, PGX, PGY, PGZ, PQ, PE, PCOUNT };
+
+struct FieldSet {           // one time level of every potential component
+  void* c[4];               // A_x, A_y, A_z, phi (phi only with space charge)
+};
+
+struct Domain {
+  mithra_grid g;
+  mithra_coeffs co;
+  int precision;            // MITHRA_PRECISION_*
+  int ncomp;                // 3, or 4 with space charge
+  size_t real_size;         // bytes per potential value
+  long N1N0, nodes;
+  hipStream_t stream;
+  void* pot_block;          // one allocation: 3 levels x ncomp x nodes
+  FieldSet lvl[3];          // physical buffers
+  int i_np1, i_n, i_nm1;    // which physical buffer is A^{n+1}, A^n, A^{n-1}
+  float4* eb;               // 2 per node
+  // particles
+  long np;                  // particle count
+  long cap;                 // allocated capacity
+  double* part;             // PCOUNT arrays of cap doubles
+  unsigned long long* counters;   // [0] transverse escapes, [1] z-boundary crossings
+  // power sampling
+  unsigned pN, pNz, pNl, pNf, prows;
+  double pc, pgamma, pbeta, pc0;
+  double* pz;               // device: plane positions (only planes inside the domain)
+  int* pk;                  // device: plane cell index
+  double* pdzr;             // device: plane fractional position
+  double* dft;              // device: [Nl][Nf][4] = cos, sin  (ep = cos + i sin, em = cos - i sin)
+  double* ring;             // device: [Nf][4][Nz][N1N0]
+  double* partial;          // device: per-block partial sums
+  double* history;          // device: [rows][N][Nl]
+  int pblocks;
+  int* plane_of;            // host: for each of N planes, its index among inside planes or -1
+};
+
+inline double* particle_array(Domain* d, int which) { return d->part + (size_t)which * d->cap; }
+
+// Launch helpers implemented in field.hip, particles.hip and power.hip.
+hipError_t launch_interior(Domain* d);
+hipError_t launch_tfsf(Domain* d, double time, const mithra_seed* seed);
+hipError_t launch_boundaries(Domain* d);
+hipError_t launch_eval_eb(Domain* d);
+hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes);
+hipError_t launch_bunch_begin(Domain* d);
+hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+                       double time_bunch, double bunch_time_step, int n_sub, int save_rnm);
+hipError_t launch_deposit(Domain* d);
+hipError_t launch_power(Domain* d, unsigned n_time, unsigned row);
+
+}  // namespace mithra

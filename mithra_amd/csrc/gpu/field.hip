@@ -1,0 +1,521 @@
+// Field kernels: interior NSFD/FD update, TF/SF seed injection, Mur absorbing boundaries
+// and the E/B evaluation, for one domain resident in HBM.
+//
+// All arithmetic keeps the reference's source order and the library is built with
+// -ffp-contract=off, so on identical inputs the FP64 path reproduces the reference's
+// potentials bit for bit.
+#include <cmath>
+
+#include "common.h"
+
+namespace mithra {
+
+namespace {
+
+constexpr int kTileJ = 64;      // one wavefront along y (contiguous in memory)
+constexpr int kTileI = 4;       // rows of x per workgroup
+constexpr int kChunkK = 64;     // z planes marched by one workgroup
+constexpr int kNumXcd = 8;      // MI355X: 8 accelerator complex dies, each with its own L2
+
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b % 8).
+// Returns the logical tile of workgroup b such that every XCD receives one contiguous range
+// of logical tiles: neighbouring tiles, which share their halo rows, then meet in one L2.
+__device__ inline int xcd_logical_block(int b, int nblk) {
+  int q = nblk / kNumXcd, r = nblk % kNumXcd;
+  int xcd = b % kNumXcd, local = b / kNumXcd;
+  int base = (xcd < r) ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return base + local;
+}
+
+struct Plane5 {  // one z plane of a column and its four transverse neighbours
+  double c, xp, xm, yp, ym;
+};
+
+template <typename R>
+__device__ inline Plane5 load_plane(const R* __restrict__ a, long m, long N1) {
+  Plane5 p;
+  p.c = (double)a[m];
+  p.xp = (double)a[m + N1];
+  p.xm = (double)a[m - N1];
+  p.yp = (double)a[m + 1];
+  p.ym = (double)a[m - 1];
+  return p;
+}
+
+// Interior update (AdvanceField::advance*PotentialNSFD / FD, reference src/database.cpp:44-134,
+// loops src/fdtd.cpp:270-303 and src/fdtdSC.cpp:317-366).  Each thread owns one (i, j) column
+// and marches kChunkK planes along z with the three planes it needs held in registers; the
+// x/y neighbours it reads are the centres of neighbouring lanes and rows, served by L1/L2.
+// J (or rho) lives in the A^{n+1} buffer and is overwritten in place, as in the reference.
+template <typename R, bool NSFD>
+__global__ __launch_bounds__(kTileJ* kTileI) void k_interior(
+    FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
+    int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
+    double alpha, double beta) {
+  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tj = lb % tilesJ;
+  lb /= tilesJ;
+  const int ti = lb % tilesI;
+  lb /= tilesI;
+  const int kc = lb % chunks;
+  const int comp = lb / chunks;
+  const int j = tj * kTileJ + (threadIdx.x % kTileJ);
+  const int i = 1 + ti * kTileI + (threadIdx.x / kTileJ);
+  const int kb = 1 + kc * kChunkK;
+  const int ke = min(kb + kChunkK, N2 - 1);
+  if (comp >= ncomp || i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
+  const R* __restrict__ an = (const R*)an_set.c[comp];
+  const R* __restrict__ anm1 = (const R*)anm1_set.c[comp];
+  R* __restrict__ anp1 = (R*)anp1_set.c[comp];
+  const double as = comp < 3 ? a4 : a5;
+  const long N1N0 = (long)N1 * N0;
+  long m = N1N0 * kb + (long)N1 * i + j;
+  Plane5 pm = load_plane(an, m - N1N0, N1);
+  Plane5 p0 = load_plane(an, m, N1);
+  for (int k = kb; k < ke; ++k, m += N1N0) {
+    Plane5 pp = load_plane(an, m + N1N0, N1);
+    double v1 = (double)anm1[m];
+    double src = (double)anp1[m];
+    double r;
+    if (NSFD) {
+      double tx = p0.xp + p0.xm + beta * (pp.xp + pm.xp + pp.xm + pm.xm);
+      double ty = p0.yp + p0.ym + beta * (pp.yp + pm.yp + pp.ym + pm.ym);
+      r = a0 * p0.c - v1 + alpha * (a1 * tx + a2 * ty) + a3 * (pp.c + pm.c) + as * src;
+    } else {
+      r = a0 * p0.c - v1 + a1 * (p0.xp + p0.xm) + a2 * (p0.yp + p0.ym) + a3 * (pp.c + pm.c) + as * src;
+    }
+    anp1[m] = (R)r;
+    pm = p0;
+    p0 = pp;
+  }
+}
+
+// ---- analytic seed (Signal::self src/classes.cpp:534-575, Seed::fields :740-855) --------
+
+__device__ double signal(const mithra_seed& s, double t, double phase) {
+  const double t0 = s.t0, ss = s.s, f0 = s.f0, cep = s.cep;
+  if (fabs(t - t0) > 10.0 * ss) return 0.0;
+  const double c = cos(2 * kPI * f0 * (t - t0) + cep + phase);
+  switch (s.signal_type) {
+    case MITHRA_SIGNAL_NEUMANN:
+      return -cos(2 * kPI * f0 * (t - t0) + cep + phase) * 2.7724 * (t - t0) / (ss * ss) *
+             exp(-1.3863 * (t - t0) * (t - t0) / (ss * ss));
+    case MITHRA_SIGNAL_GAUSSIAN:
+      return c * exp(-1.3863 * pow((t - t0) / ss, 2));
+    case MITHRA_SIGNAL_SECANT:
+      return c / cosh((t - t0) / ss);
+    case MITHRA_SIGNAL_FLATTOP:
+      if (t - t0 <= -ss / 2.0) return c * exp(-pow((t - t0 + ss / 2.0) * f0 / s.nR, 2));
+      if (t - t0 <= ss / 2.0) return c;
+      return c * exp(-pow((t - t0 - ss / 2.0) * f0 / s.nR, 2));
+    case MITHRA_SIGNAL_INVGAUSSIAN: {
+      double env = pow((1.0 + pow((t - t0) / s.sigma_invg[0], 2)) * (1.0 + pow((t - t0) / s.sigma_invg[1], 2)), 0.25);
+      if (t - t0 <= -ss / 2.0) return c * env * exp(-pow((t - t0 + ss / 2.0) * f0 / s.nR, 2));
+      if (t - t0 <= ss / 2.0) return c * env;
+      return c * env * exp(-pow((t - t0 - ss / 2.0) * f0 / s.nR, 2));
+    }
+  }
+  return 0.0;
+}
+
+// Plane-wave, truncated plane-wave and Gaussian-beam seeds (the types whose output does not
+// depend on the previous call).  Writes the boosted-frame vector potential a[3].
+__device__ void seed_fields(const mithra_seed& s, double r0, double r1, double r2, double time, double a[3]) {
+  double rv[3];
+  rv[0] = r0 - s.position[0];
+  rv[1] = r1 - s.position[1];
+  rv[2] = s.gamma * (r2 + s.beta * s.c0 * (time + s.dt_shift)) - s.position[2];
+  double tl = s.gamma * (time + s.dt_shift + s.beta / s.c0 * r2);
+  const double z = rv[0] * s.direction[0] + rv[1] * s.direction[1] + rv[2] * s.direction[2];
+  tl -= z / s.c0;
+  const double* pol = s.polarization;
+  const double* dir = s.direction;
+  double amp = 0.0;
+  double ts = signal(s, tl, 0.0);
+  bool zero = fabs(ts) < 1.0e-6;
+  if (s.seed_type == MITHRA_SEED_PLANEWAVE) {
+    amp = s.amplitude * ts;
+  } else {
+    double x = rv[0] * pol[0] + rv[1] * pol[1] + rv[2] * pol[2];
+    double yv0 = dir[1] * pol[2] - dir[2] * pol[1];
+    double yv1 = dir[2] * pol[0] - dir[0] * pol[2];
+    double yv2 = dir[0] * pol[1] - dir[1] * pol[0];
+    double y = rv[0] * yv0 + rv[1] * yv1 + rv[2] * yv2;
+    if (s.seed_type == MITHRA_SEED_PLANEWAVE_TRUNCATED) {
+      zero = zero || fabs(x) > s.radius[0] || fabs(y) > s.radius[1];
+      amp = s.amplitude * ts;
+    } else if (!zero) {
+      const double l = s.c0 / s.f0;
+      const double zRp = kPI * s.radius[0] * s.radius[0] / l;
+      const double wrp = sqrt(1.0 + z * z / (zRp * zRp));
+      const double zRs = kPI * s.radius[1] * s.radius[1] / l;
+      const double wrs = sqrt(1.0 + z * z / (zRs * zRs));
+      const double p = 0.5 * (atan(z / zRp) + atan(z / zRs) - kPI) -
+                       kPI * z / l * (pow(x / (zRp * wrp), 2) + pow(y / (zRs * wrs), 2));
+      ts = signal(s, tl, p);
+      const double t = exp(-pow(x / (s.radius[0] * wrp), 2) - pow(y / (s.radius[1] * wrs), 2)) / sqrt(wrs * wrp) *
+                       s.amplitude;
+      amp = t * ts;
+    }
+  }
+  if (zero) {
+    a[0] = a[1] = a[2] = 0.0;
+  } else {
+    a[0] = amp * pol[0];
+    a[1] = amp * pol[1];
+    a[2] = amp * pol[2];
+  }
+  a[2] *= s.gamma;
+}
+
+template <typename R>
+__device__ inline void tfsf_apply(FieldSet A, long m, double px, double py, double pz, double time,
+                                  const mithra_seed& s, double coef, bool minus) {
+  double a[3];
+  seed_fields(s, px, py, pz, time, a);
+  for (int c = 0; c < 3; ++c) {
+    R* p = (R*)A.c[c];
+    double v = (double)p[m];
+    p[m] = (R)(minus ? v - coef * a[c] : v + coef * a[c]);
+  }
+}
+
+// TF/SF injection (src/fdtd.cpp:307-373).  axis 0: planes i = 1, 2, N0-2, N0-3 over (j, k);
+// axis 1: planes j = 1, 2, N1-2, N1-3 over (i, k); axis 2: planes k = 1, 2, np-2, np-3 over (i, j).
+// The three axes run as three launches in the reference's loop order; one thread applies the
+// four plane cases of its line in the reference's order, so a node touched twice (tiny meshes)
+// receives its contributions in the same sequence.
+template <typename R>
+__global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid g, mithra_seed s, double time,
+                       double coef) {
+  const long N1N0 = (long)N1 * N0;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (axis == 0) {
+    const long nj = N1 - 4, nk = N2 - 4;
+    if (t >= nj * nk) return;
+    const long j = 2 + t % nj, k = 2 + t / nj;
+    const double y = g.ymin + j * g.dy, z = g.zmin + k * g.dz;
+    const long base = N1N0 * k + j;
+    tfsf_apply<R>(A, base + N1 * 1, g.xmin + 2 * g.dx, y, z, time, s, coef, true);
+    tfsf_apply<R>(A, base + N1 * 2, g.xmin + 1 * g.dx, y, z, time, s, coef, false);
+    tfsf_apply<R>(A, base + (long)N1 * (N0 - 2), g.xmin + (N0 - 3) * g.dx, y, z, time, s, coef, true);
+    tfsf_apply<R>(A, base + (long)N1 * (N0 - 3), g.xmin + (N0 - 2) * g.dx, y, z, time, s, coef, false);
+  } else if (axis == 1) {
+    const long ni = N0 - 4, nk = N2 - 4;
+    if (t >= ni * nk) return;
+    const long i = 2 + t % ni, k = 2 + t / ni;
+    const double x = g.xmin + i * g.dx, z = g.zmin + k * g.dz;
+    const long base = N1N0 * k + (long)N1 * i;
+    tfsf_apply<R>(A, base + 1, x, g.ymin + 2 * g.dy, z, time, s, coef, true);
+    tfsf_apply<R>(A, base + 2, x, g.ymin + 1 * g.dy, z, time, s, coef, false);
+    tfsf_apply<R>(A, base + N1 - 2, x, g.ymin + (N1 - 3) * g.dy, z, time, s, coef, true);
+    tfsf_apply<R>(A, base + N1 - 3, x, g.ymin + (N1 - 2) * g.dy, z, time, s, coef, false);
+  } else {
+    const long ni = N0 - 4, nj = N1 - 4;
+    if (t >= ni * nj) return;
+    const long j = 2 + t % nj, i = 2 + t / nj;
+    const double x = g.xmin + i * g.dx, y = g.ymin + j * g.dy;
+    const long base = (long)N1 * i + j;
+    tfsf_apply<R>(A, base + N1N0 * 1, x, y, g.zmin + 2 * g.dz, time, s, coef, true);
+    tfsf_apply<R>(A, base + N1N0 * 2, x, y, g.zmin + 1 * g.dz, time, s, coef, false);
+    tfsf_apply<R>(A, base + N1N0 * (N2 - 2), x, y, g.zmin + (N2 - 3) * g.dz, time, s, coef, true);
+    tfsf_apply<R>(A, base + N1N0 * (N2 - 3), x, y, g.zmin + (N2 - 2) * g.dz, time, s, coef, false);
+  }
+}
+
+// ---- Mur absorbing boundaries (src/database.cpp:137-288, loops src/fdtd.cpp:377-725) ------
+// Offsets are in nodes.  For a face node, s is the offset of its inward neighbour and t, u the
+// two tangential axes (B[3] multiplies the t-neighbours, B[4] the u-neighbours).  For an edge
+// node, o1 and o2 are the two inward offsets and w the offset along the edge.  For a corner,
+// sx, sy, sz are the three inward offsets.
+
+template <typename R>
+__device__ inline double rd(const R* p, long m) { return (double)p[m]; }
+
+template <typename R>
+__device__ inline double mur_face(const double* B, const R* P, const R* N, const R* M, long m, long s, long t, long u) {
+  return B[0] * (rd(M, m) + rd(P, m + s)) + B[1] * rd(M, m + s) + B[2] * (rd(N, m) + rd(N, m + s)) +
+         B[3] * (rd(N, m + s + t) + rd(N, m + s - t) + rd(N, m + t) + rd(N, m - t)) +
+         B[4] * (rd(N, m + s + u) + rd(N, m + s - u) + rd(N, m + u) + rd(N, m - u));
+}
+
+template <typename R>
+__device__ inline double mur_edge(const double* B, const R* P, const R* N, const R* M, long m, long o1, long o2,
+                                  long w) {
+  const long o3 = o1 + o2;
+  return B[0] * (rd(P, m + o1) + rd(M, m + o2)) + B[1] * (rd(M, m + o1) + rd(P, m + o2)) +
+         B[2] * (rd(M, m) + rd(P, m + o3)) + B[3] * (rd(N, m) + rd(N, m + o1) + rd(N, m + o2) + rd(N, m + o3)) +
+         B[4] * (rd(N, m - w) + rd(N, m - w + o1) + rd(N, m - w + o2) + rd(N, m - w + o3) + rd(N, m + w) +
+                 rd(N, m + w + o1) + rd(N, m + w + o2) + rd(N, m + w + o3)) -
+         rd(M, m + o3);
+}
+
+template <typename R>
+__device__ inline double mur_corner(const double* h, const R* P, const R* N, const R* M, long m, long sx, long sy,
+                                    long sz) {
+  const long o[7] = {sx, sy, sz, sx + sy, sz + sx, sz + sy, sz + sx + sy};
+  double s = rd(N, m) * h[16] + rd(M, m) * h[8];
+  for (int t = 0; t < 7; ++t) s = s + rd(P, m + o[t]) * h[1 + t] + rd(N, m + o[t]) * h[16] + rd(M, m + o[t]) * h[9 + t];
+  return -s / h[0];
+}
+
+struct MurCoeffs {
+  double bB[5], cB[5], dB[5], eE[5], fE[5], gE[5], hC[17];
+};
+
+// All six faces in one launch: x faces (j, k), y faces (i, k), z faces (i, j), each min/max.
+template <typename R>
+__global__ void k_mur_faces(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+  const long N1N0 = (long)N1 * N0;
+  const long X = N1, Y = 1, Z = N1N0;
+  const long nx = (long)(N1 - 2) * (N2 - 2), ny = (long)(N0 - 2) * (N2 - 2), nz = (long)(N0 - 2) * (N1 - 2);
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int comp = blockIdx.y;
+  if (comp >= ncomp) return;
+  const R* pn = (const R*)N.c[comp];
+  const R* pm = (const R*)M.c[comp];
+  R* pp = (R*)P.c[comp];
+  long m, s, tt, u;
+  const double* B;
+  if (t < 2 * nx) {
+    const bool hi = t >= nx;
+    t -= hi ? nx : 0;
+    const long j = 1 + t % (N1 - 2), k = 1 + t / (N1 - 2);
+    m = N1N0 * k + (hi ? N1N0 - N1 : 0) + j;
+    s = hi ? -X : X; tt = Y; u = Z; B = c.bB;
+  } else if ((t -= 2 * nx) < 2 * ny) {
+    const bool hi = t >= ny;
+    t -= hi ? ny : 0;
+    const long i = 1 + t % (N0 - 2), k = 1 + t / (N0 - 2);
+    m = N1N0 * k + (long)N1 * i + (hi ? N1 - 1 : 0);
+    s = hi ? -Y : Y; tt = X; u = Z; B = c.cB;
+  } else if ((t -= 2 * ny) < 2 * nz) {
+    const bool hi = t >= nz;
+    t -= hi ? nz : 0;
+    const long j = 1 + t % (N1 - 2), i = 1 + t / (N1 - 2);
+    m = (hi ? N1N0 * (N2 - 1) : 0) + (long)N1 * i + j;
+    s = hi ? -Z : Z; tt = X; u = Y; B = c.dB;
+  } else {
+    return;
+  }
+  pp[m] = (R)mur_face(B, pp, pn, pm, m, s, tt, u);
+}
+
+// The twelve edges (second-order Mur only).
+template <typename R>
+__global__ void k_mur_edges(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+  const long N1N0 = (long)N1 * N0;
+  const long X = N1, Y = 1, Z = N1N0;
+  const long ez = N2 - 2, ex = N0 - 2, ey = N1 - 2;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int comp = blockIdx.y;
+  if (comp >= ncomp) return;
+  const R* pn = (const R*)N.c[comp];
+  const R* pm = (const R*)M.c[comp];
+  R* pp = (R*)P.c[comp];
+  long m, o1, o2, w;
+  const double* B;
+  if (t < 4 * ez) {  // along z at the four (x, y) corners
+    const int which = t / ez;
+    const long k = 1 + t % ez;
+    const bool xh = which & 1, yh = which & 2;
+    m = N1N0 * k + (xh ? (long)N1 * (N0 - 1) : 0) + (yh ? N1 - 1 : 0);
+    o1 = xh ? -X : X; o2 = yh ? -Y : Y; w = Z; B = c.eE;
+  } else if ((t -= 4 * ez) < 4 * ex) {  // along x at the four (y, z) corners
+    const int which = t / ex;
+    const long i = 1 + t % ex;
+    const bool yh = which & 1, zh = which & 2;
+    m = (long)N1 * i + (yh ? N1 - 1 : 0) + (zh ? N1N0 * (N2 - 1) : 0);
+    o1 = yh ? -Y : Y; o2 = zh ? -Z : Z; w = X; B = c.fE;
+  } else if ((t -= 4 * ex) < 4 * ey) {  // along y at the four (z, x) corners
+    const int which = t / ey;
+    const long j = 1 + t % ey;
+    const bool zh = which & 1, xh = which & 2;
+    m = j + (zh ? N1N0 * (N2 - 1) : 0) + (xh ? N1N0 - N1 : 0);
+    o1 = zh ? -Z : Z; o2 = xh ? -X : X; w = Y; B = c.gE;
+  } else {
+    return;
+  }
+  pp[m] = (R)mur_edge(B, pp, pn, pm, m, o1, o2, w);
+}
+
+template <typename R>
+__global__ void k_mur_corners(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+  const int t = threadIdx.x;
+  const int comp = blockIdx.y;
+  if (t >= 8 || comp >= ncomp) return;
+  const long N1N0 = (long)N1 * N0;
+  const bool xh = t & 1, yh = t & 2, zh = t & 4;
+  const long m = (xh ? N1N0 - N1 : 0) + (yh ? N1 - 1 : 0) + (zh ? N1N0 * (N2 - 1) : 0);
+  const R* pn = (const R*)N.c[comp];
+  const R* pm = (const R*)M.c[comp];
+  R* pp = (R*)P.c[comp];
+  pp[m] = (R)mur_corner(c.hC, pp, pn, pm, m, xh ? -(long)N1 : N1, yh ? -1L : 1L, zh ? -N1N0 : N1N0);
+}
+
+// E/B on every interior node (FdTd::fieldEvaluate src/fdtd.cpp:818-845, FdTdSC::fieldEvaluate
+// src/fdtdSC.cpp:1110-1145); planes k = 0 and N2-1 take copies of k = 1 and N2-2
+// (src/fdtd.cpp:742-774).  E_x and E_y are one double expression rounded once to float and E_z
+// keeps the float rounding after each term: that is what the reference computes as compiled by
+// g++ -O3 (its vectorised x/y lanes drop the intermediate float round trip of
+// FieldVector<float>::dv/mdv, src/fieldvector.h:87-108), and the parity fixtures record it.
+template <typename R, bool SC>
+__global__ __launch_bounds__(256) void k_eval_eb(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1,
+                                                 int N2, double dt, double dx2, double dy2, double dz2) {
+  const int j = blockIdx.x * 64 + (threadIdx.x % 64);
+  const int i = blockIdx.y * 4 + (threadIdx.x / 64);
+  const int k = blockIdx.z;
+  if (j < 1 || j > N1 - 2 || i < 1 || i > N0 - 2 || k < 1 || k > N2 - 2) return;
+  const long N1N0 = (long)N1 * N0;
+  const long m = N1N0 * k + (long)N1 * i + j;
+  const R* n0 = (const R*)An.c[0];
+  const R* n1 = (const R*)An.c[1];
+  const R* n2 = (const R*)An.c[2];
+  const R* p0 = (const R*)Ap.c[0];
+  const R* p1 = (const R*)Ap.c[1];
+  const R* p2 = (const R*)Ap.c[2];
+  const double mdt = -dt;
+  double ex = rd(p0, m) / mdt - rd(n0, m) / mdt;
+  double ey = rd(p1, m) / mdt - rd(n1, m) / mdt;
+  float ez = (float)(rd(p2, m) / mdt);
+  ez = (float)((double)ez - rd(n2, m) / mdt);
+  if (SC) {
+    const R* f = (const R*)An.c[3];
+    ex = ex - (rd(f, m + N1) - rd(f, m - N1)) / dx2;
+    ey = ey - (rd(f, m + 1) - rd(f, m - 1)) / dy2;
+    ez = (float)((double)ez - (rd(f, m + N1N0) - rd(f, m - N1N0)) / dz2);
+  }
+  const float bx = (float)(0.5 * ((rd(n2, m + 1) - rd(n2, m - 1)) / dy2 - (rd(n1, m + N1N0) - rd(n1, m - N1N0)) / dz2 +
+                                  (rd(p2, m + 1) - rd(p2, m - 1)) / dy2 - (rd(p1, m + N1N0) - rd(p1, m - N1N0)) / dz2));
+  const float by = (float)(0.5 * ((rd(n0, m + N1N0) - rd(n0, m - N1N0)) / dz2 - (rd(n2, m + N1) - rd(n2, m - N1)) / dx2 +
+                                  (rd(p0, m + N1N0) - rd(p0, m - N1N0)) / dz2 - (rd(p2, m + N1) - rd(p2, m - N1)) / dx2));
+  const float bz = (float)(0.5 * ((rd(n1, m + N1) - rd(n1, m - N1)) / dx2 - (rd(n0, m + 1) - rd(n0, m - 1)) / dy2 +
+                                  (rd(p1, m + N1) - rd(p1, m - N1)) / dx2 - (rd(p0, m + 1) - rd(p0, m - 1)) / dy2));
+  const float4 v0 = make_float4((float)ex, (float)ey, ez, bx);
+  const float4 v1 = make_float4(by, bz, 0.0f, 0.0f);
+  eb[2 * m] = v0;
+  eb[2 * m + 1] = v1;
+  if (k == 1) {
+    eb[2 * (m - N1N0)] = v0;
+    eb[2 * (m - N1N0) + 1] = v1;
+  }
+  if (k == N2 - 2) {
+    eb[2 * (m + N1N0)] = v0;
+    eb[2 * (m + N1N0) + 1] = v1;
+  }
+}
+
+__global__ void k_fill_zero(uint4* p, size_t n16) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n16; t += (size_t)gridDim.x * blockDim.x)
+    p[t] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void k_fill_zero_bytes(unsigned char* p, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0;
+}
+
+template <typename R>
+hipError_t interior_impl(Domain* d) {
+  const mithra_grid& g = d->g;
+  const int tilesJ = (g.N1 + kTileJ - 1) / kTileJ;
+  const int tilesI = (g.N0 - 2 + kTileI - 1) / kTileI;
+  const int chunks = (g.N2 - 2 + kChunkK - 1) / kChunkK;
+  const long nblk = (long)tilesJ * tilesI * chunks * d->ncomp;
+  const double* a = d->co.a;
+  if (g.solver == MITHRA_SOLVER_NSFD)
+    hipLaunchKernelGGL((k_interior<R, true>), dim3((unsigned)nblk), dim3(kTileJ * kTileI), 0, d->stream,
+                       d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
+                       chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
+  else
+    hipLaunchKernelGGL((k_interior<R, false>), dim3((unsigned)nblk), dim3(kTileJ * kTileI), 0, d->stream,
+                       d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
+                       chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t tfsf_impl(Domain* d, double time, const mithra_seed* seed) {
+  const mithra_grid& g = d->g;
+  const long n[3] = {(long)(g.N1 - 4) * (g.N2 - 4), (long)(g.N0 - 4) * (g.N2 - 4), (long)(g.N0 - 4) * (g.N1 - 4)};
+  const double coef[3] = {d->co.a[1], d->co.a[2], d->co.a[3]};
+  for (int axis = 0; axis < 3; ++axis) {
+    if (n[axis] <= 0) continue;
+    hipLaunchKernelGGL((k_tfsf<R>), dim3((unsigned)((n[axis] + 255) / 256)), dim3(256), 0, d->stream,
+                       d->lvl[d->i_np1], axis, g.N0, g.N1, g.N2, g, *seed, time, coef[axis]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <typename R>
+hipError_t boundaries_impl(Domain* d) {
+  const mithra_grid& g = d->g;
+  MurCoeffs c;
+  for (int t = 0; t < 5; ++t) {
+    c.bB[t] = d->co.bB[t]; c.cB[t] = d->co.cB[t]; c.dB[t] = d->co.dB[t];
+    c.eE[t] = d->co.eE[t]; c.fE[t] = d->co.fE[t]; c.gE[t] = d->co.gE[t];
+  }
+  for (int t = 0; t < 17; ++t) c.hC[t] = d->co.hC[t];
+  FieldSet P = d->lvl[d->i_np1], N = d->lvl[d->i_n], M = d->lvl[d->i_nm1];
+  const long nf = 2L * ((long)(g.N1 - 2) * (g.N2 - 2) + (long)(g.N0 - 2) * (g.N2 - 2) + (long)(g.N0 - 2) * (g.N1 - 2));
+  hipLaunchKernelGGL((k_mur_faces<R>), dim3((unsigned)((nf + 255) / 256), d->ncomp), dim3(256), 0, d->stream, P, N, M,
+                     d->ncomp, g.N0, g.N1, g.N2, c);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || g.truncation_order != 2) return e;
+  const long ne = 4L * ((g.N2 - 2) + (g.N0 - 2) + (g.N1 - 2));
+  hipLaunchKernelGGL((k_mur_edges<R>), dim3((unsigned)((ne + 255) / 256), d->ncomp), dim3(256), 0, d->stream, P, N, M,
+                     d->ncomp, g.N0, g.N1, g.N2, c);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((k_mur_corners<R>), dim3(1, d->ncomp), dim3(64), 0, d->stream, P, N, M, d->ncomp, g.N0, g.N1,
+                     g.N2, c);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t eval_impl(Domain* d) {
+  const mithra_grid& g = d->g;
+  dim3 grid((g.N1 + 63) / 64, (g.N0 + 3) / 4, g.N2);
+  const double dx2 = 2.0 * g.dx, dy2 = 2.0 * g.dy, dz2 = 2.0 * g.dz;
+  if (g.space_charge)
+    hipLaunchKernelGGL((k_eval_eb<R, true>), grid, dim3(256), 0, d->stream, d->lvl[d->i_n], d->lvl[d->i_np1], d->eb,
+                       g.N0, g.N1, g.N2, g.dt, dx2, dy2, dz2);
+  else
+    hipLaunchKernelGGL((k_eval_eb<R, false>), grid, dim3(256), 0, d->stream, d->lvl[d->i_n], d->lvl[d->i_np1], d->eb,
+                       g.N0, g.N1, g.N2, g.dt, dx2, dy2, dz2);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_interior(Domain* d) {
+  return d->precision == MITHRA_PRECISION_FP64 ? interior_impl<double>(d) : interior_impl<float>(d);
+}
+
+hipError_t launch_tfsf(Domain* d, double time, const mithra_seed* seed) {
+  return d->precision == MITHRA_PRECISION_FP64 ? tfsf_impl<double>(d, time, seed) : tfsf_impl<float>(d, time, seed);
+}
+
+hipError_t launch_boundaries(Domain* d) {
+  return d->precision == MITHRA_PRECISION_FP64 ? boundaries_impl<double>(d) : boundaries_impl<float>(d);
+}
+
+hipError_t launch_eval_eb(Domain* d) {
+  return d->precision == MITHRA_PRECISION_FP64 ? eval_impl<double>(d) : eval_impl<float>(d);
+}
+
+hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes) {
+  // A fill kernel rather than hipMemsetAsync, so the step stays safe to capture in a graph.
+  const size_t n16 = bytes / 16, tail = bytes % 16;
+  if (n16) {
+    unsigned blocks = (unsigned)((n16 + 255) / 256);
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, d->stream, (uint4*)ptr, n16);
+  }
+  if (tail)
+    hipLaunchKernelGGL(k_fill_zero_bytes, dim3(1), dim3(64), 0, d->stream, (unsigned char*)ptr + n16 * 16, tail);
+  return hipGetLastError();
+}
+
+}  // namespace mithra
