@@ -1,0 +1,306 @@
+"""Benchmark: MITHRA FDTD/PIC time march, production single instance, FP64 drop-in, one MI355X.
+
+Workload (production single-instance configuration): 256 x 256 x 2048 nodes in the boosted frame,
+16,777,216 macro-particles of a 1 GeV beam (gamma 1957.9, 0.01 % energy spread, sigma_r 50 um)
+in a planar undulator (lambda_u 1.8 cm, K 1.4, 300 periods), NSFD solver with second-order Mur
+boundaries, no space charge.  As in the reference's own LCLS-scale example
+(prj/FEL-LCLS/job-files/FEL-LCLS.job) the mesh resolves the radiation (dz = 0.2 nm lab,
+lambda_r / 23) and holds a slice of the flat-top bunch: 0.28 um of it, loaded by a seeded
+quiet start (four beamlets lambda_r / 4 apart, shot-noise modulation per lambda_r bucket),
+3 kA.  Particles are synthetic; the set-up runs through the drop-in host (job-file parser,
+Lorentz boost, coefficients) exactly as `mithra job.job` does.
+
+One step = one iteration of Solver::solve()'s main loop on the GPU: field update (interior,
+Mur, E/B), bunch push, P(z) sample, shift, current reset, deposition.
+
+Prints ONE JSON line.  value = cell-updates/s (nodes x steps / s); particle-pushes/s and the
+roofline of the dominant kernel (the interior stencil) are reported beside it, with the CPU
+oracle timed on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mithra_amd import abi  # noqa: E402
+from mithra_amd.job import Job  # noqa: E402
+
+METRIC = "cell-updates/s + particle-pushes/s; FEL scan points/hour at ≤2% P(z) error"
+HBM_PEAK_GBS = 8000.0
+N_PARTICLES = 16_777_216
+GAMMA = 1957.9
+SLICE_UM = 0.28          # lab-frame length of the simulated slice of the flat-top
+CURRENT_A = 3000.0
+
+JOB = """MESH
+{{
+  length-scale            = MICROMETER
+  time-scale              = PICOSECOND
+  mesh-lengths            = ( 305.4, 305.4, 0.4093 )
+  mesh-resolution         = ( 1.2, 1.2, 2.0e-4 )
+  mesh-center             = ( 0.0, 0.0, 0.0 )
+  total-time              = 1.0e6
+  mesh-truncation-order   = 2
+  space-charge            = false
+  solver                  = NSFD
+}}
+BUNCH
+{{
+  bunch-initialization
+  {{
+    type                  = other
+    charge                = {charge}
+    number-of-particles   = {n}
+    gamma                 = {gamma}
+    direction             = ( 0.0, 0.0, 1.0 )
+  }}
+}}
+UNDULATOR
+{{
+  static-undulator
+  {{
+    undulator-parameter   = 1.4
+    period                = 1.8e4
+    length                = 300
+    polarization-angle    = 0.0
+  }}
+}}
+FEL-OUTPUT
+{{
+  radiation-power
+  {{
+    sample                = true
+    type                  = at-point
+    directory             = {dir}/
+    base-name             = power-sampling/power
+    plane-position        = 0.1977
+    normalized-frequency  = 1.00
+  }}
+}}
+"""
+
+
+def quiet_start_bunch(n, seed=2):
+    """Lab-frame records q x y z gbx gby gbz (lengths in um) of the flat-top slice."""
+    rng = np.random.default_rng(seed)
+    k = 1.4
+    lam = 1.8e4 / (2.0 * GAMMA ** 2) * (1.0 + k * k / 2.0)     # resonant wavelength, um
+    beta_rel = 0.33                                           # bunch speed in the frame before the undulator
+    z0 = beta_rel * SLICE_UM / 2.0                            # centres the slice after the frame shift
+    nb = n // 4
+    zb = z0 + rng.uniform(-SLICE_UM / 2.0, SLICE_UM / 2.0, nb)
+    x = rng.normal(0.0, 50.0, nb)
+    y = rng.normal(0.0, 50.0, nb)
+    gbx = rng.normal(0.0, 0.02, nb)
+    gby = rng.normal(0.0, 0.02, nb)
+    gbz = np.sqrt(GAMMA ** 2 - 1.0) * (1.0 + 1.0e-4 * rng.normal(0.0, 1.0, nb))
+    electrons = CURRENT_A * SLICE_UM * 1e-6 / 2.99792458e8 / 1.602e-19
+    n_bucket = int(np.ceil(SLICE_UM / lam)) + 2
+    ne = electrons * lam / SLICE_UM
+    amp = np.sqrt(-2.0 * np.log(rng.uniform(1e-12, 1.0, n_bucket))) / np.sqrt(ne)
+    phase = 2.0 * np.pi * rng.uniform(0.0, 1.0, n_bucket)
+    bucket = ((zb - zb.min()) / lam).astype(np.int64)
+    out = np.empty((4 * nb, 7))
+    for ii in range(4):                                       # four beamlets lambda / 4 apart
+        z = zb - lam / 4.0 * ii
+        z = z - lam / np.pi * amp[bucket] * np.sin(2.0 * np.pi / lam * z + phase[bucket])
+        s = slice(ii, 4 * nb, 4)
+        out[s, 0] = electrons / (4 * nb)
+        out[s, 1], out[s, 2], out[s, 3] = x, y, z
+        out[s, 4], out[s, 5], out[s, 6] = gbx, gby, gbz
+    return out, electrons
+
+
+def set_up(workdir):
+    bunch, electrons = quiet_start_bunch(N_PARTICLES)
+    path = os.path.join(workdir, "production.job")
+    with open(path, "w") as f:
+        f.write(JOB.format(charge=electrons, n=N_PARTICLES, gamma=GAMMA, dir=workdir))
+    job = Job(path, external_bunch=bunch)
+    del bunch
+    return job
+
+
+def open_domain(job, precision, lib=None, prefix="mithra_dev_", rows=64):
+    g, c = job.grid(), job.coeffs()
+    if lib is None:
+        dom = abi.Domain(g, c, precision=precision)
+    else:
+        dom = abi.Domain(g, c, lib=lib, prefix=prefix)
+    an, fn = job.potentials(0)
+    am, fm = job.potentials(1)
+    dom.set_potentials(0, an, fn)
+    dom.set_potentials(1, am, fm)
+    dom.set_particles(job.particles())
+    N, Nl, Nf, pc, z, w = job.power()
+    t = job.times()
+    bp = job.bunch_params()
+    dom.power_setup(abi.make_power_params(z, w, Nf, pc, bp.gamma, bp.beta, bp.c0), rows)
+    return dom
+
+
+class March:
+    """Host time bookkeeping of Solver::solve() (time_ and timeBunch_ accumulate)."""
+
+    def __init__(self, job):
+        self.t = job.times()
+        self.seed, self.bp, self.und = job.seed(), job.bunch_params(), job.undulators()
+        self.time, self.tb, self.n = self.t["time"], self.t["time_bunch"], 0
+
+    def advance(self):
+        for _ in range(self.t["n_sub"]):
+            self.tb += self.t["bunch_time_step"]
+        self.time += self.t["dt"]
+        self.n += 1
+
+
+def run_gpu(job, steps, warmup, precision):
+    dom = open_domain(job, precision)
+    m = March(job)
+    n_sub, dtb = m.t["n_sub"], m.t["bunch_time_step"]
+    rows = 64
+    for _ in range(warmup):
+        dom.step(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows)
+        m.advance()
+    dom.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dom.step(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows)
+        m.advance()
+    dom.synchronize()
+    elapsed = time.perf_counter() - t0
+    # per-phase device times, HIP events on the domain's stream, over further real steps
+    phases = np.zeros(len(abi.PHASES))
+    n_timed = max(3, min(steps, 10))
+    for _ in range(n_timed):
+        phases += np.array(dom.step_timed(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows))
+        m.advance()
+    phases /= n_timed
+    counters = dom.bunch_counters()
+    P = dom.power_fetch(0, min(rows, warmup + steps), 1, 1)[:, 0, 0]
+    dom.close()
+    return elapsed, phases, counters, P
+
+
+def run_cpu_baseline(job, steps, threads):
+    """The CPU oracle (C++ -O3, OpenMP over nodes/particles, sequential deposition) on a bounded
+    sample of the same workload: `steps` full steps of the same mesh and bunch."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_state import load_oracle
+
+    lib = load_oracle()
+    import ctypes as C
+    lib.oracle_set_threads.restype, lib.oracle_set_threads.argtypes = C.c_int, [C.c_int]
+    used = lib.oracle_set_threads(threads)
+    dom = open_domain(job, abi.PRECISION_FP64, lib=lib, prefix="oracle_", rows=max(steps, 1))
+    m = March(job)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dom.field_update(m.time, m.seed)
+        dom.bunch_begin()
+        tb = m.tb
+        for _ in range(m.t["n_sub"]):
+            dom.bunch_update(m.bp, m.und, tb)
+            tb += m.t["bunch_time_step"]
+        dom.power_sample(m.n, m.n)
+        dom.field_shift()
+        dom.current_reset()
+        dom.current_update()
+        m.advance()
+    elapsed = time.perf_counter() - t0
+    dom.close()
+    return elapsed, used
+
+
+def traffic_from_profiles(precision):
+    """HBM bytes per launch of the interior stencil from the committed rocprofv3 --pmc
+    summary of this workload, when present."""
+    path = os.path.join(ROOT, "profiles", f"traffic_interior_production_{precision}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0 skips it)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the z-slab multi-GPU "
+              "decomposition is a design, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    precision = abi.PRECISION_FP64 if args.precision == "fp64" else abi.PRECISION_FP32_BULK
+    with tempfile.TemporaryDirectory(prefix="mithra_bench_") as work:
+        t_setup = time.perf_counter()
+        job = set_up(work)
+        t_setup = time.perf_counter() - t_setup
+        g = job.grid()
+        nodes = g.N0 * g.N1 * g.N2
+        interior = (g.N0 - 2) * (g.N1 - 2) * (g.N2 - 2)
+        n_part = job.particles().shape[0]
+        n_sub = job.times()["n_sub"]
+        elapsed, phases, counters, P = run_gpu(job, args.steps, args.warmup, precision)
+        ms_step = 1e3 * elapsed / args.steps
+        cell_rate = nodes * args.steps / elapsed
+        push_rate = n_part * n_sub * args.steps / elapsed
+        bytes_per_node = 96 if precision == abi.PRECISION_FP64 else 48
+        t_int = phases[abi.PHASES.index("interior")] * 1e-3
+        achieved = bytes_per_node * interior / t_int / 1e9
+        copy_gbs = abi.copy_bandwidth()
+        cpu = None
+        if args.cpu_steps > 0:
+            cpu_t, used = run_cpu_baseline(job, args.cpu_steps, args.cpu_threads)
+            cpu = {"value": nodes * args.cpu_steps / cpu_t, "unit": "cell-updates/s", "cores": used,
+                   "kind": "port",
+                   "sample": f"{args.cpu_steps} full steps of the same 256x256x2048 mesh and 16.8M-particle "
+                             f"bunch with the CPU oracle (C++ -O3, OpenMP over nodes and particles, "
+                             f"sequential deposition), {cpu_t:.1f} s",
+                   "particle_pushes_per_s": n_part * n_sub * args.cpu_steps / cpu_t}
+    out = {
+        "metric": METRIC,
+        "value": cell_rate,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == abi.PRECISION_FP64 else "f32 fields, f64 particles",
+        "data": "synthetic: seeded quiet-start bunch, random-free job set-up through the drop-in host",
+        "config": {"workload": "production single instance (256x256x2048 nodes, 16.8M macro-particles, "
+                               "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)",
+                   "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
+                   "precision": args.precision, "parallelism": "single GPU, whole domain resident"},
+        "particle_pushes_per_s": push_rate,
+        "scan_points_per_hour": None,
+        "phase_ms": {k: round(float(v), 4) for k, v in zip(abi.PHASES, phases)},
+        "roofline": {"bound": "hbm", "kernel": "k_interior (NSFD stencil)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "frac_of_measured_copy": achieved / copy_gbs, "measured_copy_GBps": copy_gbs,
+                     "traffic": traffic_from_profiles(args.precision),
+                     "algorithmic_bytes_per_launch": bytes_per_node * interior},
+        "cpu_baseline": cpu,
+        "setup_s": t_setup,
+        "bunch_counters": {"transverse_escapes": counters[0], "z_crossings": counters[1]},
+        "power_last": float(P[-1]) if len(P) else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -158,6 +158,11 @@ long mithra_dev_particle_count(const mithra_dev* dev);
  * (src/fdtd.cpp:818-845, src/fdtdSC.cpp:1110-1145).  time = Solver::time_. */
 int mithra_dev_field_update(mithra_dev* dev, double time, const mithra_seed* seed);
 
+/* E/B from the potentials as they stand, without advancing them: what the reference's lazy
+ * FdTd::fieldEvaluate (src/fdtd.cpp:818-845) returns while fieldUpdate is not running, i.e.
+ * during the particle-only march before time zero (src/solver.cpp:1232-1291). */
+int mithra_dev_field_evaluate(mithra_dev* dev);
+
 /* rnm = rnp for every particle (src/solver.cpp:1311-1312). */
 int mithra_dev_bunch_begin(mithra_dev* dev);
 
@@ -209,6 +214,20 @@ int mithra_dev_current_update(mithra_dev* dev);
 int mithra_dev_step(mithra_dev* dev, double time, const mithra_seed* seed, const mithra_bunch_params* p,
                     const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
                     int n_sub, unsigned n_time, int power_row);
+
+/* mithra_dev_step with HIP events recorded on the handle's stream between its phases; ms[i]
+ * receives the duration of phase i: 0 interior update, 1 TF/SF, 2 Mur boundaries, 3 E/B
+ * evaluation, 4 bunch sub-steps, 5 power sample, 6 current reset, 7 current deposition.
+ * Waits for the step to finish.  No counterpart in the reference: the measurement hook. */
+#define MITHRA_STEP_PHASES 8
+int mithra_dev_step_timed(mithra_dev* dev, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+                          const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
+                          int n_sub, unsigned n_time, int power_row, float* ms);
+
+/* Device-to-device copy rate of a plain streaming kernel over two buffers of `bytes` each
+ * (read + write bytes per second, GB/s), the measured HBM ceiling the roofline fractions are
+ * also reported against.  No counterpart in the reference. */
+int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps);
 
 #ifdef __cplusplus
 }

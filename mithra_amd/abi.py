@@ -72,6 +72,21 @@ class MithraError(RuntimeError):
     pass
 
 
+PHASES = ("interior", "tfsf", "boundaries", "eval_eb", "push", "power", "current_reset", "deposit")
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Measured device copy rate (GB/s, read + write) of a plain streaming kernel."""
+    lib = load_gpu_library()
+    f = lib.mithra_copy_bandwidth
+    f.restype, f.argtypes = C.c_int, [C.c_ulong, C.c_int, C.POINTER(C.c_double)]
+    out = C.c_double()
+    rc = f(nbytes, reps, C.byref(out))
+    if rc != 0:
+        raise MithraError(f"mithra_copy_bandwidth: status {rc}")
+    return out.value
+
+
 def _declare(lib, prefix):
     vp = C.c_void_p
     sig = {
@@ -99,6 +114,9 @@ def _declare(lib, prefix):
         sig["bunch_advance"] = (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
                                           C.c_double, C.c_double, C.c_int])
         sig["synchronize"] = (C.c_int, [vp])
+        sig["step_timed"] = (C.c_int, [vp, C.c_double, C.POINTER(Seed), C.POINTER(BunchParams),
+                                       C.POINTER(Undulator), C.c_int, C.c_double, C.c_double,
+                                       C.c_int, C.c_uint, C.c_int, C.POINTER(C.c_float)])
         sig["step"] = (C.c_int, [vp, C.c_double, C.POINTER(Seed), C.POINTER(BunchParams),
                                  C.POINTER(Undulator), C.c_int, C.c_double, C.c_double, C.c_int,
                                  C.c_uint, C.c_int])
@@ -221,6 +239,16 @@ class Domain:
         self._check(self.f["step"](self.h, time, None if seed is None else C.byref(seed),
                                    C.byref(params), arr, len(undulators), time_bunch,
                                    bunch_time_step, n_sub, n_time, power_row))
+
+    def step_timed(self, time, seed, params, undulators, time_bunch, bunch_time_step, n_sub, n_time,
+                   power_row=-1):
+        """step() with per-phase device times in ms (PHASES order)."""
+        arr = (Undulator * max(1, len(undulators)))(*undulators)
+        ms = (C.c_float * 8)()
+        self._check(self.f["step_timed"](self.h, time, None if seed is None else C.byref(seed),
+                                         C.byref(params), arr, len(undulators), time_bunch,
+                                         bunch_time_step, n_sub, n_time, power_row, ms))
+        return list(ms)
 
     def synchronize(self):
         self._check(self.f["synchronize"](self.h))

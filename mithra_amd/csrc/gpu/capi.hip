@@ -14,6 +14,8 @@ struct mithra_dev : Domain {
   mithra_undulator* und_dev = nullptr;  // device copy of the undulator table
   int und_cap = 0;
   std::vector<mithra_undulator> und_host;  // what und_dev holds
+  hipEvent_t ev[MITHRA_STEP_PHASES + 1] = {};  // phase timers of mithra_dev_step_timed
+  bool ev_ready = false;
 };
 
 namespace {
@@ -129,6 +131,8 @@ int mithra_dev_destroy(mithra_dev* d) {
   hipFree(d->part);
   hipFree(d->und_dev);
   free_power(d);
+  if (d->ev_ready)
+    for (auto& e : d->ev) hipEventDestroy(e);
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
   return MITHRA_OK;
@@ -250,6 +254,11 @@ int mithra_dev_field_update(mithra_dev* d, double time, const mithra_seed* seed)
   MITHRA_TRY(launch_boundaries(d));
   MITHRA_TRY(launch_eval_eb(d));
   return MITHRA_OK;
+}
+
+int mithra_dev_field_evaluate(mithra_dev* d) {
+  if (!d) return MITHRA_E_ARG;
+  return hip_status(launch_eval_eb(d));
 }
 
 int mithra_dev_bunch_begin(mithra_dev* d) {
@@ -390,6 +399,82 @@ int mithra_dev_step(mithra_dev* d, double time, const mithra_seed* seed, const m
   if (!rc) rc = mithra_dev_current_reset(d);
   if (!rc) rc = mithra_dev_current_update(d);
   return rc;
+}
+
+int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+                          const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step, int n_sub,
+                          unsigned n_time, int power_row, float* ms) {
+  if (!d || !p || !ms) return MITHRA_E_ARG;
+  if (!d->ev_ready) {
+    for (auto& e : d->ev) MITHRA_TRY(hipEventCreate(&e));
+    d->ev_ready = true;
+  }
+  const bool seeded = seed && std::fabs(seed->amplitude) > 1.0e-50;
+  if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
+      seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
+    return MITHRA_E_UNSUPPORTED;
+  int rc = upload_undulators(d, und, n_und);
+  if (rc) return rc;
+  hipStream_t s = d->stream;
+  MITHRA_TRY(hipEventRecord(d->ev[0], s));
+  MITHRA_TRY(launch_interior(d));
+  MITHRA_TRY(hipEventRecord(d->ev[1], s));
+  if (seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4) MITHRA_TRY(launch_tfsf(d, time, seed));
+  MITHRA_TRY(hipEventRecord(d->ev[2], s));
+  MITHRA_TRY(launch_boundaries(d));
+  MITHRA_TRY(hipEventRecord(d->ev[3], s));
+  MITHRA_TRY(launch_eval_eb(d));
+  MITHRA_TRY(hipEventRecord(d->ev[4], s));
+  MITHRA_TRY(launch_push(d, p, d->und_dev, n_und, time_bunch, bunch_time_step, n_sub, 1));
+  MITHRA_TRY(hipEventRecord(d->ev[5], s));
+  if (power_row >= 0) {
+    if (!d->prows || (unsigned)power_row >= d->prows) return MITHRA_E_STATE;
+    MITHRA_TRY(launch_power(d, n_time, (unsigned)power_row));
+  }
+  MITHRA_TRY(hipEventRecord(d->ev[6], s));
+  mithra_dev_field_shift(d);
+  rc = mithra_dev_current_reset(d);
+  if (rc) return rc;
+  MITHRA_TRY(hipEventRecord(d->ev[7], s));
+  MITHRA_TRY(launch_deposit(d));
+  MITHRA_TRY(hipEventRecord(d->ev[8], s));
+  MITHRA_TRY(hipEventSynchronize(d->ev[8]));
+  for (int i = 0; i < MITHRA_STEP_PHASES; ++i) MITHRA_TRY(hipEventElapsedTime(&ms[i], d->ev[i], d->ev[i + 1]));
+  return MITHRA_OK;
+}
+
+namespace {
+__global__ void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) b[t] = a[t];
+}
+}  // namespace
+
+int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps) {
+  if (!gbps || reps < 1 || bytes < 4096) return MITHRA_E_ARG;
+  const size_t n = bytes / sizeof(double4);
+  double4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  if (hipMalloc(&a, n * sizeof(double4)) != hipSuccess || hipMalloc(&b, n * sizeof(double4)) != hipSuccess) {
+    hipFree(a);
+    return MITHRA_E_NOMEM;
+  }
+  MITHRA_TRY(hipMemset(a, 0, n * sizeof(double4)));
+  MITHRA_TRY(hipEventCreate(&e0));
+  MITHRA_TRY(hipEventCreate(&e1));
+  const unsigned blocks = 256 * 32;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);  // warm-up
+  MITHRA_TRY(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  MITHRA_TRY(hipEventRecord(e1, 0));
+  MITHRA_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  MITHRA_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * n * sizeof(double4) * reps / (ms * 1e-3) / 1e9;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return MITHRA_OK;
 }
 
 }  // extern "C"
