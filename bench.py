@@ -118,7 +118,12 @@ def quiet_start_bunch(n, seed=2):
     return out, electrons
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def set_up(workdir):
+    log("generating the quiet-start bunch and running the host set-up")
     bunch, electrons = quiet_start_bunch(N_PARTICLES)
     path = os.path.join(workdir, "production.job")
     with open(path, "w") as f:
@@ -252,7 +257,9 @@ def main():
         interior = (g.N0 - 2) * (g.N1 - 2) * (g.N2 - 2)
         n_part = job.particles().shape[0]
         n_sub = job.times()["n_sub"]
+        log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
         elapsed, phases, counters, P = run_gpu(job, args.steps, args.warmup, precision)
+        log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
         ms_step = 1e3 * elapsed / args.steps
         cell_rate = nodes * args.steps / elapsed
         push_rate = n_part * n_sub * args.steps / elapsed
@@ -262,6 +269,7 @@ def main():
         copy_gbs = abi.copy_bandwidth()
         cpu = None
         if args.cpu_steps > 0:
+            log(f"CPU oracle sample: {args.cpu_steps} steps on {args.cpu_threads} threads")
             cpu_t, used = run_cpu_baseline(job, args.cpu_steps, args.cpu_threads)
             cpu = {"value": nodes * args.cpu_steps / cpu_t, "unit": "cell-updates/s", "cores": used,
                    "kind": "port",
