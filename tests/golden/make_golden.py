@@ -22,28 +22,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, HERE)
+from bunchfile import write_bunch_file  # noqa: E402
 from dumpio import read_dump  # noqa: E402
 
 HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
 N_STEPS = 100
 PRE_KEYS = ("an", "anm1", "jn", "fn", "fnm1", "rn", "particles")
 POST_KEYS = ("anp1", "fnp1", "en", "bn")
-
-
-def write_bunch_file(path, n, seed):
-    """Lab-frame bunch for `type = file` jobs: x y z gbx gby gbz per line, no trailing newline
-    (the reader loops on ifstream::good(), src/classes.cpp:379)."""
-    rng = np.random.default_rng(seed)
-    x = rng.normal(0.0, 100.0, n)
-    y = rng.normal(0.0, 100.0, n)
-    z = rng.uniform(-5.0, 5.0, n)
-    gbx = rng.normal(0.0, 1.0e-3, n)
-    gby = rng.normal(0.0, 1.0e-3, n)
-    gbz = np.sqrt(100.41 ** 2 - 1.0) + rng.normal(0.0, 0.1, n)
-    rows = np.stack([x, y, z, gbx, gby, gbz], axis=1)
-    text = "\n".join(" ".join("%.17g" % v for v in r) for r in rows)
-    with open(path, "w") as f:
-        f.write(text)
 
 
 def run_case(job, steps=N_STEPS):

@@ -20,7 +20,7 @@ def _stage(name, tmp_path):
     shutil.copy(os.path.join(GOLDEN, "jobs", name + ".job"), tmp_path)
     if "file-name" in open(os.path.join(GOLDEN, "jobs", name + ".job")).read():
         sys.path.insert(0, GOLDEN)
-        from make_golden import write_bunch_file
+        from bunchfile import write_bunch_file
         write_bunch_file(os.path.join(tmp_path, "bunch_e.txt"), 1000, seed=5)
 
 

@@ -24,7 +24,7 @@ def setup(request, tmp_path_factory):
     if "file-name" in open(os.path.join(JOBS, name + ".job")).read():
         import sys
         sys.path.insert(0, GOLDEN)
-        from make_golden import write_bunch_file
+        from bunchfile import write_bunch_file
         write_bunch_file(os.path.join(work, "bunch_e.txt"), 1000, seed=5)
     cwd = os.getcwd()
     os.chdir(work)
