@@ -215,6 +215,13 @@ int mithra_dev_step(mithra_dev* dev, double time, const mithra_seed* seed, const
                     const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
                     int n_sub, unsigned n_time, int power_row);
 
+/* Bunching factor b = |sum q exp(i k z)| / |sum q| over all particles at their current boosted-
+ * frame z.  With k = 2 pi / lambda', lambda' = lambda_u / (gamma lambda_norm) the radiation
+ * wavelength in the frame (the one the power sampler uses, src/radiation.cpp:91-92), this is
+ * the FEL bunching of the beam; the phase k z - omega t is Lorentz invariant, so b needs no
+ * lab-frame transform.  No counterpart in the reference (its radiation diagnostics only). */
+int mithra_dev_bunching(mithra_dev* dev, double k, double* b_out);
+
 /* mithra_dev_step with HIP events recorded on the handle's stream between its phases; ms[i]
  * receives the duration of phase i: 0 interior update, 1 TF/SF, 2 Mur boundaries, 3 E/B
  * evaluation, 4 bunch sub-steps, 5 power sample, 6 current reset, 7 current deposition.

@@ -109,6 +109,7 @@ def _declare(lib, prefix):
         "field_shift": (C.c_int, [vp]),
         "current_reset": (C.c_int, [vp]),
         "current_update": (C.c_int, [vp]),
+        "bunching": (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)]),
     }
     if prefix == "mithra_dev_":
         sig["bunch_advance"] = (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
@@ -277,6 +278,12 @@ class Domain:
 
     def current_update(self):
         self._check(self.f["current_update"](self.h))
+
+    def bunching(self, k):
+        """|sum q exp(i k z)| / |sum q| over the particles."""
+        b = C.c_double()
+        self._check(self.f["bunching"](self.h, k, C.byref(b)))
+        return b.value
 
 
 def make_power_params(z, w, Nf, pc, gamma, beta, c0):

@@ -401,6 +401,28 @@ int mithra_dev_step(mithra_dev* d, double time, const mithra_seed* seed, const m
   return rc;
 }
 
+int mithra_dev_bunching(mithra_dev* d, double k, double* b_out) {
+  if (!d || !b_out) return MITHRA_E_ARG;
+  const int blocks = 512;
+  double* partial = nullptr;
+  if (hipMalloc(&partial, sizeof(double) * 3 * blocks) != hipSuccess) return MITHRA_E_NOMEM;
+  MITHRA_TRY(launch_fill_zero(d, partial, sizeof(double) * 3 * blocks));
+  hipError_t e = launch_bunching(d, k, partial, blocks);
+  std::vector<double> h(3 * blocks);
+  if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
+  hipFree(partial);
+  if (e != hipSuccess) return hip_status(e);
+  double c = 0.0, s = 0.0, q = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    c += h[3 * b];
+    s += h[3 * b + 1];
+    q += h[3 * b + 2];
+  }
+  *b_out = q != 0.0 ? std::sqrt(c * c + s * s) / std::fabs(q) : 0.0;
+  return MITHRA_OK;
+}
+
 int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
                           const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step, int n_sub,
                           unsigned n_time, int power_row, float* ms) {

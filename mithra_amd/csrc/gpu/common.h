@@ -72,6 +72,7 @@ hipError_t launch_bunch_begin(Domain* d);
 hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
                        double time_bunch, double bunch_time_step, int n_sub, int save_rnm);
 hipError_t launch_deposit(Domain* d);
+hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks);
 hipError_t launch_power(Domain* d, unsigned n_time, unsigned row);
 
 }  // namespace mithra

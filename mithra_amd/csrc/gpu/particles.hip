@@ -289,7 +289,39 @@ __global__ __launch_bounds__(256) void k_deposit(const double* __restrict__ part
   }
 }
 
+// Bunching factor partial sums: per workgroup sum of q cos(k z), q sin(k z), q over its particles.
+__global__ __launch_bounds__(256) void k_bunching(const double* __restrict__ part, long cap, long n, double k,
+                                                  double* __restrict__ partial) {
+  __shared__ double s[3][256];
+  double c = 0.0, sn = 0.0, q = 0.0;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    const double w = part[PQ * cap + p], z = part[PZ * cap + p];
+    double sv, cv;
+    sincos(k * z, &sv, &cv);
+    c += w * cv;
+    sn += w * sv;
+    q += w;
+  }
+  s[0][threadIdx.x] = c;
+  s[1][threadIdx.x] = sn;
+  s[2][threadIdx.x] = q;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int r = 0; r < 3; ++r) s[r][threadIdx.x] += s[r][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int r = 0; r < 3; ++r) partial[3 * blockIdx.x + r] = s[r][0];
+}
+
 }  // namespace
+
+hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks) {
+  if (d->np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bunching, dim3(blocks), dim3(256), 0, d->stream, d->part, d->cap, d->np, k, partial);
+  return hipGetLastError();
+}
 
 hipError_t launch_bunch_begin(Domain* d) {
   if (d->np == 0) return hipSuccess;

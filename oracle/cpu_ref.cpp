@@ -869,3 +869,17 @@ extern "C" int oracle_power_fetch(oracle_dev* d, unsigned first, unsigned count,
   std::memcpy(out, &d->history[(size_t)first * w], sizeof(double) * w * count);
   return MITHRA_OK;
 }
+
+/* Bunching factor |sum q exp(i k z)| / |sum q| (test infrastructure twin of mithra_dev_bunching). */
+extern "C" int oracle_bunching(oracle_dev* d, double k, double* b) {
+  double c = 0.0, s = 0.0, q = 0.0;
+  const long n = (long)(d->q.size() / 11);
+  for (long p = 0; p < n; ++p) {
+    const double w = d->q[11 * p], z = d->q[11 * p + 3];
+    c += w * std::cos(k * z);
+    s += w * std::sin(k * z);
+    q += w;
+  }
+  *b = q != 0.0 ? std::sqrt(c * c + s * s) / std::fabs(q) : 0.0;
+  return MITHRA_OK;
+}

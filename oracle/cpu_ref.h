@@ -46,6 +46,8 @@ int oracle_field_shift(oracle_dev* dev);
 int oracle_current_reset(oracle_dev* dev);
 int oracle_current_update(oracle_dev* dev);
 
+int oracle_bunching(oracle_dev* dev, double k, double* b_out);
+
 /* Known-answer helpers for single functions. */
 double oracle_halton(unsigned i, unsigned j);
 double oracle_signal(const mithra_seed* seed, double t, double phase);

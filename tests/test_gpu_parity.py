@@ -98,3 +98,16 @@ def test_fused_sub_steps_match_single_steps(case):
         tb += case.dtb
     b.bunch_advance(case.bunch, case.undulators, case.time_bunch0, case.dtb, case.n_sub)
     assert np.array_equal(a.get_particles(), b.get_particles())
+
+
+def test_bunching_factor(case):
+    """Bunching factor of the beam after the march, GPU vs oracle, at a wavelength of ten
+    cells."""
+    gpu = case.open("gpu")
+    ora = case.open("oracle")
+    march(case, gpu, 20)
+    march(case, ora, 20)
+    k = 2.0 * np.pi / (10.0 * case.grid.dz)
+    bg, bo = gpu.bunching(k), ora.bunching(k)
+    print(f"bunching {bg:.6e} vs {bo:.6e}", flush=True)
+    assert abs(bg - bo) <= 1e-9 * max(bo, 1e-300)

@@ -149,3 +149,27 @@ def test_zero_stays_zero(lib):
     a, phi = dom.get_potentials(0)
     e, b = dom.get_fields()
     assert not a.any() and not phi.any() and not e.any() and not b.any()
+
+
+def test_bunching_closed_form(lib):
+    """Particles at z_n = n lambda / M (M equal charges, one period) have bunching 0; all at one
+    phase have bunching 1; a density 1 + a cos(kz) sampled finely has bunching a / 2."""
+    g = small_grid()
+    dom = abi.Domain(g, abi.Coeffs(), lib=lib, prefix="oracle_")
+    lam = 3.0
+    k = 2 * np.pi / lam
+    M = 64
+    rec = np.zeros((M, 11))
+    rec[:, 0] = 1.0
+    rec[:, 3] = np.arange(M) * lam / M
+    dom.set_particles(rec)
+    assert dom.bunching(k) < 1e-14
+    rec[:, 3] = 1.234 + lam * np.arange(M)
+    dom.set_particles(rec)
+    assert abs(dom.bunching(k) - 1.0) < 1e-13
+    z = (np.arange(4096) + 0.5) * lam / 4096
+    rec = np.zeros((4096, 11))
+    rec[:, 0] = 1.0 + 0.3 * np.cos(k * z)
+    rec[:, 3] = z
+    dom.set_particles(rec)
+    assert abs(dom.bunching(k) - 0.15) < 1e-12
