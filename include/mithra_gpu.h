@@ -215,6 +215,14 @@ int mithra_dev_step(mithra_dev* dev, double time, const mithra_seed* seed, const
                     const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
                     int n_sub, unsigned n_time, int power_row);
 
+/* Deposits between two reorders of the particle arrays into the cell-row order of the
+ * deposition (default 16; 0 never reorders).  The deposition sorts its own index lists every
+ * step; reordering the arrays only makes those lists walk memory in sequence.  No per-particle
+ * result changes, and mithra_dev_get_particles always returns the records in the order they
+ * were set.  No counterpart in the reference (its particles live in a std::list,
+ * src/solver.h:271). */
+int mithra_dev_set_reorder_interval(mithra_dev* dev, int interval);
+
 /* Bunching factor b = |sum q exp(i k z)| / |sum q| over all particles at their current boosted-
  * frame z.  With k = 2 pi / lambda', lambda' = lambda_u / (gamma lambda_norm) the radiation
  * wavelength in the frame (the one the power sampler uses, src/radiation.cpp:91-92), this is

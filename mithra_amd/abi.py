@@ -112,6 +112,7 @@ def _declare(lib, prefix):
         "bunching": (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)]),
     }
     if prefix == "mithra_dev_":
+        sig["set_reorder_interval"] = (C.c_int, [vp, C.c_int])
         sig["bunch_advance"] = (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
                                           C.c_double, C.c_double, C.c_int])
         sig["synchronize"] = (C.c_int, [vp])
@@ -278,6 +279,10 @@ class Domain:
 
     def current_update(self):
         self._check(self.f["current_update"](self.h))
+
+    def set_reorder_interval(self, n):
+        """Deposits between reorders of the particle arrays (0: never); device library only."""
+        self._check(self.f["set_reorder_interval"](self.h, n))
 
     def bunching(self, k):
         """|sum q exp(i k z)| / |sum q| over the particles."""

@@ -20,6 +20,11 @@ struct mithra_dev : Domain {
 
 namespace {
 
+// Deposits between two cell sorts of the particles.  In the boosted frame a particle moves
+// well under a cell per step, so the order decays slowly; a sort costs a few deposits' worth
+// of bandwidth on the production mesh.
+constexpr int kDefaultSortInterval = 16;
+
 inline int hip_status(hipError_t e) { return e == hipSuccess ? MITHRA_OK : MITHRA_E_HIP - (int)e; }
 
 #define MITHRA_TRY(expr)                      \
@@ -51,6 +56,16 @@ int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n) {
     MITHRA_TRY(hipMemcpy(d->und_dev, und, sizeof(*und) * n, hipMemcpyHostToDevice));
   }
   d->und_host.assign(und, und + n);
+  return MITHRA_OK;
+}
+
+// current_reset only marks the A^{n+1} buffer as zero: the next deposit writes every node of
+// it anyway.  Anything else that reads or moves the potentials writes the zeros first.
+int settle(mithra_dev* d) {
+  if (!d->zero_pending) return MITHRA_OK;
+  d->zero_pending = 0;
+  for (int c = 0; c < d->ncomp; ++c)
+    MITHRA_TRY(launch_fill_zero(d, d->lvl[d->i_np1].c[c], (size_t)d->nodes * d->real_size));
   return MITHRA_OK;
 }
 
@@ -99,6 +114,7 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
   d->N1N0 = (long)grid->N1 * grid->N0;
   d->nodes = d->N1N0 * grid->N2;
   d->i_np1 = 0; d->i_n = 1; d->i_nm1 = 2;
+  d->sort_interval = kDefaultSortInterval;
   hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete d; return hip_status(e); }
   const size_t per = (size_t)d->nodes * d->real_size;
@@ -129,6 +145,8 @@ int mithra_dev_destroy(mithra_dev* d) {
   hipFree(d->eb);
   hipFree(d->counters);
   hipFree(d->part);
+  hipFree(d->pid);
+  free_sort(d);
   hipFree(d->und_dev);
   free_power(d);
   if (d->ev_ready)
@@ -145,6 +163,7 @@ int mithra_dev_synchronize(mithra_dev* d) {
 
 int mithra_dev_set_potentials(mithra_dev* d, int level, const double* a, const double* phi) {
   if (!d) return MITHRA_E_ARG;
+  if (int rc = settle(d)) return rc;
   const int li = level_of(d, level);
   if (li < 0) return MITHRA_E_ARG;
   MITHRA_TRY(hipStreamSynchronize(d->stream));
@@ -169,6 +188,7 @@ int mithra_dev_set_potentials(mithra_dev* d, int level, const double* a, const d
 
 int mithra_dev_get_potentials(mithra_dev* d, int level, double* a, double* phi) {
   if (!d) return MITHRA_E_ARG;
+  if (int rc = settle(d)) return rc;
   const int li = level_of(d, level);
   if (li < 0) return MITHRA_E_ARG;
   MITHRA_TRY(hipStreamSynchronize(d->stream));
@@ -209,16 +229,30 @@ int mithra_dev_get_fields(mithra_dev* d, float* e, float* b) {
 int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
   if (!d || n < 0 || (n > 0 && !q)) return MITHRA_E_ARG;
   MITHRA_TRY(hipStreamSynchronize(d->stream));
+  if (n >= (1L << 31)) return MITHRA_E_ARG;
   if (n > d->cap) {
     hipFree(d->part);
+    hipFree(d->pid);
+    free_sort(d);
     d->part = nullptr;
+    d->pid = nullptr;
     d->cap = 0;
     const long cap = (n + 63) / 64 * 64;
-    if (hipMalloc(&d->part, sizeof(double) * PCOUNT * (size_t)cap) != hipSuccess) return MITHRA_E_NOMEM;
+    if (hipMalloc(&d->part, sizeof(double) * PCOUNT * (size_t)cap) != hipSuccess ||
+        hipMalloc(&d->pid, sizeof(unsigned) * (size_t)cap) != hipSuccess) {
+      hipFree(d->part);
+      d->part = nullptr;
+      d->pid = nullptr;
+      return MITHRA_E_NOMEM;
+    }
     d->cap = cap;
   }
   d->np = n;
+  d->sort_count = 0;
+  d->rm_order_valid = 0;  // no cell order describes the new particles yet
+  d->order1_is_rnp = 0;
   if (n == 0) return MITHRA_OK;
+  MITHRA_TRY(launch_particle_ids(d));
   // Charge record q, rnp[3], rnm[3], gb[3], e  ->  x y z xm ym zm gx gy gz q e
   static const int from[PCOUNT] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 0, 10};
   std::vector<double> col(n);
@@ -233,10 +267,13 @@ int mithra_dev_get_particles(mithra_dev* d, double* q, long n) {
   if (!d || n != d->np || (n > 0 && !q)) return MITHRA_E_ARG;
   MITHRA_TRY(hipStreamSynchronize(d->stream));
   static const int from[PCOUNT] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 0, 10};
+  if (n == 0) return MITHRA_OK;
+  std::vector<unsigned> id(n);
+  MITHRA_TRY(hipMemcpy(id.data(), d->pid, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
   std::vector<double> col(n);
   for (int a = 0; a < PCOUNT; ++a) {
     MITHRA_TRY(hipMemcpy(col.data(), particle_array(d, a), sizeof(double) * n, hipMemcpyDeviceToHost));
-    for (long p = 0; p < n; ++p) q[11 * p + from[a]] = col[p];
+    for (long p = 0; p < n; ++p) q[11 * (long)id[p] + from[a]] = col[p];  // back to load order
   }
   return MITHRA_OK;
 }
@@ -249,6 +286,7 @@ int mithra_dev_field_update(mithra_dev* d, double time, const mithra_seed* seed)
   if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
       seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
     return MITHRA_E_UNSUPPORTED;
+  if (int rc = settle(d)) return rc;
   MITHRA_TRY(launch_interior(d));
   if (seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4) MITHRA_TRY(launch_tfsf(d, time, seed));
   MITHRA_TRY(launch_boundaries(d));
@@ -258,6 +296,7 @@ int mithra_dev_field_update(mithra_dev* d, double time, const mithra_seed* seed)
 
 int mithra_dev_field_evaluate(mithra_dev* d) {
   if (!d) return MITHRA_E_ARG;
+  if (int rc = settle(d)) return rc;
   return hip_status(launch_eval_eb(d));
 }
 
@@ -370,6 +409,8 @@ int mithra_dev_power_fetch(mithra_dev* d, unsigned first, unsigned count, double
 
 int mithra_dev_field_shift(mithra_dev* d) {
   if (!d) return MITHRA_E_ARG;
+  int rc = settle(d);
+  if (rc) return rc;
   const int t = d->i_nm1;
   d->i_nm1 = d->i_n;
   d->i_n = d->i_np1;
@@ -379,14 +420,24 @@ int mithra_dev_field_shift(mithra_dev* d) {
 
 int mithra_dev_current_reset(mithra_dev* d) {
   if (!d) return MITHRA_E_ARG;
-  for (int c = 0; c < d->ncomp; ++c)
-    MITHRA_TRY(launch_fill_zero(d, d->lvl[d->i_np1].c[c], (size_t)d->nodes * d->real_size));
+  d->zero_pending = 1;
   return MITHRA_OK;
 }
 
 int mithra_dev_current_update(mithra_dev* d) {
   if (!d) return MITHRA_E_ARG;
-  return hip_status(launch_deposit(d));
+  const int accumulate = d->zero_pending ? 0 : 1;
+  d->zero_pending = 0;
+  const hipError_t e = launch_deposit(d, accumulate);
+  if (e == hipErrorOutOfMemory) return MITHRA_E_NOMEM;
+  return hip_status(e);
+}
+
+int mithra_dev_set_reorder_interval(mithra_dev* d, int interval) {
+  if (!d || interval < 0) return MITHRA_E_ARG;
+  d->sort_interval = interval;
+  d->sort_count = 0;
+  return MITHRA_OK;
 }
 
 int mithra_dev_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
@@ -436,6 +487,7 @@ int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, c
       seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
     return MITHRA_E_UNSUPPORTED;
   int rc = upload_undulators(d, und, n_und);
+  if (!rc) rc = settle(d);
   if (rc) return rc;
   hipStream_t s = d->stream;
   MITHRA_TRY(hipEventRecord(d->ev[0], s));
@@ -458,7 +510,8 @@ int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, c
   rc = mithra_dev_current_reset(d);
   if (rc) return rc;
   MITHRA_TRY(hipEventRecord(d->ev[7], s));
-  MITHRA_TRY(launch_deposit(d));
+  rc = mithra_dev_current_update(d);
+  if (rc) return rc;
   MITHRA_TRY(hipEventRecord(d->ev[8], s));
   MITHRA_TRY(hipEventSynchronize(d->ev[8]));
   for (int i = 0; i < MITHRA_STEP_PHASES; ++i) MITHRA_TRY(hipEventElapsedTime(&ms[i], d->ev[i], d->ev[i + 1]));

@@ -29,6 +29,12 @@ struct FieldSet {           // one time level of every potential component
   void* c[4];               // A_x, A_y, A_z, phi (phi only with space charge)
 };
 
+struct CellOrder {          // particle halves sorted by the (k, i) row of their cell
+  unsigned* key;            // row k*N0 + i, or `rows` for a half that deposits nothing
+  unsigned* idx;            // particle index
+  unsigned* row;            // row r starts at row[r] in key/idx; rows + 1 entries
+};
+
 struct Domain {
   mithra_grid g;
   mithra_coeffs co;
@@ -45,6 +51,24 @@ struct Domain {
   long np;                  // particle count
   long cap;                 // allocated capacity
   double* part;             // PCOUNT arrays of cap doubles
+  unsigned* pid;            // load-order index of each particle (the arrays are kept in row order)
+  // cell-row orders of the particle halves (sort.hip); ord[0]: rnp halves of the next deposit,
+  // ord[1]: rnm halves
+  CellOrder ord[2];
+  unsigned* key_alt;        // radix-sort double buffers
+  unsigned* idx_alt;
+  unsigned* sort_block;     // one allocation holding all of the above
+  void* sort_tmp;
+  size_t sort_tmp_bytes;
+  long sort_cap;            // capacity the order buffers were sized for
+  unsigned sort_rows;
+  int rm_order_valid;       // ord[1] lists the current rnm halves
+  int order1_is_rnp;        // ord[1] lists the current rnp (true right after a deposit)
+  double* part_alt;         // target of a reorder, swapped with part
+  unsigned* pid_alt;
+  int zero_pending;         // the A^{n+1} buffer is logically zero (current_reset), not yet written
+  int sort_interval;        // deposits between reorders of the particle arrays (0: never)
+  long sort_count;          // deposits since the particles were set
   unsigned long long* counters;   // [0] transverse escapes, [1] z-boundary crossings
   // power sampling
   unsigned pN, pNz, pNl, pNf, prows;
@@ -71,8 +95,16 @@ hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes);
 hipError_t launch_bunch_begin(Domain* d);
 hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
                        double time_bunch, double bunch_time_step, int n_sub, int save_rnm);
-hipError_t launch_deposit(Domain* d);
+// Current deposition into the A^{n+1} buffer: accumulate = 0 replaces its contents with the
+// deposit, 1 adds the deposit to them.
+hipError_t launch_deposit(Domain* d, int accumulate);
+bool deposit_tiled(const Domain* d);
 hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks);
 hipError_t launch_power(Domain* d, unsigned n_time, unsigned row);
+hipError_t launch_particle_ids(Domain* d);
+void free_sort(Domain* d);
+hipError_t ensure_orders(Domain* d, unsigned rows);
+hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows);
+hipError_t reorder_particles(Domain* d, CellOrder& o);
 
 }  // namespace mithra

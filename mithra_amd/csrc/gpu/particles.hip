@@ -213,79 +213,252 @@ __device__ inline void add(R* p, double v) {
   unsafeAtomicAdd(p, (R)v);
 }
 
-// One particle's half-segment deposit (src/fdtd.cpp:94-137 / :140-183): weights at the
-// midpoint of (end, relay point), J_x at the four y/z node pairs split equally over x, etc.
-template <typename R>
-__device__ inline void deposit_half(const DepositConst& c, R* jx, R* jy, R* jz, long m, double q, const double a[3],
-                                    const double r[3], const double jc[3]) {
+// One particle's half-segment weights (src/fdtd.cpp:94-137 / :140-183), taken at the midpoint of
+// (end, relay point).  J_x is split equally over the two x nodes of each y/z pair, so each
+// component has four distinct values per half segment:
+//   v[0..3]  J_x at the (y, z) pairs (1,1) (2,1) (1,2) (2,2), nodes o[2u] and o[2u+1]
+//   v[4..7]  J_y at the (x, z) pairs, nodes o[t] with (t & 1) + 2 (t >> 2) = v - 4
+//   v[8..11] J_z at the (x, y) pairs, nodes o[t] with t & 3 = v - 8
+// Each value is formed as q 0.5 w_a w_b jc, the reference's operation order.
+__device__ inline void half_weights(const DepositConst& c, double q, const double a[3], const double r[3],
+                                    const double jc[3], double* v) {
   double cc;
   const double fx = modf((0.5 * (a[0] + r[0]) - c.xmin) / c.dx, &cc);
   const double fy = modf((0.5 * (a[1] + r[1]) - c.ymin) / c.dy, &cc);
   const double fz = modf((0.5 * (a[2] + r[2]) - c.zmin) / c.dz, &cc);
   const double x1 = 1.0 - fx, x2 = fx, y1 = 1.0 - fy, y2 = fy, z1 = 1.0 - fz, z2 = fz;
-  const long N1 = c.N1, N1N0 = (long)c.N1 * c.N0;
+  const double h = q * 0.5;
+  v[0] = h * y1 * z1 * jc[0]; v[1] = h * y2 * z1 * jc[0]; v[2] = h * y1 * z2 * jc[0]; v[3] = h * y2 * z2 * jc[0];
+  v[4] = h * x1 * z1 * jc[1]; v[5] = h * x2 * z1 * jc[1]; v[6] = h * x1 * z2 * jc[1]; v[7] = h * x2 * z2 * jc[1];
+  v[8] = h * x1 * y1 * jc[2]; v[9] = h * x2 * y1 * jc[2]; v[10] = h * x1 * y2 * jc[2]; v[11] = h * x2 * y2 * jc[2];
+}
+
+// Charge density weights at rnp (src/fdtdSC.cpp:141-160), node order o[0..7].
+__device__ inline void rho_weights(const DepositConst& c, double q, const double rp[3], double* v) {
+  double cc;
+  const double fx = modf((rp[0] - c.xmin) / c.dx, &cc);
+  const double fy = modf((rp[1] - c.ymin) / c.dy, &cc);
+  const double fz = modf((rp[2] - c.zmin) / c.dz, &cc);
+  const double x1 = 1.0 - fx, x2 = fx, y1 = 1.0 - fy, y2 = fy, z1 = 1.0 - fz, z2 = fz;
+  v[0] = q * x1 * y1 * z1; v[1] = q * x2 * y1 * z1; v[2] = q * x1 * y2 * z1; v[3] = q * x2 * y2 * z1;
+  v[4] = q * x1 * y1 * z2; v[5] = q * x2 * y1 * z2; v[6] = q * x1 * y2 * z2; v[7] = q * x2 * y2 * z2;
+}
+
+// Wavefront-wide segmented sum: lanes holding equal keys in consecutive lanes form a run, and
+// the first lane of each run ends with the run's total in v (Hillis-Steele suffix scan bounded
+// by the run's last lane).  Must be reached by all 64 lanes of the wavefront.
+template <int NV>
+__device__ inline void run_sum(long key, double (&v)[NV]) {
+  const int lane = __lane_id();
+  const long next = __shfl_down(key, 1);
+  const unsigned long long tails = __ballot(lane == 63 || next != key);
+  const int end = lane + __builtin_ctzll(tails >> lane);  // bit 63 is always set
+  for (int off = 1; off < 64; off <<= 1) {
+    const bool take = lane + off <= end;
+    if (__ballot(take) == 0ull) break;
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      const double o = __shfl_down(v[t], off);
+      if (take) v[t] += o;
+    }
+  }
+}
+
+__device__ inline bool run_head(long key) {
+  const long prev = __shfl_up(key, 1);
+  return __lane_id() == 0 || prev != key;
+}
+
+template <typename R>
+__device__ inline void scatter_j(R* jx, R* jy, R* jz, long m, long N1, long N1N0, const double* v) {
   const long o[8] = {0, N1, 1, N1 + 1, N1N0, N1N0 + N1, N1N0 + 1, N1N0 + N1 + 1};
-  const double wxa[8] = {y1, y1, y2, y2, y1, y1, y2, y2}, wxb[8] = {z1, z1, z1, z1, z2, z2, z2, z2};
-  const double wya[8] = {x1, x2, x1, x2, x1, x2, x1, x2}, wyb[8] = {z1, z1, z1, z1, z2, z2, z2, z2};
-  const double wza[8] = {x1, x2, x1, x2, x1, x2, x1, x2}, wzb[8] = {y1, y1, y2, y2, y1, y1, y2, y2};
 #pragma unroll
-  for (int t = 0; t < 8; ++t) add(jx + m + o[t], q * 0.5 * wxa[t] * wxb[t] * jc[0]);
-#pragma unroll
-  for (int t = 0; t < 8; ++t) add(jy + m + o[t], q * 0.5 * wya[t] * wyb[t] * jc[1]);
-#pragma unroll
-  for (int t = 0; t < 8; ++t) add(jz + m + o[t], q * 0.5 * wza[t] * wzb[t] * jc[2]);
+  for (int t = 0; t < 8; ++t) {
+    add(jx + m + o[t], v[t >> 1]);
+    add(jy + m + o[t], v[4 + (t & 1) + 2 * (t >> 2)]);
+    add(jz + m + o[t], v[8 + (t & 3)]);
+  }
 }
 
 // FdTd::currentUpdate / FdTdSC::currentUpdate (src/fdtd.cpp:38-185, src/fdtdSC.cpp:38-210):
 // Umeda zigzag relay point, two half-segment deposits and (with space charge) rho at rnp.
-template <typename R>
+// A particle whose two halves fall in the same cell adds them before depositing; lanes of a
+// wavefront depositing into the same cell (particles are kept sorted by cell) are summed
+// across the wavefront, and only the first lane of each run issues the atomic adds.
+template <typename R, bool SC>
 __global__ __launch_bounds__(256) void k_deposit(const double* __restrict__ part, long cap, long n, FieldSet J,
                                                  DepositConst c) {
+  constexpr int NP = SC ? 20 : 12;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const double rp[3] = {part[PX * cap + p], part[PY * cap + p], part[PZ * cap + p]};
-  const double rm[3] = {part[PXM * cap + p], part[PYM * cap + p], part[PZM * cap + p]};
-  const bool bp = inside(c, rp[0], rp[1], rp[2]);
-  const bool bm = inside(c, rm[0], rm[1], rm[2]);
-  if (!(bp || bm)) return;
-  const double q = part[PQ * cap + p];
-  const int ip = (int)floor((rp[0] - c.xmin) / c.dx), jp = (int)floor((rp[1] - c.ymin) / c.dy),
-            kp = (int)floor((rp[2] - c.zmin) / c.dz);
-  const int im = (int)floor((rm[0] - c.xmin) / c.dx), jm = (int)floor((rm[1] - c.ymin) / c.dy),
-            km = (int)floor((rm[2] - c.zmin) / c.dz);
-  double r[3];
-  r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
-  r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
-  r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
-  const double jcm[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
-  const double jcp[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
+  double vp[NP], vm[12];
+#pragma unroll
+  for (int t = 0; t < NP; ++t) vp[t] = 0.0;
+#pragma unroll
+  for (int t = 0; t < 12; ++t) vm[t] = 0.0;
+  long kp_key = -1, km_key = -1;
+  if (p < n) {
+    const double rp[3] = {part[PX * cap + p], part[PY * cap + p], part[PZ * cap + p]};
+    const double rm[3] = {part[PXM * cap + p], part[PYM * cap + p], part[PZM * cap + p]};
+    const bool bp = inside(c, rp[0], rp[1], rp[2]);
+    const bool bm = inside(c, rm[0], rm[1], rm[2]);
+    if (bp || bm) {
+      const double q = part[PQ * cap + p];
+      const int ip = (int)floor((rp[0] - c.xmin) / c.dx), jp = (int)floor((rp[1] - c.ymin) / c.dy),
+                kp = (int)floor((rp[2] - c.zmin) / c.dz);
+      const int im = (int)floor((rm[0] - c.xmin) / c.dx), jm = (int)floor((rm[1] - c.ymin) / c.dy),
+                km = (int)floor((rm[2] - c.zmin) / c.dz);
+      double r[3];
+      r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
+      r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
+      r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
+      const double jcm[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
+      const double jcp[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
+      const long N1 = c.N1, N1N0 = (long)c.N1 * c.N0;
+      if (bp && cell_ok(c, ip, jp, kp)) {
+        kp_key = N1N0 * kp + N1 * ip + jp;
+        half_weights(c, q, rp, r, jcp, vp);
+        if (SC) rho_weights(c, q, rp, vp + 12);
+      }
+      if (bm && cell_ok(c, im, jm, km)) {
+        km_key = N1N0 * km + N1 * im + jm;
+        half_weights(c, q, rm, r, jcm, vm);
+      }
+      if (kp_key >= 0 && kp_key == km_key) {
+#pragma unroll
+        for (int t = 0; t < 12; ++t) vp[t] += vm[t];
+        km_key = -1;
+      }
+    }
+  }
   R* jx = (R*)J.c[0];
   R* jy = (R*)J.c[1];
   R* jz = (R*)J.c[2];
   const long N1 = c.N1, N1N0 = (long)c.N1 * c.N0;
-  if (bp && cell_ok(c, ip, jp, kp)) {
-    const long m = N1N0 * kp + N1 * ip + jp;
-    deposit_half<R>(c, jx, jy, jz, m, q, rp, r, jcp);
-    if (c.sc) {
+  run_sum<NP>(kp_key, vp);
+  if (kp_key >= 0 && run_head(kp_key)) {
+    scatter_j(jx, jy, jz, kp_key, N1, N1N0, vp);
+    if (SC) {
       R* rho = (R*)J.c[3];
-      double cc;
-      const double fx = modf((rp[0] - c.xmin) / c.dx, &cc);
-      const double fy = modf((rp[1] - c.ymin) / c.dy, &cc);
-      const double fz = modf((rp[2] - c.zmin) / c.dz, &cc);
-      const double x1 = 1.0 - fx, x2 = fx, y1 = 1.0 - fy, y2 = fy, z1 = 1.0 - fz, z2 = fz;
-      add(rho + m, q * x1 * y1 * z1);
-      add(rho + m + N1, q * x2 * y1 * z1);
-      add(rho + m + 1, q * x1 * y2 * z1);
-      add(rho + m + N1 + 1, q * x2 * y2 * z1);
-      add(rho + m + N1N0, q * x1 * y1 * z2);
-      add(rho + m + N1N0 + N1, q * x2 * y1 * z2);
-      add(rho + m + N1N0 + 1, q * x1 * y2 * z2);
-      add(rho + m + N1N0 + N1 + 1, q * x2 * y2 * z2);
+      const long o[8] = {0, N1, 1, N1 + 1, N1N0, N1N0 + N1, N1N0 + 1, N1N0 + N1 + 1};
+#pragma unroll
+      for (int t = 0; t < 8; ++t) add(rho + kp_key + o[t], vp[12 + t]);
     }
   }
-  if (bm && cell_ok(c, im, jm, km)) {
-    const long m = N1N0 * km + N1 * im + jm;
-    deposit_half<R>(c, jx, jy, jz, m, q, rm, r, jcm);
+  if (__ballot(km_key >= 0) != 0ull) {
+    run_sum<12>(km_key, vm);
+    if (km_key >= 0 && run_head(km_key)) scatter_j(jx, jy, jz, km_key, N1, N1N0, vm);
+  }
+}
+
+// Row key of one half of every particle: the (k, i) row k*N0 + i of the cell the half deposits
+// from (rnp or rnm, the cell test of k_deposit), or `rows` when that half deposits nothing.
+__global__ __launch_bounds__(256) void k_row_keys(const double* __restrict__ part, long cap, long n, DepositConst c,
+                                                  int use_rm, unsigned rows, unsigned* __restrict__ key,
+                                                  unsigned* __restrict__ val) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int o = use_rm ? PXM : PX;
+  const double x = part[(o + 0) * cap + p], y = part[(o + 1) * cap + p], z = part[(o + 2) * cap + p];
+  unsigned k = rows;
+  if (inside(c, x, y, z)) {
+    const int i = (int)floor((x - c.xmin) / c.dx), j = (int)floor((y - c.ymin) / c.dy),
+              kk = (int)floor((z - c.zmin) / c.dz);
+    if (cell_ok(c, i, j, kk)) k = (unsigned)kk * (unsigned)c.N0 + (unsigned)i;
+  }
+  key[p] = k;
+  val[p] = (unsigned)p;
+}
+
+// Adds one half's contributions to the nodes of the tile (plane k, rows i0 .. i0+ti-1, all j).
+template <bool RHO>
+__device__ inline void add_owned(double* tile, int TI, int N1, int k, int i0, int ti, int ic, int jc, int kc,
+                                 const double* v, int ncomp) {
+  const int tsz = TI * N1;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    if (kc + (t >> 2) != k) continue;
+    const int row = ic + (t & 1) - i0;
+    if (row < 0 || row >= ti) continue;
+    double* at = tile + row * N1 + jc + ((t >> 1) & 1);
+    atomicAdd(at, v[t >> 1]);
+    atomicAdd(at + tsz, v[4 + (t & 1) + 2 * (t >> 2)]);
+    atomicAdd(at + 2 * tsz, v[8 + (t & 3)]);
+    if (RHO) atomicAdd(at + 3 * tsz, v[12 + t]);
+  }
+}
+
+// Row-tiled current deposition (same arithmetic per particle as k_deposit).  Workgroup b owns
+// the J (and rho) values of plane k and rows i0 .. i0+ti-1, all j: it adds, in LDS, the halves
+// whose cell lies in planes k-1..k and rows i0-1 .. i0+ti-1 (read from the two row orders),
+// then writes its nodes once.  Every node of the mesh belongs to exactly one workgroup, so no
+// global atomics are needed and the J buffer needs no separate reset (`accumulate` = 0 writes
+// the deposit alone, 1 adds it to what the buffer holds).
+template <typename R, bool SC>
+__global__ __launch_bounds__(256) void k_deposit_rows(const double* __restrict__ part, long cap, FieldSet J,
+                                                      DepositConst c, const unsigned* __restrict__ p_row,
+                                                      const unsigned* __restrict__ p_idx,
+                                                      const unsigned* __restrict__ m_row,
+                                                      const unsigned* __restrict__ m_idx, int TI, int tilesI,
+                                                      int accumulate) {
+  extern __shared__ double tile[];  // [ncomp][TI][N1]
+  constexpr int ncomp = SC ? 4 : 3;
+  const int k = blockIdx.x / tilesI;
+  const int i0 = (blockIdx.x - k * tilesI) * TI;
+  const int ti = min(TI, c.N0 - i0);
+  const int N1 = c.N1;
+  const long N1N0 = (long)N1 * c.N0;
+  const int tsz = TI * N1, own = ti * N1;
+  const long base = N1N0 * k + (long)N1 * i0;
+  for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
+    const int comp = t / own, o = t - comp * own;
+    tile[comp * tsz + o] = accumulate ? (double)((const R*)J.c[comp])[base + o] : 0.0;
+  }
+  __syncthreads();
+  const int lo = max(i0 - 1, 0), hi = min(i0 + ti - 1, c.N0 - 2);
+  for (int half = 0; half < 2; ++half) {
+    const unsigned* __restrict__ row = half ? m_row : p_row;
+    const unsigned* __restrict__ idx = half ? m_idx : p_idx;
+    for (int kk = max(k - 1, 0); kk <= min(k, c.N2 - 2); ++kk) {
+      if (lo > hi) break;
+      const unsigned s0 = row[(long)kk * c.N0 + lo], s1 = row[(long)kk * c.N0 + hi + 1];
+      for (unsigned s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        const long p = idx[s];
+        const double rp[3] = {part[PX * cap + p], part[PY * cap + p], part[PZ * cap + p]};
+        const double rm[3] = {part[PXM * cap + p], part[PYM * cap + p], part[PZM * cap + p]};
+        const double q = part[PQ * cap + p];
+        const int ip = (int)floor((rp[0] - c.xmin) / c.dx), jp = (int)floor((rp[1] - c.ymin) / c.dy),
+                  kp = (int)floor((rp[2] - c.zmin) / c.dz);
+        const int im = (int)floor((rm[0] - c.xmin) / c.dx), jm = (int)floor((rm[1] - c.ymin) / c.dy),
+                  km = (int)floor((rm[2] - c.zmin) / c.dz);
+        double r[3];
+        r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
+        r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
+        r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
+        // The order lists this half under row (kk, lo..hi); skip it should its cell disagree.
+        const int ic = half ? im : ip, jc = half ? jm : jp, kc = half ? km : kp;
+        if (kc != kk || ic < lo || ic > hi || jc < 0 || jc > c.N1 - 2) continue;
+        double v[SC ? 20 : 12];
+        if (half == 0) {
+          const double jc[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
+          half_weights(c, q, rp, r, jc, v);
+          if (SC) {
+            rho_weights(c, q, rp, v + 12);
+            add_owned<SC>(tile, TI, N1, k, i0, ti, ip, jp, kp, v, ncomp);
+          } else {
+            add_owned<false>(tile, TI, N1, k, i0, ti, ip, jp, kp, v, ncomp);
+          }
+        } else {
+          const double jc[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
+          half_weights(c, q, rm, r, jc, v);
+          add_owned<false>(tile, TI, N1, k, i0, ti, im, jm, km, v, ncomp);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
+    const int comp = t / own, o = t - comp * own;
+    ((R*)J.c[comp])[base + o] = (R)tile[comp * tsz + o];
   }
 }
 
@@ -324,6 +497,7 @@ hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks) {
 }
 
 hipError_t launch_bunch_begin(Domain* d) {
+  d->rm_order_valid = d->order1_is_rnp;  // rnm := rnp
   if (d->np == 0) return hipSuccess;
   hipLaunchKernelGGL(k_bunch_begin, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap,
                      d->np);
@@ -332,6 +506,8 @@ hipError_t launch_bunch_begin(Domain* d) {
 
 hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
                        double time_bunch, double bunch_time_step, int n_sub, int save_rnm) {
+  if (save_rnm) d->rm_order_valid = d->order1_is_rnp;  // rnm := rnp
+  if (n_sub > 0) d->order1_is_rnp = 0;                  // rnp moves
   if (d->np == 0 || n_sub <= 0) return hipSuccess;
   const mithra_grid& g = d->g;
   PushConst c;
@@ -348,19 +524,114 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_und
   return hipGetLastError();
 }
 
-hipError_t launch_deposit(Domain* d) {
-  if (d->np == 0) return hipSuccess;
+namespace {
+
+DepositConst deposit_const(const Domain* d) {
   const mithra_grid& g = d->g;
   DepositConst c;
   c.xmin = g.xmin; c.xmax = g.xmax; c.ymin = g.ymin; c.ymax = g.ymax; c.zmin = g.zmin;
   c.zp0 = g.zp0; c.zp1 = g.zp1; c.dx = g.dx; c.dy = g.dy; c.dz = g.dz;
   c.N0 = g.N0; c.N1 = g.N1; c.N2 = g.N2; c.sc = g.space_charge;
+  return c;
+}
+
+hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows) {
+  const DepositConst c = deposit_const(d);
+  hipLaunchKernelGGL(k_row_keys, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap,
+                     d->np, c, use_rm, rows, o.key, o.idx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return sort_order(d, o, d->np, rows);
+}
+
+// Rows per workgroup tile of the row-tiled deposit: the tile's LDS (ncomp x TI x N1 doubles)
+// stays within 48 KiB (three workgroups per CU); 0 when even one row does not fit.
+int tile_rows(const Domain* d) {
+  const long per_row = (long)d->ncomp * d->g.N1 * (long)sizeof(double);
+  const long fit = (48L << 10) / per_row;
+  if (fit < 1) return 0;
+  int ti = 1;
+  while (ti * 2 <= fit && ti * 2 <= 16) ti *= 2;
+  return ti;
+}
+
+hipError_t deposit_atomic(Domain* d) {
+  const DepositConst c = deposit_const(d);
   const dim3 grid((unsigned)((d->np + 255) / 256));
-  if (d->precision == MITHRA_PRECISION_FP64)
-    hipLaunchKernelGGL((k_deposit<double>), grid, dim3(256), 0, d->stream, d->part, d->cap, d->np, d->lvl[d->i_np1], c);
+  const FieldSet J = d->lvl[d->i_np1];
+  const bool f64 = d->precision == MITHRA_PRECISION_FP64;
+  if (f64 && c.sc)
+    hipLaunchKernelGGL((k_deposit<double, true>), grid, dim3(256), 0, d->stream, d->part, d->cap, d->np, J, c);
+  else if (f64)
+    hipLaunchKernelGGL((k_deposit<double, false>), grid, dim3(256), 0, d->stream, d->part, d->cap, d->np, J, c);
+  else if (c.sc)
+    hipLaunchKernelGGL((k_deposit<float, true>), grid, dim3(256), 0, d->stream, d->part, d->cap, d->np, J, c);
   else
-    hipLaunchKernelGGL((k_deposit<float>), grid, dim3(256), 0, d->stream, d->part, d->cap, d->np, d->lvl[d->i_np1], c);
+    hipLaunchKernelGGL((k_deposit<float, false>), grid, dim3(256), 0, d->stream, d->part, d->cap, d->np, J, c);
   return hipGetLastError();
+}
+
+}  // namespace
+
+bool deposit_tiled(const Domain* d) {
+  const long rows = (long)(d->g.N2 - 1) * d->g.N0;
+  return tile_rows(d) > 0 && rows < 0xFFFFFFF0L;
+}
+
+hipError_t launch_deposit(Domain* d, int accumulate) {
+  const mithra_grid& g = d->g;
+  if (!deposit_tiled(d)) {
+    if (!accumulate) {
+      for (int c = 0; c < d->ncomp; ++c) {
+        hipError_t e = launch_fill_zero(d, d->lvl[d->i_np1].c[c], (size_t)d->nodes * d->real_size);
+        if (e != hipSuccess) return e;
+      }
+    }
+    return d->np ? deposit_atomic(d) : hipSuccess;
+  }
+  hipError_t e;
+  if (d->np == 0) {  // nothing to add
+    if (accumulate) return hipSuccess;
+    for (int cc = 0; cc < d->ncomp; ++cc)
+      if ((e = launch_fill_zero(d, d->lvl[d->i_np1].c[cc], (size_t)d->nodes * d->real_size)) != hipSuccess) return e;
+    return hipSuccess;
+  }
+  const unsigned rows = (unsigned)((g.N2 - 1) * (long)g.N0);
+  if ((e = ensure_orders(d, rows)) != hipSuccess) return e;
+  if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows)) != hipSuccess) return e;
+  if ((e = build_order(d, d->ord[0], 0, rows)) != hipSuccess) return e;
+  const int TI = tile_rows(d);
+  const int tilesI = (g.N0 + TI - 1) / TI;
+  const DepositConst c = deposit_const(d);
+  const FieldSet J = d->lvl[d->i_np1];
+  const size_t lds = sizeof(double) * d->ncomp * TI * g.N1;
+  const dim3 grid((unsigned)((long)g.N2 * tilesI));
+  const unsigned *pr = d->ord[0].row, *pi = d->ord[0].idx, *mr = d->ord[1].row, *mi = d->ord[1].idx;
+  const bool f64 = d->precision == MITHRA_PRECISION_FP64;
+  if (f64 && c.sc)
+    hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+                       mr, mi, TI, tilesI, accumulate);
+  else if (f64)
+    hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+                       mr, mi, TI, tilesI, accumulate);
+  else if (c.sc)
+    hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+                       mr, mi, TI, tilesI, accumulate);
+  else
+    hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+                       mr, mi, TI, tilesI, accumulate);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // Periodic reorder of the particle arrays into the rnp row order.
+  if (d->sort_interval > 0 && d->sort_count % d->sort_interval == 0)
+    if ((e = reorder_particles(d, d->ord[0])) != hipSuccess) return e;
+  ++d->sort_count;
+  // The rnp order becomes the rnm order of the next deposit once rnm is set from rnp.
+  CellOrder t = d->ord[0];
+  d->ord[0] = d->ord[1];
+  d->ord[1] = t;
+  d->order1_is_rnp = 1;
+  d->rm_order_valid = 0;
+  return hipSuccess;
 }
 
 }  // namespace mithra

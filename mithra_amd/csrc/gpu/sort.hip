@@ -1,0 +1,153 @@
+// Cell-row orders of the particle halves, and the physical reordering of the particle arrays.
+//
+// The current deposition (particles.hip) gives each workgroup the J values of one z-plane and a
+// few x-rows of the mesh, and needs every particle half whose cell lies in the adjacent cell
+// rows.  An "order" is the list of particle indices sorted by the row (k, i) of one half's cell,
+// plus the start of each row in that list; a workgroup then reads two contiguous ranges per
+// half.  Sorting is a radix sort of 32-bit (row, index) pairs.
+//
+// The particle arrays themselves are permuted into row order now and then, so the sorted
+// indices walk memory almost sequentially; the original index travels with each particle in
+// `pid`, and the host always sees the records in load order.
+#include "common.h"
+
+#include <hipcub/device/device_radix_sort.hpp>
+
+namespace mithra {
+namespace {
+
+__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, double* __restrict__ dst,
+                                                const unsigned* __restrict__ id_src, unsigned* __restrict__ id_dst,
+                                                unsigned* __restrict__ perm, long cap, long n) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const long s = perm[p];
+#pragma unroll
+  for (int a = 0; a < PCOUNT; ++a) dst[a * cap + p] = src[a * cap + s];
+  id_dst[p] = id_src[s];
+  perm[p] = (unsigned)p;  // the order now lists the reordered arrays in sequence
+}
+
+__global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ id, long n) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) id[p] = (unsigned)p;
+}
+
+// row_start[r] = number of sorted keys below r, for r in [0, rows]
+__global__ __launch_bounds__(256) void k_row_start(const unsigned* __restrict__ key, long n, unsigned rows,
+                                                   unsigned* __restrict__ row_start) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > (long)rows) return;
+  long lo = 0, hi = n;
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if (key[mid] < (unsigned)r) lo = mid + 1;
+    else hi = mid;
+  }
+  row_start[r] = (unsigned)lo;
+}
+
+}  // namespace
+
+hipError_t launch_particle_ids(Domain* d) {
+  if (d->np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->pid, d->np);
+  return hipGetLastError();
+}
+
+void free_sort(Domain* d) {
+  (void)hipFree(d->part_alt);
+  (void)hipFree(d->pid_alt);
+  (void)hipFree(d->sort_block);
+  (void)hipFree(d->sort_tmp);
+  d->part_alt = nullptr;
+  d->pid_alt = nullptr;
+  d->sort_block = nullptr;
+  d->sort_tmp = nullptr;
+  d->sort_tmp_bytes = 0;
+  d->sort_cap = 0;
+  for (int s = 0; s < 2; ++s) {
+    d->ord[s].key = d->ord[s].idx = d->ord[s].row = nullptr;
+  }
+  d->key_alt = d->idx_alt = nullptr;
+  d->rm_order_valid = 0;
+  d->order1_is_rnp = 0;
+}
+
+hipError_t ensure_orders(Domain* d, unsigned rows) {
+  if (d->sort_cap == d->cap && d->sort_rows == rows) return hipSuccess;
+  free_sort(d);
+  const size_t cap = (size_t)d->cap;
+  const size_t per = (cap + 63) / 64 * 64, rper = ((size_t)rows + 1 + 63) / 64 * 64;
+  hipError_t e = hipMalloc(&d->sort_block, sizeof(unsigned) * (6 * per + 2 * rper));
+  if (e != hipSuccess) return e;
+  unsigned* b = d->sort_block;
+  for (int s = 0; s < 2; ++s) {
+    d->ord[s].key = b; b += per;
+    d->ord[s].idx = b; b += per;
+    d->ord[s].row = b; b += rper;
+  }
+  d->key_alt = b; b += per;
+  d->idx_alt = b;
+  d->sort_cap = d->cap;
+  d->sort_rows = rows;
+  return hipSuccess;
+}
+
+hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows) {
+  int bits = 1;
+  while (bits < 32 && (rows >> bits) != 0) ++bits;  // the sentinel `rows` must fit
+  hipcub::DoubleBuffer<unsigned> k(o.key, d->key_alt), v(o.idx, d->idx_alt);
+  size_t bytes = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)n, 0, bits, d->stream);
+  if (e != hipSuccess) return e;
+  if (bytes > d->sort_tmp_bytes || !d->sort_tmp) {
+    (void)hipFree(d->sort_tmp);
+    d->sort_tmp = nullptr;
+    d->sort_tmp_bytes = 0;
+    if ((e = hipMalloc(&d->sort_tmp, bytes ? bytes : 1)) != hipSuccess) return e;
+    d->sort_tmp_bytes = bytes;
+  }
+  bytes = d->sort_tmp_bytes;
+  if ((e = hipcub::DeviceRadixSort::SortPairs(d->sort_tmp, bytes, k, v, (int)n, 0, bits, d->stream)) != hipSuccess)
+    return e;
+  if (k.Current() != o.key) {  // keep the sorted pairs in the order's own arrays
+    d->key_alt = o.key;
+    o.key = k.Current();
+  }
+  if (v.Current() != o.idx) {
+    d->idx_alt = o.idx;
+    o.idx = v.Current();
+  }
+  hipLaunchKernelGGL(k_row_start, dim3((unsigned)(((long)rows + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, n,
+                     rows, o.row);
+  return hipGetLastError();
+}
+
+hipError_t reorder_particles(Domain* d, CellOrder& o) {
+  const long n = d->np;
+  if (n < 2) return hipSuccess;
+  if (!d->part_alt) {
+    hipError_t e = hipMalloc(&d->part_alt, sizeof(double) * PCOUNT * (size_t)d->cap);
+    if (e == hipSuccess) e = hipMalloc(&d->pid_alt, sizeof(unsigned) * (size_t)d->cap);
+    if (e != hipSuccess) {
+      (void)hipFree(d->part_alt);
+      d->part_alt = nullptr;
+      d->pid_alt = nullptr;
+      return e;
+    }
+  }
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream, d->part, d->part_alt,
+                     d->pid, d->pid_alt, o.idx, d->cap, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  double* t = d->part;
+  d->part = d->part_alt;
+  d->part_alt = t;
+  unsigned* ti = d->pid;
+  d->pid = d->pid_alt;
+  d->pid_alt = ti;
+  return hipSuccess;
+}
+
+}  // namespace mithra
