@@ -168,6 +168,12 @@ class March:
 
 def run_gpu(job, steps, warmup, precision):
     dom = open_domain(job, precision)
+    dom.set_lazy_fields(1)   # E/B where the bunch and the sampler read them (the reference's rule)
+    (ci, ce), tune_ms = dom.autotune(reps=2)
+    log("autotune (ms per launch, tile shapes rows x planes): " + "; ".join(
+        f"{abi.tile_shape(v)[0]}x{abi.tile_shape(v)[1]} interior {tune_ms[0, v]:.3f} eval {tune_ms[1, v]:.3f}"
+        for v in range(abi.TILE_VARIANTS)))
+    tiles = {"interior": list(abi.tile_shape(ci)), "eval": list(abi.tile_shape(ce))}
     m = March(job)
     n_sub, dtb = m.t["n_sub"], m.t["bunch_time_step"]
     rows = 64
@@ -191,7 +197,7 @@ def run_gpu(job, steps, warmup, precision):
     counters = dom.bunch_counters()
     P = dom.power_fetch(0, min(rows, warmup + steps), 1, 1)[:, 0, 0]
     dom.close()
-    return elapsed, phases, counters, P
+    return elapsed, phases, counters, P, tiles
 
 
 def run_cpu_baseline(job, steps, threads):
@@ -258,7 +264,7 @@ def main():
         n_part = job.particles().shape[0]
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
-        elapsed, phases, counters, P = run_gpu(job, args.steps, args.warmup, precision)
+        elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision)
         log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
         ms_step = 1e3 * elapsed / args.steps
         cell_rate = nodes * args.steps / elapsed
@@ -267,6 +273,7 @@ def main():
         t_int = phases[abi.PHASES.index("interior")] * 1e-3
         achieved = bytes_per_node * interior / t_int / 1e9
         copy_gbs = abi.copy_bandwidth()
+        copy8_gbs = abi.copy_bandwidth(elem_bytes=8)
         cpu = None
         if args.cpu_steps > 0:
             log(f"CPU oracle sample: {args.cpu_steps} steps on {args.cpu_threads} threads")
@@ -299,11 +306,12 @@ def main():
         "phase_ms": {k: round(float(v), 4) for k, v in zip(abi.PHASES, phases)},
         "roofline": {"bound": "hbm", "kernel": "k_interior (NSFD stencil)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "frac_of_measured_copy": achieved / copy_gbs, "measured_copy_GBps": copy_gbs,
+                     "frac_of_measured_copy": achieved / copy_gbs, "measured_copy_GBps": copy_gbs, "measured_copy8_GBps": copy8_gbs,
                      "traffic": traffic_from_profiles(args.precision),
                      "algorithmic_bytes_per_launch": bytes_per_node * interior},
         "cpu_baseline": cpu,
         "setup_s": t_setup,
+        "tiles_rows_x_planes": tiles,
         "bunch_counters": {"transverse_escapes": counters[0], "z_crossings": counters[1]},
         "power_last": float(P[-1]) if len(P) else None,
     }

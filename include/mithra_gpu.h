@@ -215,6 +215,16 @@ int mithra_dev_step(mithra_dev* dev, double time, const mithra_seed* seed, const
                     const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
                     int n_sub, unsigned n_time, int power_row);
 
+/* lazy = 1: each field update evaluates E/B only on the z-planes the bunch can read during the
+ * coming push (its cell extent after the previous push, widened by the one cell a particle can
+ * move per field step) and on the power-sampling planes -- the reference's rule of evaluating
+ * E/B only where particles and samplers need it (FdTd::fieldEvaluate, src/fdtd.cpp:818-845,
+ * called per touched node; flags pic_, src/solver.h:241).  Elsewhere the E/B array keeps older
+ * values, as the reference's en_/bn_ do.  lazy = 0 (default) evaluates every node, for callers
+ * that read the whole field with mithra_dev_get_fields.  Until a push has measured the bunch
+ * (after mithra_dev_set_particles) every node is evaluated either way. */
+int mithra_dev_set_lazy_fields(mithra_dev* dev, int lazy);
+
 /* Deposits between two reorders of the particle arrays into the cell-row order of the
  * deposition (default 16; 0 never reorders).  The deposition sorts its own index lists every
  * step; reordering the arrays only makes those lists walk memory in sequence.  No per-particle
@@ -222,6 +232,20 @@ int mithra_dev_step(mithra_dev* dev, double time, const mithra_seed* seed, const
  * were set.  No counterpart in the reference (its particles live in a std::list,
  * src/solver.h:271). */
 int mithra_dev_set_reorder_interval(mithra_dev* dev, int interval);
+
+/* Tile shapes of the two marched field kernels (interior update, E/B evaluation): variant v
+ * has `rows` x-rows of 64 y-nodes per workgroup and marches `planes` z-planes; variants 7-9
+ * stage the interior update's A^n planes in LDS.  Every shape
+ * computes identical values; only the speed differs. */
+#define MITHRA_TILE_VARIANTS 10
+int mithra_tile_shape(int variant, int* rows, int* planes);
+int mithra_dev_set_tiles(mithra_dev* dev, int interior, int eval);
+/* Autotuner: times `reps` launches of every tile shape of both kernels on this domain's mesh
+ * and keeps the fastest of each.  The domain's state is left as it was (the interior update
+ * runs on a saved copy of the J it consumes).  chosen[2] (optional) receives the two variant
+ * indices, ms[2 * MITHRA_TILE_VARIANTS] (optional) the mean time of each shape.  No
+ * counterpart in the reference (a CPU code with no launch shapes). */
+int mithra_dev_autotune(mithra_dev* dev, int reps, int* chosen, float* ms);
 
 /* Bunching factor b = |sum q exp(i k z)| / |sum q| over all particles at their current boosted-
  * frame z.  With k = 2 pi / lambda', lambda' = lambda_u / (gamma lambda_norm) the radiation
@@ -243,6 +267,8 @@ int mithra_dev_step_timed(mithra_dev* dev, double time, const mithra_seed* seed,
  * (read + write bytes per second, GB/s), the measured HBM ceiling the roofline fractions are
  * also reported against.  No counterpart in the reference. */
 int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps);
+/* The same with elem_bytes (8, 16 or 32) bytes moved per lane and load. */
+int mithra_copy_bandwidth_elem(unsigned long bytes, int reps, int elem_bytes, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -75,13 +75,28 @@ class MithraError(RuntimeError):
 PHASES = ("interior", "tfsf", "boundaries", "eval_eb", "push", "power", "current_reset", "deposit")
 
 
-def copy_bandwidth(nbytes=2 << 30, reps=10):
-    """Measured device copy rate (GB/s, read + write) of a plain streaming kernel."""
+TILE_VARIANTS = 10
+
+
+def tile_shape(variant):
+    """(rows, planes) of a field-kernel tile variant."""
     lib = load_gpu_library()
-    f = lib.mithra_copy_bandwidth
-    f.restype, f.argtypes = C.c_int, [C.c_ulong, C.c_int, C.POINTER(C.c_double)]
+    f = lib.mithra_tile_shape
+    f.restype, f.argtypes = C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    r, k = C.c_int(), C.c_int()
+    if f(variant, C.byref(r), C.byref(k)) != 0:
+        raise MithraError(f"no tile variant {variant}")
+    return r.value, k.value
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10, elem_bytes=32):
+    """Measured device copy rate (GB/s, read + write) of a plain streaming kernel moving
+    elem_bytes (8, 16, 32) per lane."""
+    lib = load_gpu_library()
+    f = lib.mithra_copy_bandwidth_elem
+    f.restype, f.argtypes = C.c_int, [C.c_ulong, C.c_int, C.c_int, C.POINTER(C.c_double)]
     out = C.c_double()
-    rc = f(nbytes, reps, C.byref(out))
+    rc = f(nbytes, reps, elem_bytes, C.byref(out))
     if rc != 0:
         raise MithraError(f"mithra_copy_bandwidth: status {rc}")
     return out.value
@@ -99,6 +114,7 @@ def _declare(lib, prefix):
         "get_particles": (C.c_int, [vp, _dp, C.c_long]),
         "particle_count": (C.c_long, [vp]),
         "field_update": (C.c_int, [vp, C.c_double, C.POINTER(Seed)]),
+        "field_evaluate": (C.c_int, [vp]),
         "bunch_begin": (C.c_int, [vp]),
         "bunch_update": (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
                                    C.c_double]),
@@ -113,6 +129,9 @@ def _declare(lib, prefix):
     }
     if prefix == "mithra_dev_":
         sig["set_reorder_interval"] = (C.c_int, [vp, C.c_int])
+        sig["set_tiles"] = (C.c_int, [vp, C.c_int, C.c_int])
+        sig["set_lazy_fields"] = (C.c_int, [vp, C.c_int])
+        sig["autotune"] = (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)])
         sig["bunch_advance"] = (C.c_int, [vp, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
                                           C.c_double, C.c_double, C.c_int])
         sig["synchronize"] = (C.c_int, [vp])
@@ -217,6 +236,10 @@ class Domain:
         return q.reshape(n, 11)
 
     # -- the step --------------------------------------------------------------------
+    def field_evaluate(self):
+        """E/B from A^n and A^{n+1} on every node (FdTd::fieldEvaluate)."""
+        self._check(self.f["field_evaluate"](self.h))
+
     def field_update(self, time, seed=None):
         self._check(self.f["field_update"](self.h, time, None if seed is None else C.byref(seed)))
 
@@ -283,6 +306,22 @@ class Domain:
     def set_reorder_interval(self, n):
         """Deposits between reorders of the particle arrays (0: never); device library only."""
         self._check(self.f["set_reorder_interval"](self.h, n))
+
+    def set_lazy_fields(self, lazy):
+        """E/B only where the bunch and the power sampler read them (1), or everywhere (0)."""
+        self._check(self.f["set_lazy_fields"](self.h, int(lazy)))
+
+    def set_tiles(self, interior, evaluation):
+        """Tile shape variants of the interior update and the E/B evaluation."""
+        self._check(self.f["set_tiles"](self.h, interior, evaluation))
+
+    def autotune(self, reps=3):
+        """Times every tile shape of both field kernels; keeps the fastest.  Returns
+        (chosen (interior, eval), ms[2][variants])."""
+        chosen = (C.c_int * 2)()
+        ms = (C.c_float * (2 * TILE_VARIANTS))()
+        self._check(self.f["autotune"](self.h, reps, chosen, ms))
+        return (chosen[0], chosen[1]), np.array(ms[:]).reshape(2, TILE_VARIANTS)
 
     def bunching(self, k):
         """|sum q exp(i k z)| / |sum q| over the particles."""

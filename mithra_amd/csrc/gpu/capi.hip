@@ -1,5 +1,6 @@
 // C-ABI of libmithra_gpu.so (include/mithra_gpu.h): device memory, transfers between the
 // reference's host layouts and the device layout, and the launch sequence of each call.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -115,6 +116,8 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
   d->nodes = d->N1N0 * grid->N2;
   d->i_np1 = 0; d->i_n = 1; d->i_nm1 = 2;
   d->sort_interval = kDefaultSortInterval;
+  d->pk_lo = 1;
+  d->pk_hi = 0;
   hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete d; return hip_status(e); }
   const size_t per = (size_t)d->nodes * d->real_size;
@@ -122,7 +125,9 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
   const size_t stride = (per + 255) / 256 * 256;
   if (hipMalloc(&d->pot_block, stride * 3 * d->ncomp) != hipSuccess ||
       hipMalloc(&d->eb, sizeof(float4) * 2 * (size_t)d->nodes) != hipSuccess ||
-      hipMalloc(&d->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&d->counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&d->zrange, 2 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&d->zpart, 1024 * 2 * sizeof(int)) != hipSuccess) {
     mithra_dev_destroy(d);
     return MITHRA_E_NOMEM;
   }
@@ -144,6 +149,8 @@ int mithra_dev_destroy(mithra_dev* d) {
   hipFree(d->pot_block);
   hipFree(d->eb);
   hipFree(d->counters);
+  hipFree(d->zrange);
+  hipFree(d->zpart);
   hipFree(d->part);
   hipFree(d->pid);
   free_sort(d);
@@ -251,6 +258,7 @@ int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
   d->sort_count = 0;
   d->rm_order_valid = 0;  // no cell order describes the new particles yet
   d->order1_is_rnp = 0;
+  d->zrange_valid = 0;    // nor a bunch extent
   if (n == 0) return MITHRA_OK;
   MITHRA_TRY(launch_particle_ids(d));
   // Charge record q, rnp[3], rnm[3], gb[3], e  ->  x y z xm ym zm gx gy gz q e
@@ -290,7 +298,7 @@ int mithra_dev_field_update(mithra_dev* d, double time, const mithra_seed* seed)
   MITHRA_TRY(launch_interior(d));
   if (seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4) MITHRA_TRY(launch_tfsf(d, time, seed));
   MITHRA_TRY(launch_boundaries(d));
-  MITHRA_TRY(launch_eval_eb(d));
+  MITHRA_TRY(launch_eval_eb_step(d));
   return MITHRA_OK;
 }
 
@@ -357,6 +365,13 @@ int mithra_dev_power_setup(mithra_dev* d, const mithra_power_params* p, unsigned
     }
   }
   d->pNz = nz;
+  d->pk_lo = 1;
+  d->pk_hi = 0;
+  for (int c : hk) {
+    if (d->pk_lo > d->pk_hi) d->pk_lo = d->pk_hi = c;
+    d->pk_lo = std::min(d->pk_lo, c);
+    d->pk_hi = std::max(d->pk_hi, c);
+  }
   // DFT phases exactly as the reference builds ep/em (src/radiation.cpp:110-117)
   std::vector<double> dft((size_t)p->Nl * p->Nf * 2);
   for (unsigned i = 0; i < p->Nl; ++i)
@@ -433,6 +448,12 @@ int mithra_dev_current_update(mithra_dev* d) {
   return hip_status(e);
 }
 
+int mithra_dev_set_lazy_fields(mithra_dev* d, int lazy) {
+  if (!d || (lazy != 0 && lazy != 1)) return MITHRA_E_ARG;
+  d->lazy_fields = lazy;
+  return MITHRA_OK;
+}
+
 int mithra_dev_set_reorder_interval(mithra_dev* d, int interval) {
   if (!d || interval < 0) return MITHRA_E_ARG;
   d->sort_interval = interval;
@@ -449,6 +470,76 @@ int mithra_dev_step(mithra_dev* d, double time, const mithra_seed* seed, const m
   if (!rc) rc = mithra_dev_field_shift(d);
   if (!rc) rc = mithra_dev_current_reset(d);
   if (!rc) rc = mithra_dev_current_update(d);
+  return rc;
+}
+
+int mithra_tile_shape(int variant, int* rows, int* planes) {
+  if (variant < 0 || variant >= kTileVariants) return MITHRA_E_ARG;
+  if (rows) *rows = kTileShapes[variant][0];
+  if (planes) *planes = kTileShapes[variant][1];
+  return MITHRA_OK;
+}
+
+int mithra_dev_set_tiles(mithra_dev* d, int interior, int eval) {
+  if (!d || interior < 0 || interior >= kTileVariants || eval < 0 || eval >= kTileVariants) return MITHRA_E_ARG;
+  d->tile_interior = interior;
+  d->tile_eval = eval;
+  return MITHRA_OK;
+}
+
+int mithra_dev_autotune(mithra_dev* d, int reps, int* chosen, float* ms) {
+  if (!d || reps < 1) return MITHRA_E_ARG;
+  if (int rc = settle(d)) return rc;
+  hipStream_t s = d->stream;
+  hipEvent_t e0, e1;
+  MITHRA_TRY(hipEventCreate(&e0));
+  MITHRA_TRY(hipEventCreate(&e1));
+  // The interior update overwrites the J it reads: time it on a saved copy of that level.
+  const size_t level_bytes = (size_t)d->ncomp * ((d->nodes * d->real_size + 255) / 256 * 256);
+  void* saved = nullptr;
+  int rc = MITHRA_OK;
+  if (hipMalloc(&saved, level_bytes) != hipSuccess) rc = MITHRA_E_NOMEM;
+  void* np1 = d->lvl[d->i_np1].c[0];  // the components of one level are contiguous
+  if (!rc) rc = hip_status(hipMemcpyAsync(saved, np1, level_bytes, hipMemcpyDeviceToDevice, s));
+  const int keep[2] = {d->tile_interior, d->tile_eval};
+  int best[2] = {keep[0], keep[1]};
+  for (int kernel = 0; kernel < 2 && !rc; ++kernel) {
+    float best_ms = 0.0f;
+    for (int v = 0; v < kTileVariants && !rc; ++v) {
+      if (kernel == 0) d->tile_interior = v;
+      else d->tile_eval = v;
+      float total = 0.0f;
+      for (int r = 0; r <= reps && !rc; ++r) {  // r = 0 warms up
+        if (kernel == 0) rc = hip_status(hipMemcpyAsync(np1, saved, level_bytes, hipMemcpyDeviceToDevice, s));
+        if (!rc) rc = hip_status(hipEventRecord(e0, s));
+        if (!rc) rc = hip_status(kernel == 0 ? launch_interior(d) : launch_eval_eb(d));
+        if (!rc) rc = hip_status(hipEventRecord(e1, s));
+        if (!rc) rc = hip_status(hipEventSynchronize(e1));
+        float t = 0.0f;
+        if (!rc) rc = hip_status(hipEventElapsedTime(&t, e0, e1));
+        if (r > 0) total += t;
+      }
+      if (!rc && (v == 0 || total < best_ms)) {
+        best_ms = total;
+        best[kernel] = v;
+      }
+      if (ms) ms[kernel * kTileVariants + v] = total / reps;
+    }
+  }
+  if (saved) {
+    hipError_t e = hipMemcpyAsync(np1, saved, level_bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) rc = hip_status(e);
+    (void)hipFree(saved);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  d->tile_interior = rc ? keep[0] : best[0];
+  d->tile_eval = rc ? keep[1] : best[1];
+  if (!rc && chosen) {
+    chosen[0] = best[0];
+    chosen[1] = best[1];
+  }
   return rc;
 }
 
@@ -497,7 +588,7 @@ int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, c
   MITHRA_TRY(hipEventRecord(d->ev[2], s));
   MITHRA_TRY(launch_boundaries(d));
   MITHRA_TRY(hipEventRecord(d->ev[3], s));
-  MITHRA_TRY(launch_eval_eb(d));
+  MITHRA_TRY(launch_eval_eb_step(d));
   MITHRA_TRY(hipEventRecord(d->ev[4], s));
   MITHRA_TRY(launch_push(d, p, d->und_dev, n_und, time_bunch, bunch_time_step, n_sub, 1));
   MITHRA_TRY(hipEventRecord(d->ev[5], s));
@@ -518,38 +609,57 @@ int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, c
   return MITHRA_OK;
 }
 
+}  // extern "C"
+
 namespace {
-__global__ void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+template <typename T>
+__global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) b[t] = a[t];
 }
-}  // namespace
 
-int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps) {
-  if (!gbps || reps < 1 || bytes < 4096) return MITHRA_E_ARG;
-  const size_t n = bytes / sizeof(double4);
-  double4 *a = nullptr, *b = nullptr;
+template <typename T>
+int copy_rate(unsigned long bytes, int reps, double* gbps) {
+  const size_t n = bytes / sizeof(T);
+  T *a = nullptr, *b = nullptr;
   hipEvent_t e0, e1;
-  if (hipMalloc(&a, n * sizeof(double4)) != hipSuccess || hipMalloc(&b, n * sizeof(double4)) != hipSuccess) {
-    hipFree(a);
+  if (hipMalloc(&a, n * sizeof(T)) != hipSuccess || hipMalloc(&b, n * sizeof(T)) != hipSuccess) {
+    (void)hipFree(a);
     return MITHRA_E_NOMEM;
   }
-  MITHRA_TRY(hipMemset(a, 0, n * sizeof(double4)));
+  MITHRA_TRY(hipMemset(a, 0, n * sizeof(T)));
   MITHRA_TRY(hipEventCreate(&e0));
   MITHRA_TRY(hipEventCreate(&e1));
   const unsigned blocks = 256 * 32;
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);  // warm-up
+  hipLaunchKernelGGL(k_copy<T>, dim3(blocks), dim3(256), 0, 0, a, b, n);  // warm-up
   MITHRA_TRY(hipEventRecord(e0, 0));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy<T>, dim3(blocks), dim3(256), 0, 0, a, b, n);
   MITHRA_TRY(hipEventRecord(e1, 0));
   MITHRA_TRY(hipEventSynchronize(e1));
   float ms = 0;
   MITHRA_TRY(hipEventElapsedTime(&ms, e0, e1));
-  *gbps = 2.0 * n * sizeof(double4) * reps / (ms * 1e-3) / 1e9;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
+  *gbps = 2.0 * n * sizeof(T) * reps / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return MITHRA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps) {
+  return mithra_copy_bandwidth_elem(bytes, reps, 32, gbps);
+}
+
+int mithra_copy_bandwidth_elem(unsigned long bytes, int reps, int elem_bytes, double* gbps) {
+  if (!gbps || reps < 1 || bytes < 4096) return MITHRA_E_ARG;
+  switch (elem_bytes) {
+    case 8: return copy_rate<double>(bytes, reps, gbps);
+    case 16: return copy_rate<double2>(bytes, reps, gbps);
+    case 32: return copy_rate<double4>(bytes, reps, gbps);
+  }
+  return MITHRA_E_ARG;
 }
 
 }  // extern "C"

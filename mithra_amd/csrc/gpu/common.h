@@ -66,6 +66,15 @@ struct Domain {
   int order1_is_rnp;        // ord[1] lists the current rnp (true right after a deposit)
   double* part_alt;         // target of a reorder, swapped with part
   unsigned* pid_alt;
+  int tile_interior;        // tile shape of the interior update (index into kTileShapes)
+  int tile_eval;            // tile shape of the E/B evaluation
+  // lazy E/B evaluation: only the planes the bunch (z cells zrange[0..1], device) and the power
+  // sampler (planes pk_lo .. pk_hi + 1) read
+  int lazy_fields;
+  int* zrange;              // device: bunch z-cell extent after the last push
+  void* zpart;              // device: per-workgroup partial extents
+  int zrange_valid;         // zrange describes the current particle positions
+  int pk_lo, pk_hi;         // power planes' cells (pk_lo > pk_hi: none)
   int zero_pending;         // the A^{n+1} buffer is logically zero (current_reset), not yet written
   int sort_interval;        // deposits between reorders of the particle arrays (0: never)
   long sort_count;          // deposits since the particles were set
@@ -84,6 +93,18 @@ struct Domain {
   int* plane_of;            // host: for each of N planes, its index among inside planes or -1
 };
 
+// Correctly rounded x / d given r = RN(1 / d): q = RN(x r) is within one ulp of x / d, and one
+// residual step with exact FMA residual rounds it correctly (Markstein, IBM J. Res. Dev. 34,
+// 1990, theorem 1).  Valid while the operands and quotient are normal numbers, which holds for
+// the mesh coordinates and potentials divided here by fixed steps; an exact quotient (zero
+// residual) is returned as q, so the sign of a zero quotient is the IEEE one.  Three
+// instructions instead of the general division sequence, same result bit for bit.
+__device__ inline double div_rn(double x, double d, double r) {
+  const double q = x * r;
+  const double e = fma(-q, d, x);
+  return e == 0.0 ? q : fma(e, r, q);
+}
+
 inline double* particle_array(Domain* d, int which) { return d->part + (size_t)which * d->cap; }
 
 // Launch helpers implemented in field.hip, particles.hip and power.hip.
@@ -91,6 +112,12 @@ hipError_t launch_interior(Domain* d);
 hipError_t launch_tfsf(Domain* d, double time, const mithra_seed* seed);
 hipError_t launch_boundaries(Domain* d);
 hipError_t launch_eval_eb(Domain* d);
+hipError_t launch_eval_eb_range(Domain* d, int klo, int khi);  // planes klo..khi (clipped to 1..N2-2)
+hipError_t launch_eval_eb_step(Domain* d);  // all planes, or the lazy subset when enabled and known
+hipError_t launch_zrange(Domain* d);
+// Tile shapes {rows, planes} of the marched field kernels, selected by Domain::tile_*.
+constexpr int kTileVariants = 10;  // 7..9: LDS-staged interior update
+extern const int kTileShapes[kTileVariants][2];
 hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes);
 hipError_t launch_bunch_begin(Domain* d);
 hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,

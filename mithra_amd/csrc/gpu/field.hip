@@ -4,6 +4,7 @@
 // All arithmetic keeps the reference's source order and the library is built with
 // -ffp-contract=off, so on identical inputs the FP64 path reproduces the reference's
 // potentials bit for bit.
+#include <algorithm>
 #include <cmath>
 
 #include "common.h"
@@ -13,8 +14,6 @@ namespace mithra {
 namespace {
 
 constexpr int kTileJ = 64;      // one wavefront along y (contiguous in memory)
-constexpr int kTileI = 4;       // rows of x per workgroup
-constexpr int kChunkK = 64;     // z planes marched by one workgroup
 constexpr int kNumXcd = 8;      // MI355X: 8 accelerator complex dies, each with its own L2
 
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b % 8).
@@ -47,8 +46,8 @@ __device__ inline Plane5 load_plane(const R* __restrict__ a, long m, long N1) {
 // and marches kChunkK planes along z with the three planes it needs held in registers; the
 // x/y neighbours it reads are the centres of neighbouring lanes and rows, served by L1/L2.
 // J (or rho) lives in the A^{n+1} buffer and is overwritten in place, as in the reference.
-template <typename R, bool NSFD>
-__global__ __launch_bounds__(kTileJ* kTileI) void k_interior(
+template <typename R, bool NSFD, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_interior(
     FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
     int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
     double alpha, double beta) {
@@ -60,9 +59,9 @@ __global__ __launch_bounds__(kTileJ* kTileI) void k_interior(
   const int kc = lb % chunks;
   const int comp = lb / chunks;
   const int j = tj * kTileJ + (threadIdx.x % kTileJ);
-  const int i = 1 + ti * kTileI + (threadIdx.x / kTileJ);
-  const int kb = 1 + kc * kChunkK;
-  const int ke = min(kb + kChunkK, N2 - 1);
+  const int i = 1 + ti * TI + (threadIdx.x / kTileJ);
+  const int kb = 1 + kc * K;
+  const int ke = min(kb + K, N2 - 1);
   if (comp >= ncomp || i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
   const R* __restrict__ an = (const R*)an_set.c[comp];
   const R* __restrict__ anm1 = (const R*)anm1_set.c[comp];
@@ -87,6 +86,99 @@ __global__ __launch_bounds__(kTileJ* kTileI) void k_interior(
     anp1[m] = (R)r;
     pm = p0;
     p0 = pp;
+  }
+}
+
+// The interior update with the A^n planes staged in LDS (2.5-D blocking).  The workgroup (TI
+// wavefronts, one x-row of 64 y-nodes each) marches its K planes in lockstep: plane k+1 of its
+// tile plus a one-node halo ring is fetched once, coalesced, into a three-plane LDS ring, and
+// every neighbour of the stencil is read from LDS.  Same arithmetic, in the same order, as
+// k_interior.
+template <typename R, bool NSFD, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_interior_lds(
+    FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
+    int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
+    double alpha, double beta) {
+  constexpr int W = kTileJ + 2;           // LDS row: y-halo, 64 nodes, y-halo
+  constexpr int P = (TI + 2) * W;         // one plane of the tile with its halo ring
+  __shared__ double L[3 * P];
+  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tj = lb % tilesJ;
+  lb /= tilesJ;
+  const int ti = lb % tilesI;
+  lb /= tilesI;
+  const int kc = lb % chunks;
+  const int comp = lb / chunks;
+  const int kb = 1 + kc * K;
+  const int ke = min(kb + K, N2 - 1);
+  if (comp >= ncomp || kb >= ke) return;  // uniform over the workgroup
+  const int lane = threadIdx.x % kTileJ, w = threadIdx.x / kTileJ;
+  const int j0 = tj * kTileJ, i0 = 1 + ti * TI;
+  const int j = j0 + lane, i = i0 + w;
+  const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
+  const R* __restrict__ an = (const R*)an_set.c[comp];
+  const R* __restrict__ anm1 = (const R*)anm1_set.c[comp];
+  R* __restrict__ anp1 = (R*)anp1_set.c[comp];
+  const double as = comp < 3 ? a4 : a5;
+  const long N1N0 = (long)N1 * N0;
+  // What this thread stages per plane: its own node, and (wave 0 / wave 1) the halo rows
+  // i0-1 / i0+TI, and (lanes < 2 TI of wave 0) the halo columns j0-1 / j0+64 of rows i0..i0+TI-1.
+  const bool own_ok = i <= N0 - 1 && j <= N1 - 1;
+  const long own_off = (long)N1 * i + j;
+  const int own_slot = (w + 1) * W + lane + 1;
+  int hrow_i = -1, hrow_slot = 0;
+  if (w == 0) { hrow_i = i0 - 1; hrow_slot = lane + 1; }
+  if (w == 1) { hrow_i = i0 + TI; hrow_slot = (TI + 1) * W + lane + 1; }
+  const bool hrow_ok = hrow_i >= 0 && hrow_i <= N0 - 1 && j <= N1 - 1;
+  const long hrow_off = (long)N1 * hrow_i + j;
+  int hcol_j = -1, hcol_i = 0, hcol_slot = 0;
+  if (w == 0 && lane < TI) { hcol_i = i0 + lane; hcol_j = j0 - 1; hcol_slot = (lane + 1) * W; }
+  if (w == 0 && lane >= TI && lane < 2 * TI) {
+    hcol_i = i0 + lane - TI; hcol_j = j0 + kTileJ; hcol_slot = (lane - TI + 1) * W + kTileJ + 1;
+  }
+  const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
+  const long hcol_off = (long)N1 * hcol_i + hcol_j;
+  double r_own = 0.0, r_row = 0.0, r_col = 0.0;
+  auto fetch = [&](int k) {
+    const long b = N1N0 * k;
+    if (own_ok) r_own = (double)an[b + own_off];
+    if (hrow_ok) r_row = (double)an[b + hrow_off];
+    if (hcol_ok) r_col = (double)an[b + hcol_off];
+  };
+  auto stage = [&](int slot) {
+    double* p = L + slot * P;
+    if (own_ok) p[own_slot] = r_own;
+    if (hrow_ok) p[hrow_slot] = r_row;
+    if (hcol_ok) p[hcol_slot] = r_col;
+  };
+  fetch(kb - 1);
+  stage(0);
+  fetch(kb);
+  stage(1);
+  fetch(kb + 1);
+  long m = N1N0 * kb + own_off;
+  for (int k = kb; k < ke; ++k, m += N1N0) {
+    const int s0 = (k - kb + 1) % 3, sp = (k - kb + 2) % 3, sm = (k - kb) % 3;
+    stage(sp);
+    __syncthreads();
+    if (k + 2 <= ke) fetch(k + 2);
+    if (active) {
+      const double v1 = (double)anm1[m];
+      const double src = (double)anp1[m];
+      const double* c0 = L + s0 * P + own_slot;
+      const double* cp = L + sp * P + own_slot;
+      const double* cm = L + sm * P + own_slot;
+      double r;
+      if (NSFD) {
+        double tx = c0[W] + c0[-W] + beta * (cp[W] + cm[W] + cp[-W] + cm[-W]);
+        double ty = c0[1] + c0[-1] + beta * (cp[1] + cm[1] + cp[-1] + cm[-1]);
+        r = a0 * c0[0] - v1 + alpha * (a1 * tx + a2 * ty) + a3 * (cp[0] + cm[0]) + as * src;
+      } else {
+        r = a0 * c0[0] - v1 + a1 * (c0[W] + c0[-W]) + a2 * (c0[1] + c0[-1]) + a3 * (cp[0] + cm[0]) + as * src;
+      }
+      anp1[m] = (R)r;
+    }
+    __syncthreads();
   }
 }
 
@@ -359,49 +451,229 @@ __global__ void k_mur_corners(FieldSet P, FieldSet N, FieldSet M, int ncomp, int
 // keeps the float rounding after each term: that is what the reference computes as compiled by
 // g++ -O3 (its vectorised x/y lanes drop the intermediate float round trip of
 // FieldVector<float>::dv/mdv, src/fieldvector.h:87-108), and the parity fixtures record it.
-template <typename R, bool SC>
-__global__ __launch_bounds__(256) void k_eval_eb(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1,
-                                                 int N2, double dt, double dx2, double dy2, double dz2) {
-  const int j = blockIdx.x * 64 + (threadIdx.x % 64);
-  const int i = blockIdx.y * 4 + (threadIdx.x / 64);
-  const int k = blockIdx.z;
-  if (j < 1 || j > N1 - 2 || i < 1 || i > N0 - 2 || k < 1 || k > N2 - 2) return;
+// Lazy evaluation (the reference evaluates E/B only where particles and samplers read them,
+// src/solver.h:241, src/fdtd.cpp:818-845): planes kb..ke-1 are needed if they meet the nodes the
+// bunch reads during the coming push (its cells zr[0]..zr[1] at the start, at most one cell of
+// motion per field step, nodes k and k+1 of a cell) or the power planes pb0..pb1.
+__device__ inline bool lazy_needed(int kb, int ke, const int* zr, int pb0, int pb1) {
+  const int a0 = zr[0] - 2, a1 = zr[1] + 2;
+  return (ke - 1 >= a0 && kb <= a1) || (ke - 1 >= pb0 && kb <= pb1);
+}
+
+struct EvalConst {  // -dt, twice the steps, and their correctly rounded reciprocals (div_rn)
+  double mdt, dx2, dy2, dz2, rmdt, rdx2, rdy2, rdz2;
+};
+
+// Each thread owns one (i, j) column and marches K planes; the z neighbours of A_x and A_y (and
+// phi) at both time levels ride in a three-plane register queue, the transverse neighbours are
+// the centres of neighbouring lanes and rows.  Planes outside [klo, khi] are skipped.
+template <typename R, bool SC, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
+                                                        int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
+                                                        EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
+  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tj = lb % tilesJ;
+  lb /= tilesJ;
+  const int ti = lb % tilesI;
+  const int kc = lb / tilesI;
+  const int j = tj * kTileJ + (threadIdx.x % kTileJ);
+  const int i = 1 + ti * TI + (threadIdx.x / kTileJ);
+  const int kb = klo + kc * K;
+  const int ke = min(kb + K, khi + 1);
+  if (zr && !lazy_needed(kb, ke, zr, pb0, pb1)) return;
+  if (i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
   const long N1N0 = (long)N1 * N0;
-  const long m = N1N0 * k + (long)N1 * i + j;
-  const R* n0 = (const R*)An.c[0];
-  const R* n1 = (const R*)An.c[1];
-  const R* n2 = (const R*)An.c[2];
-  const R* p0 = (const R*)Ap.c[0];
-  const R* p1 = (const R*)Ap.c[1];
-  const R* p2 = (const R*)Ap.c[2];
-  const double mdt = -dt;
-  double ex = rd(p0, m) / mdt - rd(n0, m) / mdt;
-  double ey = rd(p1, m) / mdt - rd(n1, m) / mdt;
-  float ez = (float)(rd(p2, m) / mdt);
-  ez = (float)((double)ez - rd(n2, m) / mdt);
-  if (SC) {
-    const R* f = (const R*)An.c[3];
-    ex = ex - (rd(f, m + N1) - rd(f, m - N1)) / dx2;
-    ey = ey - (rd(f, m + 1) - rd(f, m - 1)) / dy2;
-    ez = (float)((double)ez - (rd(f, m + N1N0) - rd(f, m - N1N0)) / dz2);
+  const R* __restrict__ n0 = (const R*)An.c[0];
+  const R* __restrict__ n1 = (const R*)An.c[1];
+  const R* __restrict__ n2 = (const R*)An.c[2];
+  const R* __restrict__ p0 = (const R*)Ap.c[0];
+  const R* __restrict__ p1 = (const R*)Ap.c[1];
+  const R* __restrict__ p2 = (const R*)Ap.c[2];
+  const R* __restrict__ f = SC ? (const R*)An.c[3] : nullptr;
+  const double mdt = q.mdt, dx2 = q.dx2, dy2 = q.dy2, dz2 = q.dz2;
+  auto dt_ = [&](double x) { return div_rn(x, mdt, q.rmdt); };
+  auto dx_ = [&](double x) { return div_rn(x, dx2, q.rdx2); };
+  auto dy_ = [&](double x) { return div_rn(x, dy2, q.rdy2); };
+  auto dz_ = [&](double x) { return div_rn(x, dz2, q.rdz2); };
+  long m = N1N0 * kb + (long)N1 * i + j;
+  // queue: [0] plane k-1, [1] plane k, [2] plane k+1
+  double qn0[3], qn1[3], qp0[3], qp1[3], qf[3];
+  for (int t = 0; t < 2; ++t) {
+    const long mm = m + (t - 1) * N1N0;
+    qn0[t] = rd(n0, mm); qn1[t] = rd(n1, mm); qp0[t] = rd(p0, mm); qp1[t] = rd(p1, mm);
+    if (SC) qf[t] = rd(f, mm);
   }
-  const float bx = (float)(0.5 * ((rd(n2, m + 1) - rd(n2, m - 1)) / dy2 - (rd(n1, m + N1N0) - rd(n1, m - N1N0)) / dz2 +
-                                  (rd(p2, m + 1) - rd(p2, m - 1)) / dy2 - (rd(p1, m + N1N0) - rd(p1, m - N1N0)) / dz2));
-  const float by = (float)(0.5 * ((rd(n0, m + N1N0) - rd(n0, m - N1N0)) / dz2 - (rd(n2, m + N1) - rd(n2, m - N1)) / dx2 +
-                                  (rd(p0, m + N1N0) - rd(p0, m - N1N0)) / dz2 - (rd(p2, m + N1) - rd(p2, m - N1)) / dx2));
-  const float bz = (float)(0.5 * ((rd(n1, m + N1) - rd(n1, m - N1)) / dx2 - (rd(n0, m + 1) - rd(n0, m - 1)) / dy2 +
-                                  (rd(p1, m + N1) - rd(p1, m - N1)) / dx2 - (rd(p0, m + 1) - rd(p0, m - 1)) / dy2));
-  const float4 v0 = make_float4((float)ex, (float)ey, ez, bx);
-  const float4 v1 = make_float4(by, bz, 0.0f, 0.0f);
-  eb[2 * m] = v0;
-  eb[2 * m + 1] = v1;
-  if (k == 1) {
-    eb[2 * (m - N1N0)] = v0;
-    eb[2 * (m - N1N0) + 1] = v1;
+  for (int k = kb; k < ke; ++k, m += N1N0) {
+    qn0[2] = rd(n0, m + N1N0); qn1[2] = rd(n1, m + N1N0); qp0[2] = rd(p0, m + N1N0); qp1[2] = rd(p1, m + N1N0);
+    if (SC) qf[2] = rd(f, m + N1N0);
+    double ex = dt_(qp0[1]) - dt_(qn0[1]);
+    double ey = dt_(qp1[1]) - dt_(qn1[1]);
+    float ez = (float)dt_(rd(p2, m));
+    ez = (float)((double)ez - dt_(rd(n2, m)));
+    if (SC) {
+      ex = ex - dx_(rd(f, m + N1) - rd(f, m - N1));
+      ey = ey - dy_(rd(f, m + 1) - rd(f, m - 1));
+      ez = (float)((double)ez - dz_(qf[2] - qf[0]));
+    }
+    const float bx = (float)(0.5 * (dy_(rd(n2, m + 1) - rd(n2, m - 1)) - dz_(qn1[2] - qn1[0]) +
+                                    dy_(rd(p2, m + 1) - rd(p2, m - 1)) - dz_(qp1[2] - qp1[0])));
+    const float by = (float)(0.5 * (dz_(qn0[2] - qn0[0]) - dx_(rd(n2, m + N1) - rd(n2, m - N1)) +
+                                    dz_(qp0[2] - qp0[0]) - dx_(rd(p2, m + N1) - rd(p2, m - N1))));
+    const float bz = (float)(0.5 * (dx_(rd(n1, m + N1) - rd(n1, m - N1)) - dy_(rd(n0, m + 1) - rd(n0, m - 1)) +
+                                    dx_(rd(p1, m + N1) - rd(p1, m - N1)) - dy_(rd(p0, m + 1) - rd(p0, m - 1))));
+    const float4 v0 = make_float4((float)ex, (float)ey, ez, bx);
+    const float4 v1 = make_float4(by, bz, 0.0f, 0.0f);
+    eb[2 * m] = v0;
+    eb[2 * m + 1] = v1;
+    if (k == 1) {
+      eb[2 * (m - N1N0)] = v0;
+      eb[2 * (m - N1N0) + 1] = v1;
+    }
+    if (k == N2 - 2) {
+      eb[2 * (m + N1N0)] = v0;
+      eb[2 * (m + N1N0) + 1] = v1;
+    }
+    qn0[0] = qn0[1]; qn1[0] = qn1[1]; qp0[0] = qp0[1]; qp1[0] = qp1[1];
+    qn0[1] = qn0[2]; qn1[1] = qn1[2]; qp0[1] = qp0[2]; qp1[1] = qp1[2];
+    if (SC) { qf[0] = qf[1]; qf[1] = qf[2]; }
   }
-  if (k == N2 - 2) {
-    eb[2 * (m + N1N0)] = v0;
-    eb[2 * (m + N1N0) + 1] = v1;
+}
+
+// The E/B evaluation with plane k of A^n and A^{n+1} staged in LDS (2.5-D blocking, as
+// k_interior_lds): the workgroup marches its planes in lockstep, every transverse neighbour is
+// read from LDS, the z neighbours of A_x, A_y (and phi) ride in a register queue, and the
+// next plane is fetched while the current one is computed.  Same expressions as k_eval_eb.
+template <typename R, bool SC, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
+                                                         int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
+                                                         EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
+  constexpr int NA = SC ? 7 : 6;      // staged arrays: n0 n1 n2 p0 p1 p2 [phi]
+  constexpr int NZ = SC ? 5 : 4;      // arrays with z neighbours: n0 n1 p0 p1 [phi]
+  constexpr int W = kTileJ + 2, P = (TI + 2) * W;
+  __shared__ double L[NA * P];
+  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tj = lb % tilesJ;
+  lb /= tilesJ;
+  const int ti = lb % tilesI;
+  const int kc = lb / tilesI;
+  const int kb = klo + kc * K;
+  const int ke = min(kb + K, khi + 1);
+  if (kb >= ke) return;  // uniform over the workgroup
+  if (zr && !lazy_needed(kb, ke, zr, pb0, pb1)) return;
+  const int lane = threadIdx.x % kTileJ, w = threadIdx.x / kTileJ;
+  const int j0 = tj * kTileJ, i0 = 1 + ti * TI;
+  const int j = j0 + lane, i = i0 + w;
+  const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
+  const long N1N0 = (long)N1 * N0;
+  const R* A[7] = {(const R*)An.c[0], (const R*)An.c[1], (const R*)An.c[2], (const R*)Ap.c[0],
+                   (const R*)Ap.c[1], (const R*)Ap.c[2], SC ? (const R*)An.c[3] : nullptr};
+  constexpr int zarr[5] = {0, 1, 3, 4, 6};  // n0 n1 p0 p1 phi
+  const bool own_ok = i <= N0 - 1 && j <= N1 - 1;
+  const long own_off = (long)N1 * i + j;
+  const int own_slot = (w + 1) * W + lane + 1;
+  int hrow_i = -1, hrow_slot = 0;
+  if (w == 0) { hrow_i = i0 - 1; hrow_slot = lane + 1; }
+  if (w == 1) { hrow_i = i0 + TI; hrow_slot = (TI + 1) * W + lane + 1; }
+  const bool hrow_ok = hrow_i >= 0 && hrow_i <= N0 - 1 && j <= N1 - 1;
+  const long hrow_off = (long)N1 * hrow_i + j;
+  int hcol_j = -1, hcol_i = 0, hcol_slot = 0;
+  if (w == 0 && lane < TI) { hcol_i = i0 + lane; hcol_j = j0 - 1; hcol_slot = (lane + 1) * W; }
+  if (w == 0 && lane >= TI && lane < 2 * TI) {
+    hcol_i = i0 + lane - TI; hcol_j = j0 + kTileJ; hcol_slot = (lane - TI + 1) * W + kTileJ + 1;
+  }
+  const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
+  const long hcol_off = (long)N1 * hcol_i + hcol_j;
+  // qz[a][s]: own value of z-array a at planes k-1, k, k+1, k+2 (s = 0..3) of iteration k
+  double qz[NZ][4];
+  double pre_n2 = 0.0, pre_p2 = 0.0, pre_row[NA], pre_col[NA];
+  auto fetch_z = [&](int k, int s) {
+#pragma unroll
+    for (int a = 0; a < NZ; ++a) qz[a][s] = own_ok ? (double)A[zarr[a]][N1N0 * k + own_off] : 0.0;
+  };
+  auto fetch_plane = [&](int k) {  // what plane k stages besides the queued own values
+    const long b = N1N0 * k;
+    if (own_ok) {
+      pre_n2 = (double)A[2][b + own_off];
+      pre_p2 = (double)A[5][b + own_off];
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      if (hrow_ok) pre_row[a] = (double)A[a][b + hrow_off];
+      if (hcol_ok) pre_col[a] = (double)A[a][b + hcol_off];
+    }
+  };
+  fetch_z(kb - 1, 0);
+  fetch_z(kb, 1);
+  fetch_z(kb + 1, 2);
+  fetch_plane(kb);
+  const double mdt = q.mdt, dx2 = q.dx2, dy2 = q.dy2, dz2 = q.dz2;
+  auto dt_ = [&](double x) { return div_rn(x, mdt, q.rmdt); };
+  auto dx_ = [&](double x) { return div_rn(x, dx2, q.rdx2); };
+  auto dy_ = [&](double x) { return div_rn(x, dy2, q.rdy2); };
+  auto dz_ = [&](double x) { return div_rn(x, dz2, q.rdz2); };
+  long m = N1N0 * kb + own_off;
+  for (int k = kb; k < ke; ++k, m += N1N0) {
+    // stage plane k
+    if (own_ok) {
+      L[0 * P + own_slot] = qz[0][1];
+      L[1 * P + own_slot] = qz[1][1];
+      L[2 * P + own_slot] = pre_n2;
+      L[3 * P + own_slot] = qz[2][1];
+      L[4 * P + own_slot] = qz[3][1];
+      L[5 * P + own_slot] = pre_p2;
+      if (SC) L[6 * P + own_slot] = qz[NZ - 1][1];
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      if (hrow_ok) L[a * P + hrow_slot] = pre_row[a];
+      if (hcol_ok) L[a * P + hcol_slot] = pre_col[a];
+    }
+    __syncthreads();
+    if (k + 1 < ke) fetch_plane(k + 1);
+    if (k + 2 <= ke) fetch_z(k + 2, 3);
+    if (active) {
+      const double* n1l = L + 1 * P + own_slot;
+      const double* n2l = L + 2 * P + own_slot;
+      const double* p1l = L + 4 * P + own_slot;
+      const double* p2l = L + 5 * P + own_slot;
+      const double* n0l = L + 0 * P + own_slot;
+      const double* p0l = L + 3 * P + own_slot;
+      double ex = dt_(qz[2][1]) - dt_(qz[0][1]);
+      double ey = dt_(qz[3][1]) - dt_(qz[1][1]);
+      float ez = (float)dt_(p2l[0]);
+      ez = (float)((double)ez - dt_(n2l[0]));
+      if (SC) {
+        const double* fl = L + 6 * P + own_slot;
+        ex = ex - dx_(fl[W] - fl[-W]);
+        ey = ey - dy_(fl[1] - fl[-1]);
+        ez = (float)((double)ez - dz_(qz[NZ - 1][2] - qz[NZ - 1][0]));
+      }
+      const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_(qz[1][2] - qz[1][0]) +
+                                      dy_(p2l[1] - p2l[-1]) - dz_(qz[3][2] - qz[3][0])));
+      const float by = (float)(0.5 * (dz_(qz[0][2] - qz[0][0]) - dx_(n2l[W] - n2l[-W]) +
+                                      dz_(qz[2][2] - qz[2][0]) - dx_(p2l[W] - p2l[-W])));
+      const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
+                                      dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
+      const float4 v0 = make_float4((float)ex, (float)ey, ez, bx);
+      const float4 v1 = make_float4(by, bz, 0.0f, 0.0f);
+      eb[2 * m] = v0;
+      eb[2 * m + 1] = v1;
+      if (k == 1) {
+        eb[2 * (m - N1N0)] = v0;
+        eb[2 * (m - N1N0) + 1] = v1;
+      }
+      if (k == N2 - 2) {
+        eb[2 * (m + N1N0)] = v0;
+        eb[2 * (m + N1N0) + 1] = v1;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NZ; ++a) {
+      qz[a][0] = qz[a][1];
+      qz[a][1] = qz[a][2];
+      qz[a][2] = qz[a][3];
+    }
   }
 }
 
@@ -415,23 +687,78 @@ __global__ void k_fill_zero_bytes(unsigned char* p, size_t n) {
   if (t < n) p[t] = 0;
 }
 
-template <typename R>
-hipError_t interior_impl(Domain* d) {
+// Tile shapes of the marched field kernels: rows of x per workgroup (one wavefront of 64 y
+// nodes per row) and z planes per workgroup.  Every shape computes the same values; the
+// autotuner (capi) picks the fastest per domain.
+template <typename R, bool NSFD, int TI, int K, bool LDS = false>
+hipError_t interior_launch(Domain* d) {
   const mithra_grid& g = d->g;
   const int tilesJ = (g.N1 + kTileJ - 1) / kTileJ;
-  const int tilesI = (g.N0 - 2 + kTileI - 1) / kTileI;
-  const int chunks = (g.N2 - 2 + kChunkK - 1) / kChunkK;
+  const int tilesI = (g.N0 - 2 + TI - 1) / TI;
+  const int chunks = (g.N2 - 2 + K - 1) / K;
   const long nblk = (long)tilesJ * tilesI * chunks * d->ncomp;
   const double* a = d->co.a;
-  if (g.solver == MITHRA_SOLVER_NSFD)
-    hipLaunchKernelGGL((k_interior<R, true>), dim3((unsigned)nblk), dim3(kTileJ * kTileI), 0, d->stream,
+  if constexpr (LDS)
+    hipLaunchKernelGGL((k_interior_lds<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
                        d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
                        chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
   else
-    hipLaunchKernelGGL((k_interior<R, false>), dim3((unsigned)nblk), dim3(kTileJ * kTileI), 0, d->stream,
+    hipLaunchKernelGGL((k_interior<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
                        d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
                        chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
   return hipGetLastError();
+}
+
+template <typename R, bool SC, int TI, int K, bool LDS = false>
+hipError_t eval_launch(Domain* d, int klo, int khi, const int* zr, int pb0, int pb1) {
+  const mithra_grid& g = d->g;
+  const int tilesJ = (g.N1 + kTileJ - 1) / kTileJ;
+  const int tilesI = (g.N0 - 2 + TI - 1) / TI;
+  const int chunks = (khi - klo + 1 + K - 1) / K;
+  const long nblk = (long)tilesJ * tilesI * chunks;
+  EvalConst q;
+  q.mdt = -g.dt; q.dx2 = 2.0 * g.dx; q.dy2 = 2.0 * g.dy; q.dz2 = 2.0 * g.dz;
+  q.rmdt = 1.0 / q.mdt; q.rdx2 = 1.0 / q.dx2; q.rdy2 = 1.0 / q.dy2; q.rdz2 = 1.0 / q.dz2;
+  if constexpr (LDS)
+    hipLaunchKernelGGL((k_eval_lds<R, SC, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
+                       d->lvl[d->i_n], d->lvl[d->i_np1], d->eb, g.N0, g.N1, g.N2, tilesJ, tilesI, klo, khi, q, zr, pb0, pb1);
+  else
+    hipLaunchKernelGGL((k_eval_eb<R, SC, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
+                       d->lvl[d->i_n], d->lvl[d->i_np1], d->eb, g.N0, g.N1, g.N2, tilesJ, tilesI, klo, khi, q, zr, pb0, pb1);
+  return hipGetLastError();
+}
+
+template <typename R, bool FLAG>
+hipError_t field_variant(Domain* d, int kernel, int v, int klo, int khi, const int* zr = nullptr, int pb0 = 0,
+                         int pb1 = -1) {
+  // kernel 0: interior (FLAG = NSFD), kernel 1: E/B evaluation (FLAG = space charge)
+#define MITHRA_VARIANT(idx, TI, K)                                                            \
+  case idx:                                                                                    \
+    return kernel == 0 ? interior_launch<R, FLAG, TI, K>(d) : eval_launch<R, FLAG, TI, K>(d, klo, khi, zr, pb0, pb1);
+  switch (v) {
+    MITHRA_VARIANT(0, 4, 64)
+    MITHRA_VARIANT(1, 2, 64)
+    MITHRA_VARIANT(2, 4, 32)
+    MITHRA_VARIANT(3, 4, 128)
+    MITHRA_VARIANT(4, 8, 32)
+    MITHRA_VARIANT(5, 8, 64)
+    MITHRA_VARIANT(6, 16, 32)
+  }
+  static_assert(kTileVariants == 10, "variant table and switch disagree");
+  // variants 7..9: the LDS-staged (2.5-D blocked) forms of both kernels
+  switch (v) {
+    case 7: return kernel == 0 ? interior_launch<R, FLAG, 4, 64, true>(d) : eval_launch<R, FLAG, 4, 64, true>(d, klo, khi, zr, pb0, pb1);
+    case 8: return kernel == 0 ? interior_launch<R, FLAG, 8, 64, true>(d) : eval_launch<R, FLAG, 8, 64, true>(d, klo, khi, zr, pb0, pb1);
+    case 9: return kernel == 0 ? interior_launch<R, FLAG, 16, 64, true>(d) : eval_launch<R, FLAG, 16, 64, true>(d, klo, khi, zr, pb0, pb1);
+  }
+#undef MITHRA_VARIANT
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t interior_impl(Domain* d) {
+  return d->g.solver == MITHRA_SOLVER_NSFD ? field_variant<R, true>(d, 0, d->tile_interior, 0, 0)
+                                           : field_variant<R, false>(d, 0, d->tile_interior, 0, 0);
 }
 
 template <typename R>
@@ -474,20 +801,18 @@ hipError_t boundaries_impl(Domain* d) {
 }
 
 template <typename R>
-hipError_t eval_impl(Domain* d) {
-  const mithra_grid& g = d->g;
-  dim3 grid((g.N1 + 63) / 64, (g.N0 + 3) / 4, g.N2);
-  const double dx2 = 2.0 * g.dx, dy2 = 2.0 * g.dy, dz2 = 2.0 * g.dz;
-  if (g.space_charge)
-    hipLaunchKernelGGL((k_eval_eb<R, true>), grid, dim3(256), 0, d->stream, d->lvl[d->i_n], d->lvl[d->i_np1], d->eb,
-                       g.N0, g.N1, g.N2, g.dt, dx2, dy2, dz2);
-  else
-    hipLaunchKernelGGL((k_eval_eb<R, false>), grid, dim3(256), 0, d->stream, d->lvl[d->i_n], d->lvl[d->i_np1], d->eb,
-                       g.N0, g.N1, g.N2, g.dt, dx2, dy2, dz2);
-  return hipGetLastError();
+hipError_t eval_impl(Domain* d, int klo, int khi, const int* zr = nullptr, int pb0 = 0, int pb1 = -1) {
+  klo = std::max(klo, 1);
+  khi = std::min(khi, d->g.N2 - 2);
+  if (klo > khi) return hipSuccess;
+  return d->g.space_charge ? field_variant<R, true>(d, 1, d->tile_eval, klo, khi, zr, pb0, pb1)
+                           : field_variant<R, false>(d, 1, d->tile_eval, klo, khi, zr, pb0, pb1);
 }
 
 }  // namespace
+
+extern const int kTileShapes[kTileVariants][2] = {{4, 64}, {2, 64}, {4, 32}, {4, 128}, {8, 32},
+                                                  {8, 64}, {16, 32}, {4, 64}, {8, 64}, {16, 64}};
 
 hipError_t launch_interior(Domain* d) {
   return d->precision == MITHRA_PRECISION_FP64 ? interior_impl<double>(d) : interior_impl<float>(d);
@@ -501,8 +826,17 @@ hipError_t launch_boundaries(Domain* d) {
   return d->precision == MITHRA_PRECISION_FP64 ? boundaries_impl<double>(d) : boundaries_impl<float>(d);
 }
 
-hipError_t launch_eval_eb(Domain* d) {
-  return d->precision == MITHRA_PRECISION_FP64 ? eval_impl<double>(d) : eval_impl<float>(d);
+hipError_t launch_eval_eb(Domain* d) { return launch_eval_eb_range(d, 1, d->g.N2 - 2); }
+
+hipError_t launch_eval_eb_step(Domain* d) {
+  if (!d->lazy_fields || !d->zrange_valid) return launch_eval_eb(d);
+  const int pb0 = d->pk_lo, pb1 = d->pk_lo <= d->pk_hi ? d->pk_hi + 1 : -1;
+  return d->precision == MITHRA_PRECISION_FP64 ? eval_impl<double>(d, 1, d->g.N2 - 2, d->zrange, pb0, pb1)
+                                               : eval_impl<float>(d, 1, d->g.N2 - 2, d->zrange, pb0, pb1);
+}
+
+hipError_t launch_eval_eb_range(Domain* d, int klo, int khi) {
+  return d->precision == MITHRA_PRECISION_FP64 ? eval_impl<double>(d, klo, khi) : eval_impl<float>(d, klo, khi);
 }
 
 hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes) {

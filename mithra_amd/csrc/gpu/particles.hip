@@ -11,6 +11,7 @@ namespace {
 struct PushConst {
   // mesh
   double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz, Lz;
+  double ix, iy, iz;  // RN(1/dx), RN(1/dy), RN(1/dz) for div_rn
   int N0, N1, N2;
   // frame and Boris constants
   double gamma, beta, c0, dt_shift, r1, r2, dtb;
@@ -103,9 +104,9 @@ __global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long ca
     if (e == 1.0) {
       if (b1x && b1y && b1z) {
         double ip, jp, kp;
-        const double dxr = modf((x - c.xmin) / c.dx, &ip);
-        const double dyr = modf((y - c.ymin) / c.dy, &jp);
-        const double dzr = modf((z - c.zmin) / c.dz, &kp);
+        const double dxr = modf(div_rn(x - c.xmin, c.dx, c.ix), &ip);
+        const double dyr = modf(div_rn(y - c.ymin, c.dy, c.iy), &jp);
+        const double dzr = modf(div_rn(z - c.zmin, c.dz, c.iz), &kp);
         const int ci = (int)ip, cj = (int)jp, ck = (int)kp;
         // The domain test above keeps the cell inside the mesh in exact arithmetic; a quotient
         // rounded up to the last node would index one plane past the mesh (out of bounds in the
@@ -186,6 +187,56 @@ __global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long ca
   if (ncross) atomicAdd(&counters[1], (unsigned long long)ncross);
 }
 
+// z extent of the bunch in cells (clamped to the mesh), for the lazy E/B evaluation: per
+// workgroup min / max, then one workgroup reduces the partials into zr[0..1].
+constexpr int kZBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_zcells(const double* __restrict__ part, long cap, long n, double zmin,
+                                                double dz, double iz, int kmax_cell, int2* __restrict__ partial) {
+  __shared__ int s_lo[256], s_hi[256];
+  int lo = 0x7fffffff, hi = -1;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+    const double zz = floor(div_rn(part[PZ * cap + p] - zmin, dz, iz));
+    const int k = zz >= 0.0 ? (zz <= (double)kmax_cell ? (int)zz : kmax_cell) : 0;
+    lo = min(lo, k);
+    hi = max(hi, k);
+  }
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_lo[threadIdx.x] = min(s_lo[threadIdx.x], s_lo[threadIdx.x + h]);
+      s_hi[threadIdx.x] = max(s_hi[threadIdx.x], s_hi[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_int2(s_lo[0], s_hi[0]);
+}
+
+__global__ __launch_bounds__(256) void k_zcells_final(const int2* __restrict__ partial, int nb, int* __restrict__ zr) {
+  __shared__ int s_lo[256], s_hi[256];
+  int lo = 0x7fffffff, hi = -1;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    lo = min(lo, partial[b].x);
+    hi = max(hi, partial[b].y);
+  }
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_lo[threadIdx.x] = min(s_lo[threadIdx.x], s_lo[threadIdx.x + h]);
+      s_hi[threadIdx.x] = max(s_hi[threadIdx.x], s_hi[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    zr[0] = s_lo[0];
+    zr[1] = s_hi[0];
+  }
+}
+
 __global__ void k_bunch_begin(double* __restrict__ part, long cap, long n) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -196,6 +247,7 @@ __global__ void k_bunch_begin(double* __restrict__ part, long cap, long n) {
 
 struct DepositConst {
   double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz;
+  double ix, iy, iz;        // RN(1/dx), RN(1/dy), RN(1/dz) for div_rn
   int N0, N1, N2, sc;
 };
 
@@ -223,9 +275,9 @@ __device__ inline void add(R* p, double v) {
 __device__ inline void half_weights(const DepositConst& c, double q, const double a[3], const double r[3],
                                     const double jc[3], double* v) {
   double cc;
-  const double fx = modf((0.5 * (a[0] + r[0]) - c.xmin) / c.dx, &cc);
-  const double fy = modf((0.5 * (a[1] + r[1]) - c.ymin) / c.dy, &cc);
-  const double fz = modf((0.5 * (a[2] + r[2]) - c.zmin) / c.dz, &cc);
+  const double fx = modf(div_rn(0.5 * (a[0] + r[0]) - c.xmin, c.dx, c.ix), &cc);
+  const double fy = modf(div_rn(0.5 * (a[1] + r[1]) - c.ymin, c.dy, c.iy), &cc);
+  const double fz = modf(div_rn(0.5 * (a[2] + r[2]) - c.zmin, c.dz, c.iz), &cc);
   const double x1 = 1.0 - fx, x2 = fx, y1 = 1.0 - fy, y2 = fy, z1 = 1.0 - fz, z2 = fz;
   const double h = q * 0.5;
   v[0] = h * y1 * z1 * jc[0]; v[1] = h * y2 * z1 * jc[0]; v[2] = h * y1 * z2 * jc[0]; v[3] = h * y2 * z2 * jc[0];
@@ -236,9 +288,9 @@ __device__ inline void half_weights(const DepositConst& c, double q, const doubl
 // Charge density weights at rnp (src/fdtdSC.cpp:141-160), node order o[0..7].
 __device__ inline void rho_weights(const DepositConst& c, double q, const double rp[3], double* v) {
   double cc;
-  const double fx = modf((rp[0] - c.xmin) / c.dx, &cc);
-  const double fy = modf((rp[1] - c.ymin) / c.dy, &cc);
-  const double fz = modf((rp[2] - c.zmin) / c.dz, &cc);
+  const double fx = modf(div_rn(rp[0] - c.xmin, c.dx, c.ix), &cc);
+  const double fy = modf(div_rn(rp[1] - c.ymin, c.dy, c.iy), &cc);
+  const double fz = modf(div_rn(rp[2] - c.zmin, c.dz, c.iz), &cc);
   const double x1 = 1.0 - fx, x2 = fx, y1 = 1.0 - fy, y2 = fy, z1 = 1.0 - fz, z2 = fz;
   v[0] = q * x1 * y1 * z1; v[1] = q * x2 * y1 * z1; v[2] = q * x1 * y2 * z1; v[3] = q * x2 * y2 * z1;
   v[4] = q * x1 * y1 * z2; v[5] = q * x2 * y1 * z2; v[6] = q * x1 * y2 * z2; v[7] = q * x2 * y2 * z2;
@@ -303,10 +355,10 @@ __global__ __launch_bounds__(256) void k_deposit(const double* __restrict__ part
     const bool bm = inside(c, rm[0], rm[1], rm[2]);
     if (bp || bm) {
       const double q = part[PQ * cap + p];
-      const int ip = (int)floor((rp[0] - c.xmin) / c.dx), jp = (int)floor((rp[1] - c.ymin) / c.dy),
-                kp = (int)floor((rp[2] - c.zmin) / c.dz);
-      const int im = (int)floor((rm[0] - c.xmin) / c.dx), jm = (int)floor((rm[1] - c.ymin) / c.dy),
-                km = (int)floor((rm[2] - c.zmin) / c.dz);
+      const int ip = (int)floor(div_rn(rp[0] - c.xmin, c.dx, c.ix)), jp = (int)floor(div_rn(rp[1] - c.ymin, c.dy, c.iy)),
+                kp = (int)floor(div_rn(rp[2] - c.zmin, c.dz, c.iz));
+      const int im = (int)floor(div_rn(rm[0] - c.xmin, c.dx, c.ix)), jm = (int)floor(div_rn(rm[1] - c.ymin, c.dy, c.iy)),
+                km = (int)floor(div_rn(rm[2] - c.zmin, c.dz, c.iz));
       double r[3];
       r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
       r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
@@ -361,8 +413,8 @@ __global__ __launch_bounds__(256) void k_row_keys(const double* __restrict__ par
   const double x = part[(o + 0) * cap + p], y = part[(o + 1) * cap + p], z = part[(o + 2) * cap + p];
   unsigned k = rows;
   if (inside(c, x, y, z)) {
-    const int i = (int)floor((x - c.xmin) / c.dx), j = (int)floor((y - c.ymin) / c.dy),
-              kk = (int)floor((z - c.zmin) / c.dz);
+    const int i = (int)floor(div_rn(x - c.xmin, c.dx, c.ix)), j = (int)floor(div_rn(y - c.ymin, c.dy, c.iy)),
+              kk = (int)floor(div_rn(z - c.zmin, c.dz, c.iz));
     if (cell_ok(c, i, j, kk)) k = (unsigned)kk * (unsigned)c.N0 + (unsigned)i;
   }
   key[p] = k;
@@ -372,7 +424,7 @@ __global__ __launch_bounds__(256) void k_row_keys(const double* __restrict__ par
 // Adds one half's contributions to the nodes of the tile (plane k, rows i0 .. i0+ti-1, all j).
 template <bool RHO>
 __device__ inline void add_owned(double* tile, int TI, int N1, int k, int i0, int ti, int ic, int jc, int kc,
-                                 const double* v, int ncomp) {
+                                 const double* v, int) {
   const int tsz = TI * N1;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
@@ -387,19 +439,69 @@ __device__ inline void add_owned(double* tile, int TI, int N1, int k, int i0, in
   }
 }
 
+// One listed particle half of the row-tiled deposit: positions and charge as loaded.
+struct HalfItem {
+  double rp[3], rm[3], q;
+  int half, kk;
+  bool ok;
+};
+
+__device__ inline void load_item(HalfItem& it, const double* __restrict__ part, long cap, long p) {
+  it.rp[0] = part[PX * cap + p];
+  it.rp[1] = part[PY * cap + p];
+  it.rp[2] = part[PZ * cap + p];
+  it.rm[0] = part[PXM * cap + p];
+  it.rm[1] = part[PYM * cap + p];
+  it.rm[2] = part[PZM * cap + p];
+  it.q = part[PQ * cap + p];
+}
+
+template <bool SC>
+__device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, double* tile, int TI, int k, int i0,
+                                    int ti, int lo, int hi) {
+  const double* rp = it.rp;
+  const double* rm = it.rm;
+  const int ip = (int)floor(div_rn(rp[0] - c.xmin, c.dx, c.ix)), jp = (int)floor(div_rn(rp[1] - c.ymin, c.dy, c.iy)),
+            kp = (int)floor(div_rn(rp[2] - c.zmin, c.dz, c.iz));
+  const int im = (int)floor(div_rn(rm[0] - c.xmin, c.dx, c.ix)), jm = (int)floor(div_rn(rm[1] - c.ymin, c.dy, c.iy)),
+            km = (int)floor(div_rn(rm[2] - c.zmin, c.dz, c.iz));
+  // The order lists this half under row (kk, lo..hi); skip it should its cell disagree.
+  const int ic = it.half ? im : ip, jc = it.half ? jm : jp, kc = it.half ? km : kp;
+  if (kc != it.kk || ic < lo || ic > hi || jc < 0 || jc > c.N1 - 2) return;
+  double r[3];
+  r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
+  r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
+  r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
+  double v[SC ? 20 : 12];
+  if (it.half == 0) {
+    const double jc3[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
+    half_weights(c, it.q, rp, r, jc3, v);
+    if (SC) {
+      rho_weights(c, it.q, rp, v + 12);
+      add_owned<SC>(tile, TI, c.N1, k, i0, ti, ic, jc, kc, v, 0);
+      return;
+    }
+  } else {
+    const double jc3[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
+    half_weights(c, it.q, rm, r, jc3, v);
+  }
+  add_owned<false>(tile, TI, c.N1, k, i0, ti, ic, jc, kc, v, 0);
+}
+
 // Row-tiled current deposition (same arithmetic per particle as k_deposit).  Workgroup b owns
 // the J (and rho) values of plane k and rows i0 .. i0+ti-1, all j: it adds, in LDS, the halves
-// whose cell lies in planes k-1..k and rows i0-1 .. i0+ti-1 (read from the two row orders),
-// then writes its nodes once.  Every node of the mesh belongs to exactly one workgroup, so no
+// whose cell lies in planes k-1..k and rows i0-1 .. i0+ti-1 (up to four contiguous ranges of
+// the two row orders, walked as one index space, two halves in flight per thread), then
+// writes its nodes once.  Every node of the mesh belongs to exactly one workgroup, so no
 // global atomics are needed and the J buffer needs no separate reset (`accumulate` = 0 writes
 // the deposit alone, 1 adds it to what the buffer holds).
+constexpr int kDepositThreads = 512;
+
 template <typename R, bool SC>
-__global__ __launch_bounds__(256) void k_deposit_rows(const double* __restrict__ part, long cap, FieldSet J,
-                                                      DepositConst c, const unsigned* __restrict__ p_row,
-                                                      const unsigned* __restrict__ p_idx,
-                                                      const unsigned* __restrict__ m_row,
-                                                      const unsigned* __restrict__ m_idx, int TI, int tilesI,
-                                                      int accumulate) {
+__global__ __launch_bounds__(kDepositThreads) void k_deposit_rows(
+    const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
+    const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
+    int TI, int tilesI, int accumulate) {
   extern __shared__ double tile[];  // [ncomp][TI][N1]
   constexpr int ncomp = SC ? 4 : 3;
   const int k = blockIdx.x / tilesI;
@@ -413,47 +515,40 @@ __global__ __launch_bounds__(256) void k_deposit_rows(const double* __restrict__
     const int comp = t / own, o = t - comp * own;
     tile[comp * tsz + o] = accumulate ? (double)((const R*)J.c[comp])[base + o] : 0.0;
   }
-  __syncthreads();
+  // The ranges: (half 0 | 1) x (plane k-1 | k), rows lo..hi of each.
   const int lo = max(i0 - 1, 0), hi = min(i0 + ti - 1, c.N0 - 2);
-  for (int half = 0; half < 2; ++half) {
-    const unsigned* __restrict__ row = half ? m_row : p_row;
-    const unsigned* __restrict__ idx = half ? m_idx : p_idx;
-    for (int kk = max(k - 1, 0); kk <= min(k, c.N2 - 2); ++kk) {
-      if (lo > hi) break;
-      const unsigned s0 = row[(long)kk * c.N0 + lo], s1 = row[(long)kk * c.N0 + hi + 1];
-      for (unsigned s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-        const long p = idx[s];
-        const double rp[3] = {part[PX * cap + p], part[PY * cap + p], part[PZ * cap + p]};
-        const double rm[3] = {part[PXM * cap + p], part[PYM * cap + p], part[PZM * cap + p]};
-        const double q = part[PQ * cap + p];
-        const int ip = (int)floor((rp[0] - c.xmin) / c.dx), jp = (int)floor((rp[1] - c.ymin) / c.dy),
-                  kp = (int)floor((rp[2] - c.zmin) / c.dz);
-        const int im = (int)floor((rm[0] - c.xmin) / c.dx), jm = (int)floor((rm[1] - c.ymin) / c.dy),
-                  km = (int)floor((rm[2] - c.zmin) / c.dz);
-        double r[3];
-        r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
-        r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
-        r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
-        // The order lists this half under row (kk, lo..hi); skip it should its cell disagree.
-        const int ic = half ? im : ip, jc = half ? jm : jp, kc = half ? km : kp;
-        if (kc != kk || ic < lo || ic > hi || jc < 0 || jc > c.N1 - 2) continue;
-        double v[SC ? 20 : 12];
-        if (half == 0) {
-          const double jc[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
-          half_weights(c, q, rp, r, jc, v);
-          if (SC) {
-            rho_weights(c, q, rp, v + 12);
-            add_owned<SC>(tile, TI, N1, k, i0, ti, ip, jp, kp, v, ncomp);
-          } else {
-            add_owned<false>(tile, TI, N1, k, i0, ti, ip, jp, kp, v, ncomp);
-          }
-        } else {
-          const double jc[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
-          half_weights(c, q, rm, r, jc, v);
-          add_owned<false>(tile, TI, N1, k, i0, ti, im, jm, km, v, ncomp);
-        }
+  unsigned start[4], len[4];
+  int qhalf[4], qk[4], nq = 0;
+  unsigned total = 0;
+  if (lo <= hi)
+    for (int half = 0; half < 2; ++half)
+      for (int kk = max(k - 1, 0); kk <= min(k, c.N2 - 2); ++kk) {
+        const unsigned* row = half ? m_row : p_row;
+        start[nq] = row[(long)kk * c.N0 + lo];
+        len[nq] = row[(long)kk * c.N0 + hi + 1] - start[nq];
+        qhalf[nq] = half;
+        qk[nq] = kk;
+        total += len[nq];
+        ++nq;
       }
-    }
+  __syncthreads();
+  auto locate = [&](unsigned t, HalfItem& it) {
+    int q = 0;
+    while (q < nq - 1 && t >= len[q]) t -= len[q++];
+    it.half = qhalf[q];
+    it.kk = qk[q];
+    return (long)(qhalf[q] ? m_idx : p_idx)[start[q] + t];
+  };
+  for (unsigned t = threadIdx.x; t < total; t += 2 * blockDim.x) {
+    HalfItem a, b;
+    const unsigned tb = t + blockDim.x;
+    const long pa = locate(t, a);
+    b.ok = tb < total;
+    const long pb = b.ok ? locate(tb, b) : pa;
+    load_item(a, part, cap, pa);
+    load_item(b, part, cap, pb);
+    deposit_item<SC>(a, c, tile, TI, k, i0, ti, lo, hi);
+    if (b.ok) deposit_item<SC>(b, c, tile, TI, k, i0, ti, lo, hi);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
@@ -496,6 +591,21 @@ hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks) {
   return hipGetLastError();
 }
 
+hipError_t launch_zrange(Domain* d) {
+  d->zrange_valid = 0;
+  if (d->np == 0) return hipSuccess;
+  const mithra_grid& g = d->g;
+  const long want = (d->np + 255) / 256;
+  const int nb = (int)(want < kZBlocks ? want : kZBlocks);
+  hipLaunchKernelGGL(k_zcells, dim3(nb), dim3(256), 0, d->stream, d->part, d->cap, d->np, g.zmin, g.dz, 1.0 / g.dz,
+                     g.N2 - 2, (int2*)d->zpart);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_zcells_final, dim3(1), dim3(256), 0, d->stream, (const int2*)d->zpart, nb, d->zrange);
+  if ((e = hipGetLastError()) == hipSuccess) d->zrange_valid = 1;
+  return e;
+}
+
 hipError_t launch_bunch_begin(Domain* d) {
   d->rm_order_valid = d->order1_is_rnp;  // rnm := rnp
   if (d->np == 0) return hipSuccess;
@@ -514,6 +624,7 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_und
   c.xmin = g.xmin; c.xmax = g.xmax; c.ymin = g.ymin; c.ymax = g.ymax; c.zmin = g.zmin;
   c.zp0 = g.zp0; c.zp1 = g.zp1; c.dx = g.dx; c.dy = g.dy; c.dz = g.dz; c.Lz = g.Lz;
   c.N0 = g.N0; c.N1 = g.N1; c.N2 = g.N2;
+  c.ix = 1.0 / g.dx; c.iy = 1.0 / g.dy; c.iz = 1.0 / g.dz;
   c.gamma = p->gamma; c.beta = p->beta; c.c0 = p->c0; c.dt_shift = p->dt_shift;
   c.r1 = p->r1; c.r2 = p->r2; c.dtb = p->dtb;
   c.tb0 = time_bunch; c.dtb_time = bunch_time_step;
@@ -521,7 +632,9 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_und
   c.dist0 = n_und > 0 ? und[0].dist : 0.0;
   hipLaunchKernelGGL(k_push, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap, d->np,
                      d->eb, (const mithra_undulator*)und, c, d->counters);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_zrange(d);
 }
 
 namespace {
@@ -532,6 +645,7 @@ DepositConst deposit_const(const Domain* d) {
   c.xmin = g.xmin; c.xmax = g.xmax; c.ymin = g.ymin; c.ymax = g.ymax; c.zmin = g.zmin;
   c.zp0 = g.zp0; c.zp1 = g.zp1; c.dx = g.dx; c.dy = g.dy; c.dz = g.dz;
   c.N0 = g.N0; c.N1 = g.N1; c.N2 = g.N2; c.sc = g.space_charge;
+  c.ix = 1.0 / g.dx; c.iy = 1.0 / g.dy; c.iz = 1.0 / g.dz;
   return c;
 }
 
@@ -609,16 +723,16 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   const unsigned *pr = d->ord[0].row, *pi = d->ord[0].idx, *mr = d->ord[1].row, *mi = d->ord[1].idx;
   const bool f64 = d->precision == MITHRA_PRECISION_FP64;
   if (f64 && c.sc)
-    hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+    hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
                        mr, mi, TI, tilesI, accumulate);
   else if (f64)
-    hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+    hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
                        mr, mi, TI, tilesI, accumulate);
   else if (c.sc)
-    hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+    hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
                        mr, mi, TI, tilesI, accumulate);
   else
-    hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(256), lds, d->stream, d->part, d->cap, J, c, pr, pi,
+    hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
                        mr, mi, TI, tilesI, accumulate);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // Periodic reorder of the particle arrays into the rnp row order.

@@ -504,6 +504,8 @@ void Solver::upload(int precision) {
   mithra_grid g = grid();
   mithra_coeffs c = coeffs();
   check(mithra_dev_create(&g, &c, precision, &dev_), "device allocation");
+  // E/B only where the bunch and the power sampler read them, as the reference evaluates it
+  check(mithra_dev_set_lazy_fields(dev_, 1), "lazy field evaluation");
   check(mithra_dev_set_potentials(dev_, 0, an_.data(), mesh_.spaceCharge_ ? fn_.data() : nullptr), "A^n upload");
   check(mithra_dev_set_potentials(dev_, 1, anm1_.data(), mesh_.spaceCharge_ ? fnm1_.data() : nullptr), "A^n-1 upload");
   std::vector<double> q;

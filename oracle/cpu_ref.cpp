@@ -547,6 +547,35 @@ static void eval_eb(oracle_dev* d, long m) {
                 (A(ap, N1, 1) - A(ap, -N1, 1)) / dx2 - (A(ap, 1, 0) - A(ap, -1, 0)) / dy2);
 }
 
+/* E/B on every node a particle or sampler reads; planes k=0, np-1 copy k=1, np-2
+ * (src/fdtd.cpp:742-774). */
+static void eval_all(oracle_dev* d) {
+  const mithra_grid& g = d->g;
+  const long N0 = g.N0, N1 = g.N1, np = g.N2, N1N0 = d->N1N0;
+#pragma omp parallel for collapse(2)
+  for (long k = 1; k < np - 1; k++)
+    for (long i = 1; i < N0 - 1; i++)
+      for (long j = 1; j < N1 - 1; j++) eval_eb(d, N1N0 * k + N1 * i + j);
+  for (long i = 1; i < N0 - 1; i++)
+    for (long j = 1; j < N1 - 1; j++) {
+      long m = N1N0 + N1 * i + j;
+      for (int c = 0; c < 3; ++c) {
+        d->en[3 * (m - N1N0) + c] = d->en[3 * m + c];
+        d->bn[3 * (m - N1N0) + c] = d->bn[3 * m + c];
+      }
+      m = N1N0 * (np - 2) + N1 * i + j;
+      for (int c = 0; c < 3; ++c) {
+        d->en[3 * (m + N1N0) + c] = d->en[3 * m + c];
+        d->bn[3 * (m + N1N0) + c] = d->bn[3 * m + c];
+      }
+    }
+}
+
+extern "C" int oracle_field_evaluate(oracle_dev* d) {
+  eval_all(d);
+  return MITHRA_OK;
+}
+
 extern "C" int oracle_field_update(oracle_dev* d, double time, const mithra_seed* seed) {
   const mithra_grid& g = d->g;
   const long N0 = g.N0, N1 = g.N1, np = g.N2, N1N0 = d->N1N0;
@@ -582,25 +611,7 @@ extern "C" int oracle_field_update(oracle_dev* d, double time, const mithra_seed
   }
   boundaries(d, P, N, M, 3);
   if (g.space_charge) boundaries(d, d->fnp1(), d->fn(), d->fnm1(), 1);
-  // E/B on every node a particle or sampler reads; planes k=0, np-1 copy k=1, np-2
-  // (src/fdtd.cpp:742-774)
-#pragma omp parallel for collapse(2)
-  for (long k = 1; k < np - 1; k++)
-    for (long i = 1; i < N0 - 1; i++)
-      for (long j = 1; j < N1 - 1; j++) eval_eb(d, N1N0 * k + N1 * i + j);
-  for (long i = 1; i < N0 - 1; i++)
-    for (long j = 1; j < N1 - 1; j++) {
-      long m = N1N0 + N1 * i + j;
-      for (int c = 0; c < 3; ++c) {
-        d->en[3 * (m - N1N0) + c] = d->en[3 * m + c];
-        d->bn[3 * (m - N1N0) + c] = d->bn[3 * m + c];
-      }
-      m = N1N0 * (np - 2) + N1 * i + j;
-      for (int c = 0; c < 3; ++c) {
-        d->en[3 * (m + N1N0) + c] = d->en[3 * m + c];
-        d->bn[3 * (m + N1N0) + c] = d->bn[3 * m + c];
-      }
-    }
+  eval_all(d);
   return MITHRA_OK;
 }
 

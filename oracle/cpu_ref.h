@@ -35,6 +35,7 @@ int oracle_set_particles(oracle_dev* dev, const double* charge11, long n);
 int oracle_get_particles(oracle_dev* dev, double* charge11, long n);
 long oracle_particle_count(const oracle_dev* dev);
 int oracle_field_update(oracle_dev* dev, double time, const mithra_seed* seed);
+int oracle_field_evaluate(oracle_dev* dev);
 int oracle_bunch_begin(oracle_dev* dev);
 int oracle_bunch_update(oracle_dev* dev, const mithra_bunch_params* p, const mithra_undulator* und,
                         int n_und, double time_bunch);

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from golden_state import CASES, Case, march, rel_l2
+from mithra_amd import abi
 
 pytestmark = pytest.mark.gpu
 
@@ -131,3 +132,47 @@ def test_cell_sort_changes_no_particle_result(case):
         march(case, dom, 4, start_step=1, time=t, time_bunch=tb)
     assert rel_l2(sorted_dom.get_particles(), plain.get_particles()) < 1e-13
     assert rel_l2(sorted_dom.get_potentials(0)[0], plain.get_potentials(0)[0]) < 1e-12
+
+
+def test_tile_shapes_give_identical_fields(case):
+    """Every tile shape of the interior update and the E/B evaluation gives bit-identical
+    potentials and fields for the field update of the reference's first step (the field
+    kernels are deterministic; the autotuner may pick any shape)."""
+    ref = None
+    for v in range(abi.TILE_VARIANTS):
+        dom = case.open("gpu")
+        dom.set_tiles(v, v)
+        dom.field_update(case.time0, case.seed)
+        out = (dom.get_potentials(2)[0], *dom.get_fields())
+        if ref is None:
+            ref = out
+            assert rel_l2(out[0], case.arr("post0_anp1")) < 1e-13
+        else:
+            for a, b in zip(out, ref):
+                assert np.array_equal(a, b), f"tile variant {v} {abi.tile_shape(v)}"
+
+
+def test_autotune_keeps_state(case):
+    dom = case.open("gpu")
+    march(case, dom, 2)
+    before = dom.get_potentials(2)[0], dom.get_potentials(0)[0]
+    (ci, ce), ms = dom.autotune(reps=1)
+    assert 0 <= ci < abi.TILE_VARIANTS and 0 <= ce < abi.TILE_VARIANTS
+    assert np.all(ms > 0)
+    after = dom.get_potentials(2)[0], dom.get_potentials(0)[0]
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_lazy_fields_change_no_result(case):
+    """E/B evaluated only on the planes the bunch and the power sampler read gives the same
+    march as evaluating every node (to the rounding of the deposit's summation order)."""
+    lazy = case.open("gpu")
+    lazy.set_lazy_fields(1)
+    full = case.open("gpu")
+    for dom in (lazy, full):
+        march(case, dom, 8)
+    assert rel_l2(lazy.get_particles(), full.get_particles()) < 1e-13
+    assert rel_l2(lazy.get_potentials(0)[0], full.get_potentials(0)[0]) < 1e-12
+    p_l = lazy.power_fetch(0, 8, case.power_N, case.power_Nl)
+    p_f = full.power_fetch(0, 8, case.power_N, case.power_Nl)
+    assert rel_l2(p_l, p_f) < 1e-12
