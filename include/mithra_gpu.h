@@ -247,6 +247,40 @@ int mithra_dev_set_tiles(mithra_dev* dev, int interior, int eval);
  * counterpart in the reference (a CPU code with no launch shapes). */
 int mithra_dev_autotune(mithra_dev* dev, int reps, int* chosen, float* ms);
 
+/* ---- batched parameter scans ------------------------------------------------------------
+ * A batch marches many independent domains of uneven size together: each step is one fixed
+ * sequence of launches whose persistent workgroups walk tables of (instance, tile) entries, and
+ * it replays as one captured HIP graph.  Each instance is an ordinary mithra_dev, created and
+ * filled through the calls above (potentials, particles, power set-up) and read back through
+ * them afterwards (power fetch, particles, potentials); the batch does not own the handles,
+ * which must outlive it and must not be used while it runs.  Supported: no space charge, no
+ * TF/SF seed, static undulators, rows of the mesh that fit the row-tiled deposit.  Each
+ * instance computes exactly its single-domain march (mithra_dev_step with a fused push and
+ * power row = step): only the deposit's summation order differs.  No counterpart in the
+ * reference (one job per process run). */
+typedef struct mithra_batch mithra_batch;
+int mithra_batch_create(mithra_dev* const* devs, int n, mithra_batch** out);
+int mithra_batch_destroy(mithra_batch* batch);
+/* Describes instance i's march: start time and bunch time, the bunch time step and sub-steps
+ * per field step, the number of field steps, the push constants and undulators; and its
+ * bunching-factor samples b = |sum q exp(i k z)| / |sum q| taken after the push of every step
+ * that is a multiple of `bunching_every`, into slots 0 .. bunching_slots-1. */
+int mithra_batch_instance(mithra_batch* batch, int i, double time, double time_bunch, double bunch_time_step,
+                          int n_sub, long n_steps, const mithra_bunch_params* p, const mithra_undulator* und,
+                          int n_und, double bunching_k, long bunching_every, int bunching_slots);
+/* Tile shape of the field kernels (a variant of mithra_tile_shape) and whether steps replay as
+ * a captured graph (1) or are launched one by one (0); before the first run. */
+int mithra_batch_set_tiles(mithra_batch* batch, int variant, int use_graph);
+/* Times the field kernels of the whole batch for each tile shape and keeps the fastest (the
+ * domains' state is left as it was); before the first run.  ms[MITHRA_TILE_VARIANTS]
+ * (optional): per shape. */
+int mithra_batch_autotune(mithra_batch* batch, int reps, int* chosen, float* ms);
+/* Advances every unfinished instance by up to `steps` field steps (instances stop at their own
+ * n_steps); *done receives the steps taken.  Returns with each mithra_dev updated. */
+int mithra_batch_run(mithra_batch* batch, long steps, long* done);
+int mithra_batch_steps(const mithra_batch* batch, int i, long* step);
+int mithra_batch_bunching(mithra_batch* batch, int i, double* out);
+
 /* Bunching factor b = |sum q exp(i k z)| / |sum q| over all particles at their current boosted-
  * frame z.  With k = 2 pi / lambda', lambda' = lambda_u / (gamma lambda_norm) the radiation
  * wavelength in the frame (the one the power sampler uses, src/radiation.cpp:91-92), this is

@@ -337,3 +337,80 @@ def make_power_params(z, w, Nf, pc, gamma, beta, c0):
     p = PowerParams(len(z), len(w), int(Nf), pc, gamma, beta, c0, _ptr(z), _ptr(w))
     p._arrays = (z, w)
     return p
+
+
+class Batch:
+    """A batch of device domains marched together (mithra_batch_*, include/mithra_gpu.h).
+
+    `domains` are `Domain`s of the device library, already filled (potentials, particles,
+    power set-up); they must stay alive and unused while the batch runs."""
+
+    def __init__(self, domains):
+        self.lib = load_gpu_library()
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        for name, res, args in (
+            ("mithra_batch_create", C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+            ("mithra_batch_destroy", C.c_int, [vp]),
+            ("mithra_batch_instance", C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                C.c_long, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
+                                                C.c_double, C.c_long, C.c_int]),
+            ("mithra_batch_set_tiles", C.c_int, [vp, C.c_int, C.c_int]),
+            ("mithra_batch_autotune", C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+            ("mithra_batch_run", C.c_int, [vp, C.c_long, C.POINTER(C.c_long)]),
+            ("mithra_batch_steps", C.c_int, [vp, C.c_int, C.POINTER(C.c_long)]),
+            ("mithra_batch_bunching", C.c_int, [vp, C.c_int, dp]),
+        ):
+            f = getattr(self.lib, name)
+            f.restype, f.argtypes = res, args
+        self.domains = list(domains)
+        handles = (vp * len(self.domains))(*[d.h for d in self.domains])
+        self.h = vp()
+        self._check(self.lib.mithra_batch_create(handles, len(self.domains), C.byref(self.h)))
+        self.slots = [0] * len(self.domains)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MithraError(f"status {rc}: {self.lib.mithra_strerror(rc).decode()}")
+
+    def instance(self, i, time, time_bunch, bunch_time_step, n_sub, n_steps, bunch_params, undulators,
+                 bunching_k=0.0, bunching_every=0, bunching_slots=0):
+        und = (Undulator * max(1, len(undulators)))(*undulators)
+        self._check(self.lib.mithra_batch_instance(self.h, i, time, time_bunch, bunch_time_step, n_sub, n_steps,
+                                                   C.byref(bunch_params), und, len(undulators), bunching_k,
+                                                   bunching_every, bunching_slots))
+        self.slots[i] = bunching_slots
+
+    def set_tiles(self, variant, use_graph=True):
+        self._check(self.lib.mithra_batch_set_tiles(self.h, variant, 1 if use_graph else 0))
+
+    def autotune(self, reps=2):
+        chosen = C.c_int()
+        ms = (C.c_float * TILE_VARIANTS)()
+        self._check(self.lib.mithra_batch_autotune(self.h, reps, C.byref(chosen), ms))
+        return chosen.value, list(ms)
+
+    def run(self, steps):
+        done = C.c_long()
+        self._check(self.lib.mithra_batch_run(self.h, steps, C.byref(done)))
+        return done.value
+
+    def steps(self, i):
+        s = C.c_long()
+        self._check(self.lib.mithra_batch_steps(self.h, i, C.byref(s)))
+        return s.value
+
+    def bunching(self, i):
+        out = np.zeros(max(1, self.slots[i]))
+        self._check(self.lib.mithra_batch_bunching(self.h, i, _ptr(out)))
+        return out[:self.slots[i]]
+
+    def close(self):
+        if self.h:
+            self.lib.mithra_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,13 +11,6 @@
 
 using namespace mithra;
 
-struct mithra_dev : Domain {
-  mithra_undulator* und_dev = nullptr;  // device copy of the undulator table
-  int und_cap = 0;
-  std::vector<mithra_undulator> und_host;  // what und_dev holds
-  hipEvent_t ev[MITHRA_STEP_PHASES + 1] = {};  // phase timers of mithra_dev_step_timed
-  bool ev_ready = false;
-};
 
 namespace {
 
@@ -40,7 +33,9 @@ bool valid_grid(const mithra_grid* g) {
          (long)g->N0 * g->N1 * g->N2 < (1L << 40);
 }
 
-int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n) {
+}  // namespace
+
+int mithra::upload_undulators(mithra_dev* d, const mithra_undulator* und, int n) {
   if (n < 0 || (n > 0 && !und)) return MITHRA_E_ARG;
   for (int u = 0; u < n; ++u)
     if (und[u].type != MITHRA_UNDULATOR_STATIC) return MITHRA_E_UNSUPPORTED;
@@ -60,15 +55,21 @@ int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n) {
   return MITHRA_OK;
 }
 
+namespace {
+
 // current_reset only marks the A^{n+1} buffer as zero: the next deposit writes every node of
 // it anyway.  Anything else that reads or moves the potentials writes the zeros first.
-int settle(mithra_dev* d) {
+}  // namespace
+
+int mithra::settle(mithra_dev* d) {
   if (!d->zero_pending) return MITHRA_OK;
   d->zero_pending = 0;
   for (int c = 0; c < d->ncomp; ++c)
     MITHRA_TRY(launch_fill_zero(d, d->lvl[d->i_np1].c[c], (size_t)d->nodes * d->real_size));
   return MITHRA_OK;
 }
+
+namespace {
 
 int level_of(const mithra_dev* d, int level) {
   return level == 0 ? d->i_n : level == 1 ? d->i_nm1 : level == 2 ? d->i_np1 : -1;

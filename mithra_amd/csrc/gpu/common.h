@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../../include/mithra_gpu.h"
 
@@ -27,6 +28,34 @@ enum Particle { PX, PY, PZ, PXM, PYM, PZM, PGX, PGY, PGZ, PQ, PE, PCOUNT };
 
 struct FieldSet {           // one time level of every potential component
   void* c[4];               // A_x, A_y, A_z, phi (phi only with space charge)
+};
+
+// Launch constants of the kernels, shared by the single-domain launchers and the batch views.
+struct MurCoeffs {
+  double bB[5], cB[5], dB[5], eE[5], fE[5], gE[5], hC[17];
+};
+
+struct EvalConst {  // -dt, twice the steps, and their correctly rounded reciprocals (div_rn)
+  double mdt, dx2, dy2, dz2, rmdt, rdx2, rdy2, rdz2;
+};
+
+struct PushConst {
+  // mesh
+  double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz, Lz;
+  double ix, iy, iz;  // RN(1/dx), RN(1/dy), RN(1/dz) for div_rn
+  int N0, N1, N2;
+  // frame and Boris constants
+  double gamma, beta, c0, dt_shift, r1, r2, dtb;
+  // time of the first sub-step and the bunch time step added after each sub-step
+  double tb0, dtb_time;
+  int n_sub, n_und, save_rnm;
+  double dist0;  // undulator[0].dist_: entry test of the emission flag
+};
+
+struct DepositConst {
+  double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz;
+  double ix, iy, iz;        // RN(1/dx), RN(1/dy), RN(1/dz) for div_rn
+  int N0, N1, N2, sc;
 };
 
 struct CellOrder {          // particle halves sorted by the (k, i) row of their cell
@@ -105,6 +134,61 @@ __device__ inline double div_rn(double x, double d, double r) {
   return e == 0.0 ? q : fma(e, r, q);
 }
 
+// One instance of a batch as the batched kernels read it (device memory, one per instance).  The
+// first block is advanced on the device after every step, so one captured step graph replays
+// for the whole run; the rest describes the instance's domain and does not change.
+struct BatchView {
+  int active;               // step < n_steps
+  int i_np1, i_n, i_nm1;    // potential levels, rotated like Domain's
+  int parity;               // ord[parity]: rnp order of this step's deposit, ord[1-parity]: rnm order
+  long step, n_steps;
+  double time, time_bunch;
+  // mesh, potentials, E/B
+  int N0, N1, N2, ncomp, nsfd, trunc2;
+  FieldSet lvl[3];
+  float4* eb;
+  int tilesJ, tilesI, chunks;  // interior / E-B tiles of the batch's tile shape
+  double a[6], alpha, beta;
+  MurCoeffs mur;
+  EvalConst ev;
+  double dt;
+  // particles
+  double* part;
+  double* part_alt;         // spare arrays of the periodic reorder (swapped in by kb_advance)
+  unsigned* pid;
+  unsigned* pid_alt;
+  int reorder;              // steps between reorders of the particle arrays (0: never)
+  long cap, np;
+  PushConst push;           // tb0 is taken from time_bunch each step
+  const mithra_undulator* und;
+  unsigned long long* counters;
+  int n_sub;
+  double dtb;
+  DepositConst dep;
+  CellOrder ord[2];
+  unsigned rows;            // cell rows (N2-1)*N0 of the deposition
+  unsigned* cnt;            // rows+1 counters of the counting sort, zero between sorts
+  unsigned* bsum;           // per-1024-row block sums of the scan
+  int dTI, dtilesI;         // row-tiled deposit geometry
+  // power sampling
+  double* ring;
+  const double* dft;
+  const int* pk;
+  const double* pdzr;
+  double* partial;
+  double* history;
+  const int* plane_of;
+  unsigned pNz, pN, pNl, pNf, prows;
+  int pblocks;
+  double pgamma, pbeta, pc0, pc;
+  // bunching factor samples: every bevery steps into bout[step / bevery] (bslots of them)
+  double bk;
+  long bevery;
+  double* bout;
+  double* bpart;            // per particle-block partial sums, 3 doubles each
+  int bslots;
+};
+
 inline double* particle_array(Domain* d, int which) { return d->part + (size_t)which * d->cap; }
 
 // Launch helpers implemented in field.hip, particles.hip and power.hip.
@@ -134,4 +218,40 @@ hipError_t ensure_orders(Domain* d, unsigned rows);
 hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows);
 hipError_t reorder_particles(Domain* d, CellOrder& o);
 
+// Batched forms of the kernels (persistent workgroups walking a table of (instance, local tile)
+// entries; an entry of an instance that has finished is skipped).
+struct BatchTiles {
+  const int2* t;
+  int n;
+};
+hipError_t launch_batch_interior(BatchView* V, BatchTiles T, int variant, int precision, int nsfd, hipStream_t s);
+hipError_t launch_batch_mur(BatchView* V, BatchTiles faces, BatchTiles edges, BatchTiles corners, int precision,
+                            hipStream_t s);
+hipError_t launch_batch_eval(BatchView* V, BatchTiles T, int variant, int precision, hipStream_t s);
+hipError_t launch_batch_push(BatchView* V, BatchTiles T, hipStream_t s);
+hipError_t launch_batch_bunching(BatchView* V, BatchTiles particles, BatchTiles inst, hipStream_t s);
+hipError_t launch_batch_sort(BatchView* V, BatchTiles particles, BatchTiles rowblocks, BatchTiles inst, int use_rm,
+                             int slot_sel, hipStream_t s);
+hipError_t launch_batch_deposit(BatchView* V, BatchTiles T, int precision, size_t lds, hipStream_t s);
+hipError_t launch_batch_reorder(BatchView* V, BatchTiles particles, hipStream_t s);
+hipError_t launch_batch_power(BatchView* V, BatchTiles nodes, BatchTiles reduce, hipStream_t s);
+int batch_grid(int ntiles, int per_cu);  // persistent grid size for a table of ntiles entries
+
+int deposit_tile_rows(const Domain* d);      // rows per tile of the row-tiled deposit (0: atomic path)
+DepositConst make_deposit_const(const Domain* d);
+
+}  // namespace mithra
+
+// The C-ABI handle of one domain (include/mithra_gpu.h).
+struct mithra_dev : mithra::Domain {
+  mithra_undulator* und_dev = nullptr;      // device copy of the undulator table
+  int und_cap = 0;
+  std::vector<mithra_undulator> und_host;   // what und_dev holds
+  hipEvent_t ev[MITHRA_STEP_PHASES + 1] = {};  // phase timers of mithra_dev_step_timed
+  bool ev_ready = false;
+};
+
+namespace mithra {
+int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n);  // capi.hip
+int settle(mithra_dev* d);  // writes the zeros current_reset left pending
 }  // namespace mithra

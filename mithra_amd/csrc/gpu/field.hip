@@ -47,11 +47,9 @@ __device__ inline Plane5 load_plane(const R* __restrict__ a, long m, long N1) {
 // x/y neighbours it reads are the centres of neighbouring lanes and rows, served by L1/L2.
 // J (or rho) lives in the A^{n+1} buffer and is overwritten in place, as in the reference.
 template <typename R, bool NSFD, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI) void k_interior(
-    FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
-    int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
-    double alpha, double beta) {
-  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+__device__ inline void interior_body(FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0,
+                                     int N1, int N2, int tilesJ, int tilesI, int chunks, double a0, double a1,
+                                     double a2, double a3, double a4, double a5, double alpha, double beta, int lb) {
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
@@ -89,20 +87,28 @@ __global__ __launch_bounds__(kTileJ* TI) void k_interior(
   }
 }
 
+template <typename R, bool NSFD, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_interior(
+    FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
+    int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
+    double alpha, double beta) {
+  interior_body<R, NSFD, TI, K>(an_set, anm1_set, anp1_set, ncomp, N0, N1, N2, tilesJ, tilesI, chunks, a0, a1, a2,
+                                a3, a4, a5, alpha, beta, xcd_logical_block(blockIdx.x, gridDim.x));
+}
+
 // The interior update with the A^n planes staged in LDS (2.5-D blocking).  The workgroup (TI
 // wavefronts, one x-row of 64 y-nodes each) marches its K planes in lockstep: plane k+1 of its
 // tile plus a one-node halo ring is fetched once, coalesced, into a three-plane LDS ring, and
 // every neighbour of the stencil is read from LDS.  Same arithmetic, in the same order, as
 // k_interior.
 template <typename R, bool NSFD, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI) void k_interior_lds(
-    FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
-    int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
-    double alpha, double beta) {
+__device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0,
+                                         int N1, int N2, int tilesJ, int tilesI, int chunks, double a0, double a1,
+                                         double a2, double a3, double a4, double a5, double alpha, double beta,
+                                         int lb) {
   constexpr int W = kTileJ + 2;           // LDS row: y-halo, 64 nodes, y-halo
   constexpr int P = (TI + 2) * W;         // one plane of the tile with its halo ring
   __shared__ double L[3 * P];
-  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
@@ -180,6 +186,15 @@ __global__ __launch_bounds__(kTileJ* TI) void k_interior_lds(
     }
     __syncthreads();
   }
+}
+
+template <typename R, bool NSFD, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_interior_lds(
+    FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
+    int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
+    double alpha, double beta) {
+  interior_lds_body<R, NSFD, TI, K>(an_set, anm1_set, anp1_set, ncomp, N0, N1, N2, tilesJ, tilesI, chunks, a0, a1,
+                                    a2, a3, a4, a5, alpha, beta, xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
 // ---- analytic seed (Signal::self src/classes.cpp:534-575, Seed::fields :740-855) --------
@@ -351,18 +366,14 @@ __device__ inline double mur_corner(const double* h, const R* P, const R* N, con
   return -s / h[0];
 }
 
-struct MurCoeffs {
-  double bB[5], cB[5], dB[5], eE[5], fE[5], gE[5], hC[17];
-};
 
 // All six faces in one launch: x faces (j, k), y faces (i, k), z faces (i, j), each min/max.
 template <typename R>
-__global__ void k_mur_faces(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+__device__ inline void mur_faces_body(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2,
+                                      const MurCoeffs& c, long t, int comp) {
   const long N1N0 = (long)N1 * N0;
   const long X = N1, Y = 1, Z = N1N0;
   const long nx = (long)(N1 - 2) * (N2 - 2), ny = (long)(N0 - 2) * (N2 - 2), nz = (long)(N0 - 2) * (N1 - 2);
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int comp = blockIdx.y;
   if (comp >= ncomp) return;
   const R* pn = (const R*)N.c[comp];
   const R* pm = (const R*)M.c[comp];
@@ -393,14 +404,18 @@ __global__ void k_mur_faces(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N
   pp[m] = (R)mur_face(B, pp, pn, pm, m, s, tt, u);
 }
 
+template <typename R>
+__global__ void k_mur_faces(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+  mur_faces_body<R>(P, N, M, ncomp, N0, N1, N2, c, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
 // The twelve edges (second-order Mur only).
 template <typename R>
-__global__ void k_mur_edges(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+__device__ inline void mur_edges_body(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2,
+                                      const MurCoeffs& c, long t, int comp) {
   const long N1N0 = (long)N1 * N0;
   const long X = N1, Y = 1, Z = N1N0;
   const long ez = N2 - 2, ex = N0 - 2, ey = N1 - 2;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int comp = blockIdx.y;
   if (comp >= ncomp) return;
   const R* pn = (const R*)N.c[comp];
   const R* pm = (const R*)M.c[comp];
@@ -432,9 +447,13 @@ __global__ void k_mur_edges(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N
 }
 
 template <typename R>
-__global__ void k_mur_corners(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
-  const int t = threadIdx.x;
-  const int comp = blockIdx.y;
+__global__ void k_mur_edges(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+  mur_edges_body<R>(P, N, M, ncomp, N0, N1, N2, c, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+template <typename R>
+__device__ inline void mur_corners_body(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2,
+                                        const MurCoeffs& c, int t, int comp) {
   if (t >= 8 || comp >= ncomp) return;
   const long N1N0 = (long)N1 * N0;
   const bool xh = t & 1, yh = t & 2, zh = t & 4;
@@ -443,6 +462,11 @@ __global__ void k_mur_corners(FieldSet P, FieldSet N, FieldSet M, int ncomp, int
   const R* pm = (const R*)M.c[comp];
   R* pp = (R*)P.c[comp];
   pp[m] = (R)mur_corner(c.hC, pp, pn, pm, m, xh ? -(long)N1 : N1, yh ? -1L : 1L, zh ? -N1N0 : N1N0);
+}
+
+template <typename R>
+__global__ void k_mur_corners(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
+  mur_corners_body<R>(P, N, M, ncomp, N0, N1, N2, c, threadIdx.x, blockIdx.y);
 }
 
 // E/B on every interior node (FdTd::fieldEvaluate src/fdtd.cpp:818-845, FdTdSC::fieldEvaluate
@@ -460,18 +484,14 @@ __device__ inline bool lazy_needed(int kb, int ke, const int* zr, int pb0, int p
   return (ke - 1 >= a0 && kb <= a1) || (ke - 1 >= pb0 && kb <= pb1);
 }
 
-struct EvalConst {  // -dt, twice the steps, and their correctly rounded reciprocals (div_rn)
-  double mdt, dx2, dy2, dz2, rmdt, rdx2, rdy2, rdz2;
-};
 
 // Each thread owns one (i, j) column and marches K planes; the z neighbours of A_x and A_y (and
 // phi) at both time levels ride in a three-plane register queue, the transverse neighbours are
 // the centres of neighbouring lanes and rows.  Planes outside [klo, khi] are skipped.
 template <typename R, bool SC, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
-                                                        int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
-                                                        EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
-  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+__device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1, int N2,
+                                 int tilesJ, int tilesI, int klo, int khi, const EvalConst& q,
+                                 const int* __restrict__ zr, int pb0, int pb1, int lb) {
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
@@ -539,19 +559,26 @@ __global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap
   }
 }
 
+template <typename R, bool SC, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
+                                                        int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
+                                                        EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
+  eval_body<R, SC, TI, K>(An, Ap, eb, N0, N1, N2, tilesJ, tilesI, klo, khi, q, zr, pb0, pb1,
+                          xcd_logical_block(blockIdx.x, gridDim.x));
+}
+
 // The E/B evaluation with plane k of A^n and A^{n+1} staged in LDS (2.5-D blocking, as
 // k_interior_lds): the workgroup marches its planes in lockstep, every transverse neighbour is
 // read from LDS, the z neighbours of A_x, A_y (and phi) ride in a register queue, and the
 // next plane is fetched while the current one is computed.  Same expressions as k_eval_eb.
 template <typename R, bool SC, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
-                                                         int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
-                                                         EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
+__device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1, int N2,
+                                     int tilesJ, int tilesI, int klo, int khi, const EvalConst& q,
+                                     const int* __restrict__ zr, int pb0, int pb1, int lb) {
   constexpr int NA = SC ? 7 : 6;      // staged arrays: n0 n1 n2 p0 p1 p2 [phi]
   constexpr int NZ = SC ? 5 : 4;      // arrays with z neighbours: n0 n1 p0 p1 [phi]
   constexpr int W = kTileJ + 2, P = (TI + 2) * W;
   __shared__ double L[NA * P];
-  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
@@ -673,6 +700,84 @@ __global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet A
       qz[a][0] = qz[a][1];
       qz[a][1] = qz[a][2];
       qz[a][2] = qz[a][3];
+    }
+  }
+}
+
+template <typename R, bool SC, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
+                                                         int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
+                                                         EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
+  eval_lds_body<R, SC, TI, K>(An, Ap, eb, N0, N1, N2, tilesJ, tilesI, klo, khi, q, zr, pb0, pb1,
+                              xcd_logical_block(blockIdx.x, gridDim.x));
+}
+
+// ---- batched forms: persistent workgroups walking a table of (instance, local tile) ---------
+// Workgroup b takes entries r * G + l(b), r = 0, 1, ..., where l gives every XCD a contiguous
+// run of entries, so neighbouring tiles of one instance meet in one L2.
+__device__ inline int batch_entry(int r) { return r * (int)gridDim.x + xcd_logical_block(blockIdx.x, gridDim.x); }
+
+template <typename R, bool NSFD, int TI, int K, bool LDS>
+__global__ __launch_bounds__(kTileJ* TI) void kb_interior(const BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (v.active) {
+      if constexpr (LDS)
+        interior_lds_body<R, NSFD, TI, K>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1, v.N2,
+                                          v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3], v.a[4],
+                                          v.a[5], v.alpha, v.beta, e.y);
+      else
+        interior_body<R, NSFD, TI, K>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1, v.N2,
+                                      v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3], v.a[4], v.a[5],
+                                      v.alpha, v.beta, e.y);
+    }
+    if constexpr (LDS) __syncthreads();  // the LDS ring is reused by the next tile
+  }
+}
+
+template <typename R, int TI, int K, bool LDS>
+__global__ __launch_bounds__(kTileJ* TI) void kb_eval(const BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (v.active) {
+      const EvalConst q = v.ev;  // a local copy: stores to eb cannot alias it
+      if constexpr (LDS)
+        eval_lds_body<R, false, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
+                                       v.N2 - 2, q, nullptr, 0, -1, e.y);
+      else
+        eval_body<R, false, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
+                                   v.N2 - 2, q, nullptr, 0, -1, e.y);
+    }
+    if constexpr (LDS) __syncthreads();
+  }
+}
+
+// Mur faces / edges / corners: entry .y = comp * blocks_per_comp + block
+template <typename R, int WHICH>
+__global__ __launch_bounds__(256) void kb_mur(const BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active) continue;
+    const FieldSet P = v.lvl[v.i_np1], N = v.lvl[v.i_n], M = v.lvl[v.i_nm1];
+    if (WHICH == 0) {
+      const long nf = 2L * ((long)(v.N1 - 2) * (v.N2 - 2) + (long)(v.N0 - 2) * (v.N2 - 2) + (long)(v.N0 - 2) * (v.N1 - 2));
+      const int nb = (int)((nf + 255) / 256);
+      mur_faces_body<R>(P, N, M, v.ncomp, v.N0, v.N1, v.N2, v.mur, (long)(e.y % nb) * 256 + threadIdx.x, e.y / nb);
+    } else if (WHICH == 1) {
+      const long ne = 4L * ((v.N2 - 2) + (v.N0 - 2) + (v.N1 - 2));
+      const int nb = (int)((ne + 255) / 256);
+      mur_edges_body<R>(P, N, M, v.ncomp, v.N0, v.N1, v.N2, v.mur, (long)(e.y % nb) * 256 + threadIdx.x, e.y / nb);
+    } else {
+      mur_corners_body<R>(P, N, M, v.ncomp, v.N0, v.N1, v.N2, v.mur, threadIdx.x, e.y);
     }
   }
 }
@@ -850,6 +955,67 @@ hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes) {
   if (tail)
     hipLaunchKernelGGL(k_fill_zero_bytes, dim3(1), dim3(64), 0, d->stream, (unsigned char*)ptr + n16 * 16, tail);
   return hipGetLastError();
+}
+
+
+// ---- batched launchers --------------------------------------------------------------------
+template <typename R, int TI, int K, bool LDS>
+hipError_t batch_field(BatchView* V, BatchTiles T, int kernel, int nsfd, hipStream_t s) {
+  // workgroups resident per CU, by shape: LDS forms are register- or LDS-bound, the others small
+  const int per_cu = LDS ? (TI >= 16 ? 1 : TI >= 8 ? 2 : 4) : (TI >= 16 ? 2 : TI >= 8 ? 4 : 8);
+  const int g = batch_grid(T.n, per_cu);
+  if (kernel == 0) {
+    if (nsfd) hipLaunchKernelGGL((kb_interior<R, true, TI, K, LDS>), dim3(g), dim3(kTileJ * TI), 0, s, V, T);
+    else hipLaunchKernelGGL((kb_interior<R, false, TI, K, LDS>), dim3(g), dim3(kTileJ * TI), 0, s, V, T);
+  } else {
+    hipLaunchKernelGGL((kb_eval<R, TI, K, LDS>), dim3(g), dim3(kTileJ * TI), 0, s, V, T);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t batch_field_variant(BatchView* V, BatchTiles T, int kernel, int variant, int nsfd, hipStream_t s) {
+  if (T.n == 0) return hipSuccess;
+  switch (variant) {
+    case 0: return batch_field<R, 4, 64, false>(V, T, kernel, nsfd, s);
+    case 1: return batch_field<R, 2, 64, false>(V, T, kernel, nsfd, s);
+    case 2: return batch_field<R, 4, 32, false>(V, T, kernel, nsfd, s);
+    case 3: return batch_field<R, 4, 128, false>(V, T, kernel, nsfd, s);
+    case 4: return batch_field<R, 8, 32, false>(V, T, kernel, nsfd, s);
+    case 5: return batch_field<R, 8, 64, false>(V, T, kernel, nsfd, s);
+    case 6: return batch_field<R, 16, 32, false>(V, T, kernel, nsfd, s);
+    case 7: return batch_field<R, 4, 64, true>(V, T, kernel, nsfd, s);
+    case 8: return batch_field<R, 8, 64, true>(V, T, kernel, nsfd, s);
+    case 9: return batch_field<R, 16, 64, true>(V, T, kernel, nsfd, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_interior(BatchView* V, BatchTiles T, int variant, int precision, int nsfd, hipStream_t s) {
+  return precision == MITHRA_PRECISION_FP64 ? batch_field_variant<double>(V, T, 0, variant, nsfd, s)
+                                            : batch_field_variant<float>(V, T, 0, variant, nsfd, s);
+}
+
+hipError_t launch_batch_eval(BatchView* V, BatchTiles T, int variant, int precision, hipStream_t s) {
+  return precision == MITHRA_PRECISION_FP64 ? batch_field_variant<double>(V, T, 1, variant, 0, s)
+                                            : batch_field_variant<float>(V, T, 1, variant, 0, s);
+}
+
+template <typename R>
+hipError_t batch_mur_impl(BatchView* V, BatchTiles F, BatchTiles E, BatchTiles C, hipStream_t s) {
+  if (F.n) hipLaunchKernelGGL((kb_mur<R, 0>), dim3(batch_grid(F.n, 8)), dim3(256), 0, s, V, F);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (E.n) hipLaunchKernelGGL((kb_mur<R, 1>), dim3(batch_grid(E.n, 8)), dim3(256), 0, s, V, E);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (C.n) hipLaunchKernelGGL((kb_mur<R, 2>), dim3(batch_grid(C.n, 8)), dim3(64), 0, s, V, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_mur(BatchView* V, BatchTiles faces, BatchTiles edges, BatchTiles corners, int precision,
+                            hipStream_t s) {
+  return precision == MITHRA_PRECISION_FP64 ? batch_mur_impl<double>(V, faces, edges, corners, s)
+                                            : batch_mur_impl<float>(V, faces, edges, corners, s);
 }
 
 }  // namespace mithra

@@ -8,18 +8,6 @@ namespace mithra {
 
 namespace {
 
-struct PushConst {
-  // mesh
-  double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz, Lz;
-  double ix, iy, iz;  // RN(1/dx), RN(1/dy), RN(1/dz) for div_rn
-  int N0, N1, N2;
-  // frame and Boris constants
-  double gamma, beta, c0, dt_shift, r1, r2, dtb;
-  // time of the first sub-step and the bunch time step added after each sub-step
-  double tb0, dtb_time;
-  int n_sub, n_und, save_rnm;
-  double dist0;  // undulator[0].dist_: entry test of the emission flag
-};
 
 __device__ inline double pmod(double a, double b) {  // src/stdinclude.cpp:88-93
   double x = fmod(a, b);
@@ -76,10 +64,9 @@ __device__ void undulator_field(const PushConst& c, const mithra_undulator* __re
 // E/B grid does not change between sub-steps, so a particle is read from HBM once per field
 // step and advanced n_sub times in registers; the time of each sub-step is accumulated exactly
 // as Solver::solve() accumulates timeBunch_ (src/solver.cpp:1316-1321).
-__global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long cap, long n,
-                                              const float4* __restrict__ eb, const mithra_undulator* __restrict__ und,
-                                              PushConst c, unsigned long long* __restrict__ counters) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void push_body(double* __restrict__ part, long cap, long n, const float4* __restrict__ eb,
+                                 const mithra_undulator* __restrict__ und, const PushConst& c,
+                                 unsigned long long* __restrict__ counters, long p) {
   if (p >= n) return;
   double* X = part;
   double x = X[PX * cap + p], y = X[PY * cap + p], z = X[PZ * cap + p];
@@ -187,6 +174,12 @@ __global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long ca
   if (ncross) atomicAdd(&counters[1], (unsigned long long)ncross);
 }
 
+__global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long cap, long n,
+                                              const float4* __restrict__ eb, const mithra_undulator* __restrict__ und,
+                                              PushConst c, unsigned long long* __restrict__ counters) {
+  push_body(part, cap, n, eb, und, c, counters, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // z extent of the bunch in cells (clamped to the mesh), for the lazy E/B evaluation: per
 // workgroup min / max, then one workgroup reduces the partials into zr[0..1].
 constexpr int kZBlocks = 1024;
@@ -245,11 +238,6 @@ __global__ void k_bunch_begin(double* __restrict__ part, long cap, long n) {
   part[PZM * cap + p] = part[PZ * cap + p];
 }
 
-struct DepositConst {
-  double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz;
-  double ix, iy, iz;        // RN(1/dx), RN(1/dy), RN(1/dz) for div_rn
-  int N0, N1, N2, sc;
-};
 
 // A deposit at cell (i, j, k) writes nodes i..i+1, j..j+1, k..k+1.
 __device__ inline bool cell_ok(const DepositConst& c, int i, int j, int k) {
@@ -498,14 +486,14 @@ __device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, d
 constexpr int kDepositThreads = 512;
 
 template <typename R, bool SC>
-__global__ __launch_bounds__(kDepositThreads) void k_deposit_rows(
-    const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
-    const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-    int TI, int tilesI, int accumulate) {
+__device__ inline void deposit_rows_body(const double* __restrict__ part, long cap, FieldSet J, const DepositConst& c,
+                                         const unsigned* __restrict__ p_row, const unsigned* __restrict__ p_idx,
+                                         const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
+                                         int TI, int tilesI, int accumulate, int block) {
   extern __shared__ double tile[];  // [ncomp][TI][N1]
   constexpr int ncomp = SC ? 4 : 3;
-  const int k = blockIdx.x / tilesI;
-  const int i0 = (blockIdx.x - k * tilesI) * TI;
+  const int k = block / tilesI;
+  const int i0 = (block - k * tilesI) * TI;
   const int ti = min(TI, c.N0 - i0);
   const int N1 = c.N1;
   const long N1N0 = (long)N1 * c.N0;
@@ -557,6 +545,14 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_rows(
   }
 }
 
+template <typename R, bool SC>
+__global__ __launch_bounds__(kDepositThreads) void k_deposit_rows(
+    const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
+    const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
+    int TI, int tilesI, int accumulate) {
+  deposit_rows_body<R, SC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, accumulate, blockIdx.x);
+}
+
 // Bunching factor partial sums: per workgroup sum of q cos(k z), q sin(k z), q over its particles.
 __global__ __launch_bounds__(256) void k_bunching(const double* __restrict__ part, long cap, long n, double k,
                                                   double* __restrict__ partial) {
@@ -581,6 +577,277 @@ __global__ __launch_bounds__(256) void k_bunching(const double* __restrict__ par
   }
   if (threadIdx.x == 0)
     for (int r = 0; r < 3; ++r) partial[3 * blockIdx.x + r] = s[r][0];
+}
+
+// ---- batched forms: persistent workgroups walking a table of (instance, local block) -------
+__device__ inline int batch_entry_p(int r) { return r * (int)gridDim.x + blockIdx.x; }
+
+__global__ __launch_bounds__(256) void kb_push(BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active) continue;
+    PushConst c = v.push;
+    c.tb0 = v.time_bunch;
+    push_body(v.part, v.cap, v.np, v.eb, v.und, c, v.counters, (long)e.y * 256 + threadIdx.x);
+  }
+}
+
+// Counting sort of one half of every particle by cell row (the batch's graph-safe replacement
+// of the radix sort): keys + counts, a three-phase exclusive scan of the counts into the row
+// starts, and a scatter that counts back down, leaving the counters at zero.  Halves that
+// deposit nothing form a last bucket (key = rows), so the index list is a permutation of all
+// particles, as the radix sort's is.
+__device__ inline unsigned batch_slot(const BatchView& v, int slot_sel) { return slot_sel ? 1 - v.parity : v.parity; }
+
+// Runs of equal keys in consecutive lanes: the run's first lane updates the counter for the
+// whole run (length len), every lane knows its place o in the run.  All 64 lanes take part.
+struct KeyRun {
+  bool head;
+  unsigned len, o;
+};
+
+__device__ inline KeyRun key_run(unsigned key) {
+  const int lane = __lane_id();
+  const unsigned prev = __shfl_up(key, 1), next = __shfl_down(key, 1);
+  const unsigned long long heads = __ballot(lane == 0 || prev != key);
+  const unsigned long long tails = __ballot(lane == 63 || next != key);
+  const int first = 63 - __builtin_clzll(heads & (~0ull >> (63 - lane)));  // last head at or before lane
+  const int last = lane + __builtin_ctzll(tails >> lane);                    // first tail at or after lane
+  KeyRun r;
+  r.head = lane == first;
+  r.len = (unsigned)(last - first + 1);
+  r.o = (unsigned)(lane - first);
+  return r;
+}
+
+constexpr unsigned kNoKey = 0xffffffffu;  // lanes past the particle count
+
+__global__ __launch_bounds__(256) void kb_sort_keys(BatchView* __restrict__ V, BatchTiles T, int use_rm,
+                                                    int slot_sel) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active) continue;  // uniform over the workgroup
+    const long p = (long)e.y * 256 + threadIdx.x;
+    unsigned k = kNoKey;
+    if (p < v.np) {
+      const DepositConst c = v.dep;
+      const int o = use_rm ? PXM : PX;
+      const double x = v.part[(o + 0) * v.cap + p], y = v.part[(o + 1) * v.cap + p], z = v.part[(o + 2) * v.cap + p];
+      k = v.rows;  // the bucket of halves that deposit nothing, last
+      if (inside(c, x, y, z)) {
+        const int i = (int)floor(div_rn(x - c.xmin, c.dx, c.ix)), j = (int)floor(div_rn(y - c.ymin, c.dy, c.iy)),
+                  kk = (int)floor(div_rn(z - c.zmin, c.dz, c.iz));
+        if (cell_ok(c, i, j, kk)) k = (unsigned)kk * (unsigned)c.N0 + (unsigned)i;
+      }
+      v.ord[batch_slot(v, slot_sel)].key[p] = k;
+    }
+    const KeyRun run = key_run(k);
+    if (k != kNoKey && run.head) atomicAdd(&v.cnt[k], run.len);
+  }
+}
+
+constexpr int kScanBlock = 1024;  // rows per scan block (256 threads x 4)
+
+// phase 1: exclusive scan of the counts within each block of 1024 rows (rows + 1 entries, the
+// last one zero), block totals into bsum
+__global__ __launch_bounds__(256) void kb_scan1(BatchView* __restrict__ V, BatchTiles T, int slot_sel) {
+  __shared__ unsigned s[256];
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active) continue;  // uniform over the workgroup
+    const long r0 = (long)e.y * kScanBlock + 4 * threadIdx.x;
+    unsigned c[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c[q] = (r0 + q <= (long)v.rows) ? v.cnt[r0 + q] : 0u;
+      sum += c[q];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int h = 1; h < 256; h <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
+      const unsigned add = threadIdx.x >= (unsigned)h ? s[threadIdx.x - h] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += add;
+      __syncthreads();
+    }
+    unsigned run = s[threadIdx.x] - sum;
+    unsigned* row = v.ord[batch_slot(v, slot_sel)].row;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (r0 + q <= (long)v.rows) row[r0 + q] = run;
+      run += c[q];
+    }
+    if (threadIdx.x == 255) v.bsum[e.y] = s[255];
+    __syncthreads();
+  }
+}
+
+// phase 2: exclusive scan of the block totals, one workgroup per instance
+__global__ __launch_bounds__(256) void kb_scan2(BatchView* __restrict__ V, BatchTiles T) {
+  __shared__ unsigned s[256];
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const BatchView& v = V[T.t[t].x];
+    if (!v.active) continue;
+    const int nb = (int)(((long)v.rows + 1 + kScanBlock - 1) / kScanBlock);
+    unsigned carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+      const int b = b0 + threadIdx.x;
+      const unsigned x = b < nb ? v.bsum[b] : 0u;
+      s[threadIdx.x] = x;
+      __syncthreads();
+      for (int h = 1; h < 256; h <<= 1) {
+        const unsigned add = threadIdx.x >= (unsigned)h ? s[threadIdx.x - h] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+      }
+      if (b < nb) v.bsum[b] = carry + s[threadIdx.x] - x;
+      carry += s[255];
+      __syncthreads();
+    }
+  }
+}
+
+// phase 3: add the block offsets
+__global__ __launch_bounds__(256) void kb_scan3(BatchView* __restrict__ V, BatchTiles T, int slot_sel) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active) continue;
+    const unsigned off = v.bsum[e.y];
+    unsigned* row = v.ord[batch_slot(v, slot_sel)].row;
+    for (int q = threadIdx.x; q < kScanBlock; q += 256) {
+      const long rr = (long)e.y * kScanBlock + q;
+      if (rr <= (long)v.rows) row[rr] += off;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_sort_scatter(BatchView* __restrict__ V, BatchTiles T, int slot_sel) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active) continue;  // uniform over the workgroup
+    const long p = (long)e.y * 256 + threadIdx.x;
+    const CellOrder o = v.ord[batch_slot(v, slot_sel)];
+    // every particle gets a position (k <= rows), so the list is a permutation of all particles
+    const unsigned k = p < v.np ? o.key[p] : kNoKey;
+    const KeyRun run = key_run(k);
+    unsigned end = 0;
+    if (k != kNoKey && run.head) end = atomicSub(&v.cnt[k], run.len);  // counts back down to zero
+    // the run's first lane holds the end of the run's range; lanes of one run are consecutive
+    const int lane = __lane_id();
+    end = __shfl(end, lane - (int)run.o);
+    if (k != kNoKey) o.idx[o.row[k] + end - run.len + run.o] = (unsigned)p;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (v.active) {
+      const CellOrder po = v.ord[v.parity];
+      const CellOrder mo = v.ord[1 - v.parity];
+      const DepositConst c = v.dep;  // a local copy: the LDS atomics cannot alias it
+      deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
+                                  v.dtilesI, 0, e.y);
+    }
+    __syncthreads();  // the LDS tile is reused by the next entry
+  }
+}
+
+// Reorder of the particle arrays into this step's rnp row order every `reorder` steps, after
+// the deposit (as the single-domain path reorders): a gather into the spare arrays, which
+// kb_advance swaps in; the order's index list becomes the identity.
+__device__ inline bool reorder_step(const BatchView& v) { return v.reorder > 0 && v.step % v.reorder == 0; }
+
+__global__ __launch_bounds__(256) void kb_reorder(BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    const long p = (long)e.y * 256 + threadIdx.x;
+    if (!v.active || !reorder_step(v) || p >= v.np) continue;
+    unsigned* perm = v.ord[v.parity].idx;
+    const long s = perm[p];
+#pragma unroll
+    for (int a = 0; a < PCOUNT; ++a) v.part_alt[a * v.cap + p] = v.part[a * v.cap + s];
+    v.pid_alt[p] = v.pid[s];
+    perm[p] = (unsigned)p;
+  }
+}
+
+// Bunching factor samples: pass 1 writes each particle block's (sum q cos, sum q sin, sum q);
+// pass 2 (one workgroup per instance) adds them in block order.
+__global__ __launch_bounds__(256) void kb_bunch1(BatchView* __restrict__ V, BatchTiles T) {
+  __shared__ double s[3][256];
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active || v.bevery <= 0 || v.step % v.bevery != 0 || v.step / v.bevery >= v.bslots) continue;
+    const long p = (long)e.y * 256 + threadIdx.x;
+    double c = 0.0, sn = 0.0, q = 0.0;
+    if (p < v.np) {
+      const double w = v.part[PQ * v.cap + p], z = v.part[PZ * v.cap + p];
+      double sv, cv;
+      sincos(v.bk * z, &sv, &cv);
+      c = w * cv;
+      sn = w * sv;
+      q = w;
+    }
+    s[0][threadIdx.x] = c;
+    s[1][threadIdx.x] = sn;
+    s[2][threadIdx.x] = q;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h)
+        for (int a = 0; a < 3; ++a) s[a][threadIdx.x] += s[a][threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x < 3) v.bpart[3 * (long)e.y + threadIdx.x] = s[threadIdx.x][0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void kb_bunch2(BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry_p(r);
+    if (t >= T.n) break;
+    const BatchView& v = V[T.t[t].x];
+    if (!v.active || v.bevery <= 0 || v.step % v.bevery != 0 || v.step / v.bevery >= v.bslots) continue;
+    if (threadIdx.x == 0) {
+      const long nb = (v.np + 255) / 256;
+      double c = 0.0, sn = 0.0, q = 0.0;
+      for (long b = 0; b < nb; ++b) {
+        c += v.bpart[3 * b];
+        sn += v.bpart[3 * b + 1];
+        q += v.bpart[3 * b + 2];
+      }
+      v.bout[v.step / v.bevery] = q != 0.0 ? sqrt(c * c + sn * sn) / fabs(q) : 0.0;
+    }
+  }
 }
 
 }  // namespace
@@ -687,6 +954,10 @@ hipError_t deposit_atomic(Domain* d) {
 
 }  // namespace
 
+int deposit_tile_rows(const Domain* d) { return tile_rows(d); }
+
+DepositConst make_deposit_const(const Domain* d) { return deposit_const(d); }
+
 bool deposit_tiled(const Domain* d) {
   const long rows = (long)(d->g.N2 - 1) * d->g.N0;
   return tile_rows(d) > 0 && rows < 0xFFFFFFF0L;
@@ -746,6 +1017,53 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   d->order1_is_rnp = 1;
   d->rm_order_valid = 0;
   return hipSuccess;
+}
+
+
+// ---- batched launchers --------------------------------------------------------------------
+hipError_t launch_batch_push(BatchView* V, BatchTiles T, hipStream_t s) {
+  if (T.n) hipLaunchKernelGGL(kb_push, dim3(batch_grid(T.n, 8)), dim3(256), 0, s, V, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_bunching(BatchView* V, BatchTiles particles, BatchTiles inst, hipStream_t s) {
+  if (particles.n) hipLaunchKernelGGL(kb_bunch1, dim3(batch_grid(particles.n, 8)), dim3(256), 0, s, V, particles);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !inst.n) return e;
+  hipLaunchKernelGGL(kb_bunch2, dim3(inst.n), dim3(64), 0, s, V, inst);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_sort(BatchView* V, BatchTiles particles, BatchTiles rowblocks, BatchTiles inst, int use_rm,
+                             int slot_sel, hipStream_t s) {
+  if (!particles.n) return hipSuccess;
+  const int gp = batch_grid(particles.n, 8);
+  hipLaunchKernelGGL(kb_sort_keys, dim3(gp), dim3(256), 0, s, V, particles, use_rm, slot_sel);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kb_scan1, dim3(batch_grid(rowblocks.n, 8)), dim3(256), 0, s, V, rowblocks, slot_sel);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(kb_scan2, dim3(inst.n), dim3(256), 0, s, V, inst);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(kb_scan3, dim3(batch_grid(rowblocks.n, 8)), dim3(256), 0, s, V, rowblocks, slot_sel);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(kb_sort_scatter, dim3(gp), dim3(256), 0, s, V, particles, slot_sel);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_reorder(BatchView* V, BatchTiles particles, hipStream_t s) {
+  if (particles.n) hipLaunchKernelGGL(kb_reorder, dim3(batch_grid(particles.n, 8)), dim3(256), 0, s, V, particles);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_deposit(BatchView* V, BatchTiles T, int precision, size_t lds, hipStream_t s) {
+  if (!T.n) return hipSuccess;
+  const int g = batch_grid(T.n, lds > (40u << 10) ? 2 : 3);
+  if (precision == MITHRA_PRECISION_FP64)
+    hipLaunchKernelGGL((kb_deposit<double>), dim3(g), dim3(kDepositThreads), lds, s, V, T);
+  else
+    hipLaunchKernelGGL((kb_deposit<float>), dim3(g), dim3(kDepositThreads), lds, s, V, T);
+  return hipGetLastError();
 }
 
 }  // namespace mithra

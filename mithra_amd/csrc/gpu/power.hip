@@ -12,13 +12,13 @@ namespace {
 
 constexpr int kPowerBlock = 256;
 
-__global__ __launch_bounds__(kPowerBlock) void k_power_nodes(
-    const float4* __restrict__ eb, double* __restrict__ ring, const double* __restrict__ dft,
-    const int* __restrict__ plane_k, const double* __restrict__ plane_dzr, double* __restrict__ partial, int N0,
-    int N1, unsigned Nz, unsigned Nl, unsigned Nf, unsigned slot, double gamma, double beta, double c0, double pc) {
-  const unsigned pz = blockIdx.y;
+__device__ inline void power_nodes_body(const float4* __restrict__ eb, double* __restrict__ ring,
+                                        const double* __restrict__ dft, const int* __restrict__ plane_k,
+                                        const double* __restrict__ plane_dzr, double* __restrict__ partial, int N0,
+                                        int N1, unsigned Nz, unsigned Nl, unsigned Nf, unsigned slot, double gamma,
+                                        double beta, double c0, double pc, int bx, int nbx, unsigned pz) {
   const long ni_count = (long)(N0 - 4) * (N1 - 4);
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)bx * blockDim.x + threadIdx.x;
   const long N1N0 = (long)N1 * N0;
   const size_t ring_slot = (size_t)4 * Nz * N1N0;  // doubles per slot
   __shared__ double red[kPowerBlock];
@@ -63,14 +63,21 @@ __global__ __launch_bounds__(kPowerBlock) void k_power_nodes(
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[((size_t)pz * gridDim.x + blockIdx.x) * Nl + l] = red[0];
+    if (threadIdx.x == 0) partial[((size_t)pz * nbx + bx) * Nl + l] = red[0];
     __syncthreads();
   }
 }
 
-__global__ void k_power_reduce(const double* __restrict__ partial, const int* __restrict__ plane_of,
-                               double* __restrict__ row, unsigned N, unsigned Nl, int blocks) {
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kPowerBlock) void k_power_nodes(
+    const float4* __restrict__ eb, double* __restrict__ ring, const double* __restrict__ dft,
+    const int* __restrict__ plane_k, const double* __restrict__ plane_dzr, double* __restrict__ partial, int N0,
+    int N1, unsigned Nz, unsigned Nl, unsigned Nf, unsigned slot, double gamma, double beta, double c0, double pc) {
+  power_nodes_body(eb, ring, dft, plane_k, plane_dzr, partial, N0, N1, Nz, Nl, Nf, slot, gamma, beta, c0, pc,
+                   blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+__device__ inline void power_reduce_body(const double* __restrict__ partial, const int* __restrict__ plane_of,
+                                         double* __restrict__ row, unsigned N, unsigned Nl, int blocks, unsigned t) {
   if (t >= N * Nl) return;
   const unsigned k = t / Nl, l = t % Nl;
   const int pz = plane_of[k];
@@ -78,6 +85,36 @@ __global__ void k_power_reduce(const double* __restrict__ partial, const int* __
   if (pz >= 0)
     for (int b = 0; b < blocks; ++b) s += partial[((size_t)pz * blocks + b) * Nl + l];
   row[t] = s;
+}
+
+__global__ void k_power_reduce(const double* __restrict__ partial, const int* __restrict__ plane_of,
+                               double* __restrict__ row, unsigned N, unsigned Nl, int blocks) {
+  power_reduce_body(partial, plane_of, row, N, Nl, blocks, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(kPowerBlock) void kb_power_nodes(const BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = r * (int)gridDim.x + blockIdx.x;
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active || v.step >= (long)v.prows) continue;  // uniform over the workgroup
+    power_nodes_body(v.eb, v.ring, v.dft, v.pk, v.pdzr, v.partial, v.N0, v.N1, v.pNz, v.pNl, v.pNf,
+                     (unsigned)(v.step % v.pNf), v.pgamma, v.pbeta, v.pc0, v.pc, e.y % v.pblocks, v.pblocks,
+                     e.y / v.pblocks);
+  }
+}
+
+__global__ __launch_bounds__(64) void kb_power_reduce(const BatchView* __restrict__ V, BatchTiles T) {
+  for (int r = 0;; ++r) {
+    const int t = r * (int)gridDim.x + blockIdx.x;
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active || v.step >= (long)v.prows) continue;
+    power_reduce_body(v.partial, v.plane_of, v.history + (size_t)v.step * v.pN * v.pNl, v.pN, v.pNl, v.pblocks,
+                      (unsigned)e.y * 64 + threadIdx.x);
+  }
 }
 
 }  // namespace
@@ -96,6 +133,15 @@ hipError_t launch_power(Domain* d, unsigned n_time, unsigned row) {
   const unsigned n = d->pN * d->pNl;
   hipLaunchKernelGGL(k_power_reduce, dim3((n + 63) / 64), dim3(64), 0, d->stream, d->partial,
                      (const int*)(d->pk + d->pNz), out, d->pN, d->pNl, d->pblocks);
+  return hipGetLastError();
+}
+
+
+hipError_t launch_batch_power(BatchView* V, BatchTiles nodes, BatchTiles reduce, hipStream_t s) {
+  if (nodes.n) hipLaunchKernelGGL(kb_power_nodes, dim3(batch_grid(nodes.n, 4)), dim3(kPowerBlock), 0, s, V, nodes);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (reduce.n) hipLaunchKernelGGL(kb_power_reduce, dim3(batch_grid(reduce.n, 8)), dim3(64), 0, s, V, reduce);
   return hipGetLastError();
 }
 

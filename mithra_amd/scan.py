@@ -1,0 +1,258 @@
+"""FEL parameter scans on one GPU: the seeded instance generator, each instance's set-up through
+the drop-in host, and the two ways to march a scan -- instance after instance through the
+single-domain path, or all instances together as one batch.
+
+An instance is a boosted-frame slice simulation of a planar undulator FEL, written as an
+ordinary job file (so any instance can be run, or checked, on its own):
+
+* beam energy gamma, undulator parameter K, period lambda_u and relative energy spread are
+  drawn from the scan ranges; the resonant wavelength lambda_r = lambda_u (1 + K^2/2) / (2 gamma^2)
+  sets the lab-frame cell length dz = lambda_r / 16;
+* the mesh has n_perp x n_perp x n_z cells, the transverse cell dx = 2 gamma dz (above the
+  NSFD solver's dx >= 1.44 gamma dz, so the reference keeps the mesh as written);
+* the bunch is Gaussian with sigma_r and sigma_z one eighth of the mesh, n_particles macro-
+  particles of a 1 kA beam, normalised transverse momentum spread 2e-3;
+* the march runs n_z field steps (with dt = dz/c in the frame, the light-crossing time of the
+  window), and P(z) and the bunching factor are sampled at SAMPLES evenly spaced steps, the
+  power on one plane ahead of the bunch at the resonant frequency.
+"""
+import dataclasses
+import math
+import os
+import time
+
+import numpy as np
+
+from . import abi
+from .job import Job
+
+SAMPLES = 128
+C0_UM_PS = 299.792458          # speed of light in um / ps
+
+JOB = """MESH
+{{
+  length-scale            = MICROMETER
+  time-scale              = PICOSECOND
+  mesh-lengths            = ( {lx:.10g}, {lx:.10g}, {lz:.10g} )
+  mesh-resolution         = ( {dx:.10g}, {dx:.10g}, {dz:.10g} )
+  mesh-center             = ( 0.0, 0.0, 0.0 )
+  total-time              = 1.0e6
+  mesh-truncation-order   = 2
+  space-charge            = false
+  solver                  = NSFD
+}}
+BUNCH
+{{
+  bunch-initialization
+  {{
+    type                  = other
+    charge                = {charge:.10g}
+    number-of-particles   = {n}
+    gamma                 = {gamma:.10g}
+    direction             = ( 0.0, 0.0, 1.0 )
+  }}
+}}
+UNDULATOR
+{{
+  static-undulator
+  {{
+    undulator-parameter   = {K:.10g}
+    period                = {lu:.10g}
+    length                = 100000
+    polarization-angle    = 0.0
+  }}
+}}
+FEL-OUTPUT
+{{
+  radiation-power
+  {{
+    sample                = true
+    type                  = at-point
+    directory             = {dir}/
+    base-name             = power-sampling/power-{idx}
+    plane-position        = {zp:.10g}
+    normalized-frequency  = 1.00
+  }}
+}}
+"""
+
+
+@dataclasses.dataclass
+class Instance:
+    """One scan point (lab frame; lengths in um)."""
+    n_perp: int
+    n_z: int
+    n_particles: int
+    gamma: float
+    K: float
+    lambda_u: float
+    spread: float
+    seed: int
+
+    @property
+    def lambda_r(self):
+        return self.lambda_u * (1.0 + self.K ** 2 / 2.0) / (2.0 * self.gamma ** 2)
+
+    @property
+    def dz(self):
+        return self.lambda_r / 16.0
+
+    @property
+    def dx(self):
+        return 2.0 * self.gamma * self.dz
+
+    @property
+    def n_steps(self):
+        return self.n_z
+
+    @property
+    def every(self):
+        return max(1, self.n_steps // SAMPLES)
+
+
+def generate(n=64, seed=4, n_perp=(32, 128), n_z=(128, 1024), n_particles=(1e4, 1e6)):
+    """The scan: sizes log-uniform, gamma U[100, 2000], K U[0.5, 2], lambda_u U[1, 3] cm,
+    energy spread log-uniform in [1e-4, 1e-2]."""
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi):
+        return math.exp(rng.uniform(math.log(lo), math.log(hi)))
+
+    out = []
+    for i in range(n):
+        out.append(Instance(n_perp=int(round(logu(*n_perp))), n_z=int(round(logu(*n_z))),
+                            n_particles=int(round(logu(*n_particles))), gamma=rng.uniform(100.0, 2000.0),
+                            K=rng.uniform(0.5, 2.0), lambda_u=rng.uniform(1.0e4, 3.0e4),
+                            spread=logu(1e-4, 1e-2), seed=int(rng.integers(1 << 31))))
+    return out
+
+
+def bunch_records(inst):
+    """Lab-frame records q x y z gbx gby gbz of the instance's Gaussian bunch."""
+    rng = np.random.default_rng(inst.seed)
+    n = inst.n_particles
+    lx, lz = (inst.n_perp - 1) * inst.dx, (inst.n_z - 1) * inst.dz
+    sr, sz = lx / 8.0, lz / 8.0
+    out = np.empty((n, 7))
+    electrons = 1000.0 * (math.sqrt(2.0 * math.pi) * sz * 1e-6) / 2.99792458e8 / 1.602e-19
+    out[:, 0] = electrons / n
+    out[:, 1] = rng.normal(0.0, sr, n)
+    out[:, 2] = rng.normal(0.0, sr, n)
+    out[:, 3] = rng.normal(0.0, sz, n)
+    out[:, 4] = rng.normal(0.0, 2e-3, n)
+    out[:, 5] = rng.normal(0.0, 2e-3, n)
+    out[:, 6] = math.sqrt(inst.gamma ** 2 - 1.0) * (1.0 + inst.spread * rng.normal(0.0, 1.0, n))
+    return out, electrons
+
+
+def job_text(inst, idx, workdir, electrons):
+    # the reference makes (int)(L / d) + 2 nodes of a mesh length L (src/solver.cpp:610-612)
+    lx, lz = (inst.n_perp - 1.5) * inst.dx, (inst.n_z - 1.5) * inst.dz
+    return JOB.format(lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=inst.n_particles,
+                      gamma=inst.gamma, K=inst.K, lu=inst.lambda_u, dir=workdir, idx=idx, zp=0.375 * lz)
+
+
+def set_up(inst, idx, workdir):
+    """Writes the instance's job file and runs the drop-in host's set-up on it."""
+    bunch, electrons = bunch_records(inst)
+    path = os.path.join(workdir, f"scan-{idx}.job")
+    with open(path, "w") as f:
+        f.write(job_text(inst, idx, workdir, electrons))
+    return Job(path, external_bunch=bunch)
+
+
+def open_domain(job, precision, rows):
+    """A device domain holding the job's state before its first step."""
+    g, c = job.grid(), job.coeffs()
+    dom = abi.Domain(g, c, precision=precision)
+    an, fn = job.potentials(0)
+    am, fm = job.potentials(1)
+    dom.set_potentials(0, an, fn)
+    dom.set_potentials(1, am, fm)
+    dom.set_particles(job.particles())
+    N, Nl, Nf, pc, z, w = job.power()
+    bp = job.bunch_params()
+    dom.power_setup(abi.make_power_params(z, w, Nf, pc, bp.gamma, bp.beta, bp.c0), rows)
+    return dom
+
+
+def bunching_wavenumber(job):
+    """Frame wavenumber of the sampled (resonant) frequency: omega / c."""
+    N, Nl, Nf, pc, z, w = job.power()
+    return float(w[0]) / job.bunch_params().c0
+
+
+class Prepared:
+    """One instance set up on the device: its job, domain and march constants."""
+
+    def __init__(self, inst, idx, workdir, precision):
+        self.inst, self.idx = inst, idx
+        self.job = set_up(inst, idx, workdir)
+        self.t = self.job.times()
+        self.bp, self.und, self.seed = self.job.bunch_params(), self.job.undulators(), self.job.seed()
+        self.k = bunching_wavenumber(self.job)
+        self.dom = open_domain(self.job, precision, inst.n_steps)
+
+    def result(self, bunching):
+        """P(z) and b(z) at the SAMPLES sample steps."""
+        P = self.dom.power_fetch(0, self.inst.n_steps, 1, 1)[:, 0, 0]
+        idx = np.arange(SAMPLES) * self.inst.every
+        idx = idx[idx < self.inst.n_steps]
+        return P[idx], np.asarray(bunching)[:len(idx)]
+
+
+def prepare(instances, workdir, precision=abi.PRECISION_FP32_BULK, log=None):
+    out = []
+    for i, inst in enumerate(instances):
+        out.append(Prepared(inst, i, workdir, precision))
+        if log and (i + 1) % 8 == 0:
+            log(f"set up {i + 1}/{len(instances)} instances")
+    return out
+
+
+def march_single(p):
+    """The plain single-instance GPU path: mithra_dev_step per field step, the bunching factor
+    sampled after the steps the batch samples."""
+    t = dict(p.t)
+    time_, tb = t["time"], t["time_bunch"]
+    bunch = []
+    for s in range(p.inst.n_steps):
+        p.dom.step(time_, p.seed, p.bp, p.und, tb, t["bunch_time_step"], t["n_sub"], s, s)
+        if s % p.inst.every == 0 and len(bunch) < SAMPLES:
+            bunch.append(p.dom.bunching(p.k))
+        for _ in range(t["n_sub"]):
+            tb += t["bunch_time_step"]
+        time_ += t["dt"]
+    p.dom.synchronize()
+    return p.result(bunch)
+
+
+def run_single(prepared):
+    """All instances one after the other; returns (results, seconds)."""
+    t0 = time.perf_counter()
+    res = [march_single(p) for p in prepared]
+    return res, time.perf_counter() - t0
+
+
+def make_batch(prepared, variant=None, use_graph=True, autotune=True):
+    b = abi.Batch([p.dom for p in prepared])
+    for i, p in enumerate(prepared):
+        b.instance(i, p.t["time"], p.t["time_bunch"], p.t["bunch_time_step"], p.t["n_sub"], p.inst.n_steps, p.bp,
+                   p.und, p.k, p.inst.every, SAMPLES)
+    tuned = None
+    if autotune:
+        tuned = b.autotune()
+    if variant is not None or not use_graph:
+        b.set_tiles(variant if variant is not None else (tuned[0] if tuned else 8), use_graph)
+    return b, tuned
+
+
+def run_batch(prepared, variant=None, use_graph=True, autotune=True):
+    """All instances as one batch; returns (results, seconds of the march, autotune result)."""
+    b, tuned = make_batch(prepared, variant, use_graph, autotune)
+    t0 = time.perf_counter()
+    b.run(max(p.inst.n_steps for p in prepared))
+    seconds = time.perf_counter() - t0
+    res = [p.result(b.bunching(i)) for i, p in enumerate(prepared)]
+    b.close()
+    return res, seconds, tuned

@@ -1,0 +1,74 @@
+"""Batched scans: every instance of a batch marches exactly as it would on its own.
+
+A small scan of uneven instances (different meshes, particle counts, beams and undulators,
+all set up through the drop-in host) is marched twice from identical state: instance by
+instance through the single-domain C-ABI (mithra_dev_step), and all together through
+mithra_batch_run, as a captured graph and launched step by step.  P(z) and the bunching
+factor at the sample steps, and the final particles, must agree to the rounding of the
+deposit's summation order, the only arithmetic that differs."""
+import tempfile
+
+import numpy as np
+import pytest
+
+from golden_state import rel_l2
+from mithra_amd import abi, scan
+
+pytestmark = pytest.mark.gpu
+
+
+def small_scan(n=4, seed=11):
+    return scan.generate(n, seed=seed, n_perp=(12, 28), n_z=(40, 90), n_particles=(2000, 9000))
+
+
+@pytest.mark.parametrize("precision,graph,tol", [
+    (abi.PRECISION_FP64, True, 1e-12),
+    (abi.PRECISION_FP64, False, 1e-12),
+    (abi.PRECISION_FP32_BULK, True, 1e-5),
+], ids=["fp64-graph", "fp64-eager", "fp32bulk-graph"])
+def test_batch_equals_single_instance_runs(precision, graph, tol):
+    insts = small_scan()
+    print("scan", [(i.n_perp, i.n_z, i.n_particles) for i in insts], flush=True)
+    with tempfile.TemporaryDirectory() as w1, tempfile.TemporaryDirectory() as w2:
+        single = scan.prepare(insts, w1, precision)
+        batch = scan.prepare(insts, w2, precision)
+        res_s, _ = scan.run_single(single)
+        res_b, _, tuned = scan.run_batch(batch, variant=None if graph else 8, use_graph=graph)
+        for i, ((ps, bs), (pb, bb)) in enumerate(zip(res_s, res_b)):
+            print(f"instance {i}: |P| {np.max(np.abs(ps)):.3e} dP {rel_l2(pb, ps):.2e} "
+                  f"b {np.max(bs):.3e} db {np.max(np.abs(bb - bs)):.2e}", flush=True)
+            assert len(pb) == len(ps) and len(bb) == len(bs) > 0
+            assert rel_l2(pb, ps) <= tol
+            assert np.max(np.abs(bb - bs)) <= tol * max(1e-3, np.max(np.abs(bs)))
+            qs, qb = single[i].dom.get_particles(), batch[i].dom.get_particles()
+            assert rel_l2(qb, qs) <= tol
+
+
+def test_batch_continues_in_single_domain_calls():
+    """After a batch run stops part-way, each domain continues through mithra_dev_step from
+    where the batch left it, exactly like an uninterrupted single-domain march."""
+    insts = small_scan(3, seed=12)
+    with tempfile.TemporaryDirectory() as w1, tempfile.TemporaryDirectory() as w2:
+        ref = scan.prepare(insts, w1, abi.PRECISION_FP64)
+        scan.run_single(ref)
+        mixed = scan.prepare(insts, w2, abi.PRECISION_FP64)
+        b, _ = scan.make_batch(mixed, autotune=False)
+        half = 17
+        assert b.run(half) == half
+        assert all(b.steps(i) == half for i in range(len(mixed)))
+        b.close()
+        for i, p in enumerate(mixed):
+            t = dict(p.t)
+            time_, tb = t["time"], t["time_bunch"]
+            for s in range(p.inst.n_steps):
+                if s >= half:
+                    p.dom.step(time_, p.seed, p.bp, p.und, tb, t["bunch_time_step"], t["n_sub"], s, s)
+                for _ in range(t["n_sub"]):
+                    tb += t["bunch_time_step"]
+                time_ += t["dt"]
+            n = p.inst.n_steps
+            P = p.dom.power_fetch(0, n, 1, 1)[:, 0, 0]
+            P_ref = ref[i].dom.power_fetch(0, n, 1, 1)[:, 0, 0]
+            print(f"instance {i}: dP {rel_l2(P, P_ref):.2e}", flush=True)
+            assert rel_l2(P, P_ref) <= 1e-12
+            assert rel_l2(p.dom.get_particles(), ref[i].dom.get_particles()) <= 1e-12
