@@ -281,6 +281,32 @@ int mithra_batch_run(mithra_batch* batch, long steps, long* done);
 int mithra_batch_steps(const mithra_batch* batch, int i, long* step);
 int mithra_batch_bunching(mithra_batch* batch, int i, double* out);
 
+/* ---- learned surrogate: an ensemble of multilayer perceptrons --------------------------
+ * `members` networks of `layers` fully connected layers of widths dims[0] -> dims[1] -> ... ->
+ * dims[layers], SiLU after every layer but the last.  Weights live in FP32 (the Adam master)
+ * and BF16 (what the products read); every product is a BF16 x BF16 -> FP32 MFMA GEMM.  The
+ * parameters of one member are laid out W_0 (dims[0] x dims[1], row-major), b_0, W_1, b_1, ...
+ * Inputs and outputs cross the boundary as FP32 host arrays, n rows each.  No counterpart in
+ * the reference (a CPU FDTD/PIC code). */
+typedef struct mithra_mlp mithra_mlp;
+int mithra_mlp_create(int members, int layers, const int* dims, unsigned long long seed, mithra_mlp** out);
+int mithra_mlp_destroy(mithra_mlp* mlp);
+long mithra_mlp_param_count(const mithra_mlp* mlp);  /* per member */
+int mithra_mlp_set_params(mithra_mlp* mlp, int member, const float* params);  /* member -1: all */
+int mithra_mlp_get_params(mithra_mlp* mlp, int member, float* params);
+/* y: members x n x dims[layers] */
+int mithra_mlp_forward(mithra_mlp* mlp, const float* x, long n, float* y);
+/* mean squared error over n x dims[layers] targets and its gradient for one member */
+int mithra_mlp_gradients(mithra_mlp* mlp, int member, const float* x, const float* t, long n, float* grads,
+                         float* loss);
+/* one Adam step (0.9, 0.999, 1e-8) of every member on one mini-batch; loss[members] before it */
+int mithra_mlp_train_step(mithra_mlp* mlp, const float* x, const float* t, long n, float lr, float* loss);
+/* one product of the surrogate's GEMM on host data (BF16 as 16-bit patterns): C (M x N, FP32,
+ * row-major) = op(A) op(B) with op(A)(m,k) = A[m*sam + k*sak], op(B)(k,n) = B[k*sbk + n*sbn];
+ * reps timed launches, mean time in *ms */
+int mithra_mlp_gemm(const unsigned short* A, long sam, long sak, long a_len, const unsigned short* B, long sbk,
+                    long sbn, long b_len, float* C, int M, int N, int K, int reps, float* ms);
+
 /* Bunching factor b = |sum q exp(i k z)| / |sum q| over all particles at their current boosted-
  * frame z.  With k = 2 pi / lambda', lambda' = lambda_u / (gamma lambda_norm) the radiation
  * wavelength in the frame (the one the power sampler uses, src/radiation.cpp:91-92), this is

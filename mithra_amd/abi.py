@@ -414,3 +414,120 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def bf16_bits(x):
+    """float32 -> bfloat16 bit patterns (round to nearest even), as the device converts."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+    r = ((u >> 16) & 1) + 0x7FFF
+    return ((u + r) >> 16).astype(np.uint16)
+
+
+def bf16_round(x):
+    """float32 values rounded to bfloat16 and back."""
+    return (bf16_bits(x).astype(np.uint32) << 16).view(np.float32)
+
+
+class MLP:
+    """Ensemble of multilayer perceptrons on the GPU (mithra_mlp_*, include/mithra_gpu.h)."""
+
+    def __init__(self, dims, members=5, seed=1):
+        self.lib = load_gpu_library()
+        vp, fp = C.c_void_p, C.POINTER(C.c_float)
+        for name, res, args in (
+            ("mithra_mlp_create", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_ulonglong, C.POINTER(vp)]),
+            ("mithra_mlp_destroy", C.c_int, [vp]),
+            ("mithra_mlp_param_count", C.c_long, [vp]),
+            ("mithra_mlp_set_params", C.c_int, [vp, C.c_int, fp]),
+            ("mithra_mlp_get_params", C.c_int, [vp, C.c_int, fp]),
+            ("mithra_mlp_forward", C.c_int, [vp, fp, C.c_long, fp]),
+            ("mithra_mlp_gradients", C.c_int, [vp, C.c_int, fp, fp, C.c_long, fp, fp]),
+            ("mithra_mlp_train_step", C.c_int, [vp, fp, fp, C.c_long, C.c_float, fp]),
+        ):
+            f = getattr(self.lib, name)
+            f.restype, f.argtypes = res, args
+        self.dims, self.members = list(dims), members
+        d = (C.c_int * len(dims))(*dims)
+        self.h = vp()
+        self._check(self.lib.mithra_mlp_create(members, len(dims) - 1, d, seed, C.byref(self.h)))
+        self.n_params = self.lib.mithra_mlp_param_count(self.h)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MithraError(f"status {rc}: {self.lib.mithra_strerror(rc).decode()}")
+
+    @staticmethod
+    def _f(a):
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    def params(self, member=-1):
+        n = self.n_params * (self.members if member < 0 else 1)
+        out = np.empty(n, np.float32)
+        self._check(self.lib.mithra_mlp_get_params(self.h, member, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p, member=-1):
+        p = self._f(p)
+        self._check(self.lib.mithra_mlp_set_params(self.h, member, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, x):
+        x = self._f(x)
+        n = x.shape[0]
+        y = np.empty((self.members, n, self.dims[-1]), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.mithra_mlp_forward(self.h, x.ctypes.data_as(fp), n, y.ctypes.data_as(fp)))
+        return y
+
+    def gradients(self, member, x, t):
+        x, t = self._f(x), self._f(t)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.mithra_mlp_gradients(self.h, member, x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                                  x.shape[0], g.ctypes.data_as(fp), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, x, t, lr):
+        x, t = self._f(x), self._f(t)
+        loss = np.empty(self.members, np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.mithra_mlp_train_step(self.h, x.ctypes.data_as(fp), t.ctypes.data_as(fp), x.shape[0],
+                                                   lr, loss.ctypes.data_as(fp)))
+        return loss
+
+    def close(self):
+        if self.h:
+            self.lib.mithra_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gemm_bf16(A, B, transpose_a=False, transpose_b=False, reps=1):
+    """C = op(A) op(B) through the surrogate's MFMA GEMM (inputs rounded to BF16); returns
+    (C, mean ms per launch)."""
+    lib = load_gpu_library()
+    f = lib.mithra_mlp_gemm
+    u16p, fp = C.POINTER(C.c_ushort), C.POINTER(C.c_float)
+    f.restype = C.c_int
+    f.argtypes = [u16p, C.c_long, C.c_long, C.c_long, u16p, C.c_long, C.c_long, C.c_long, fp, C.c_int, C.c_int,
+                  C.c_int, C.c_int, fp]
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = (A.shape[1], A.shape[0]) if transpose_a else A.shape
+    K2, N = (B.shape[1], B.shape[0]) if transpose_b else B.shape
+    assert K == K2
+    sam, sak = (1, A.shape[1]) if transpose_a else (A.shape[1], 1)
+    sbk, sbn = (1, B.shape[1]) if transpose_b else (B.shape[1], 1)
+    a16, b16 = bf16_bits(A), bf16_bits(B)
+    out = np.empty((M, N), np.float32)
+    ms = C.c_float()
+    rc = f(a16.ctypes.data_as(u16p), sam, sak, a16.size, b16.ctypes.data_as(u16p), sbk, sbn, b16.size,
+           out.ctypes.data_as(fp), M, N, K, reps, C.byref(ms))
+    if rc != 0:
+        raise MithraError(f"mithra_mlp_gemm: status {rc}")
+    return out, ms.value
