@@ -15,7 +15,10 @@ Mur, E/B), bunch push, P(z) sample, shift, current reset, deposition.
 
 Prints ONE JSON line.  value = cell-updates/s (nodes x steps / s); particle-pushes/s and the
 roofline of the dominant kernel (the interior stencil) are reported beside it, with the CPU
-oracle timed on a bounded sample of the same workload on this host.
+oracle timed on a bounded sample of the same workload on this host.  scan_points_per_hour is
+the batched parameter scan's rate (64 seeded instances of uneven size, each a job taken to its
+P(z) and bunching at 128 positions, host set-up included), measured in the same run beside the
+same scan through the single-instance path.
 """
 import argparse
 import json
@@ -200,6 +203,39 @@ def run_gpu(job, steps, warmup, precision):
     return elapsed, phases, counters, P, tiles
 
 
+def run_scan(n, work):
+    """Batched parameter scan (seed 4: meshes 32-128 transverse x 128-1024 longitudinal cells,
+    1e4-1e6 macro-particles, FP32-bulk fields, FP64 particles): the scan once as one batch and
+    once instance by instance through the single-domain path.  A scan point is a job taken to
+    its P(z) and b(z) at 128 positions; the times include each instance's host set-up."""
+    from mithra_amd import scan
+    insts = scan.generate(n, seed=4)
+    log(f"scan: {n} instances, {sum(i.n_perp ** 2 * i.n_z for i in insts):.3g} nodes, "
+        f"{sum(i.n_particles for i in insts):.3g} particles")
+    t0 = time.perf_counter()
+    prep = scan.prepare(insts, work, abi.PRECISION_FP32_BULK)
+    t_setup_b = time.perf_counter() - t0
+    _, t_batch, tuned = scan.run_batch(prep)
+    for p in prep:
+        p.dom.close()
+        p.job.close()
+    log(f"scan batched: set-up {t_setup_b:.1f} s, march {t_batch:.2f} s (tile shape {tuned[0]})")
+    t0 = time.perf_counter()
+    prep = scan.prepare(insts, work, abi.PRECISION_FP32_BULK)
+    t_setup_s = time.perf_counter() - t0
+    _, t_single = scan.run_single(prep)
+    for p in prep:
+        p.dom.close()
+        p.job.close()
+    log(f"scan single-instance: set-up {t_setup_s:.1f} s, march {t_single:.2f} s")
+    return {"instances": n, "precision": "fp32-bulk fields, fp64 particles",
+            "batched_setup_s": t_setup_b, "batched_march_s": t_batch, "single_setup_s": t_setup_s,
+            "single_march_s": t_single, "tile_shape": list(abi.tile_shape(tuned[0])),
+            "points_per_hour_batched": 3600.0 * n / (t_setup_b + t_batch),
+            "points_per_hour_single": 3600.0 * n / (t_setup_s + t_single),
+            "points_per_hour_batched_march_only": 3600.0 * n / t_batch}
+
+
 def run_cpu_baseline(job, steps, threads):
     """The CPU oracle (C++ -O3, OpenMP over nodes/particles, sequential deposition) on a bounded
     sample of the same workload: `steps` full steps of the same mesh and bunch."""
@@ -248,6 +284,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--scan", type=int, default=64,
+                    help="instances of the batched parameter scan measured for scan points/hour (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the z-slab multi-GPU "
@@ -272,6 +310,7 @@ def main():
         bytes_per_node = 96 if precision == abi.PRECISION_FP64 else 48
         t_int = phases[abi.PHASES.index("interior")] * 1e-3
         achieved = bytes_per_node * interior / t_int / 1e9
+        scan_out = run_scan(args.scan, work) if args.scan > 0 else None
         copy_gbs = abi.copy_bandwidth()
         copy8_gbs = abi.copy_bandwidth(elem_bytes=8)
         cpu = None
@@ -302,7 +341,8 @@ def main():
                    "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
                    "precision": args.precision, "parallelism": "single GPU, whole domain resident"},
         "particle_pushes_per_s": push_rate,
-        "scan_points_per_hour": None,
+        "scan_points_per_hour": scan_out["points_per_hour_batched"] if scan_out else None,
+        "scan": scan_out,
         "phase_ms": {k: round(float(v), 4) for k, v in zip(abi.PHASES, phases)},
         "roofline": {"bound": "hbm", "kernel": "k_interior (NSFD stencil)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -19,6 +19,18 @@
 
 namespace mithra {
 
+// Pointers into device memory (address space 1).  Kernel bodies receive their buffers through
+// structures (field sets, batch views); casting them back to global pointers lets the compiler
+// use global_load/store instead of flat accesses, which also count against the LDS wait counter.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+// Four floats as a builtin vector (float4's member operators cannot act on address space 1).
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ inline gptr<T> G(T* p) {
+  return (gptr<T>)p;
+}
+
 // Constants of the reference (src/stdinclude.h:43-50); PI is truncated on purpose.
 constexpr double kPI = 3.1415926535;
 constexpr double kEC = 1.602e-19;

@@ -30,8 +30,8 @@ struct Plane5 {  // one z plane of a column and its four transverse neighbours
   double c, xp, xm, yp, ym;
 };
 
-template <typename R>
-__device__ inline Plane5 load_plane(const R* __restrict__ a, long m, long N1) {
+template <typename PT>
+__device__ inline Plane5 load_plane(PT a, long m, long N1) {
   Plane5 p;
   p.c = (double)a[m];
   p.xp = (double)a[m + N1];
@@ -61,9 +61,9 @@ __device__ inline void interior_body(FieldSet an_set, FieldSet anm1_set, FieldSe
   const int kb = 1 + kc * K;
   const int ke = min(kb + K, N2 - 1);
   if (comp >= ncomp || i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
-  const R* __restrict__ an = (const R*)an_set.c[comp];
-  const R* __restrict__ anm1 = (const R*)anm1_set.c[comp];
-  R* __restrict__ anp1 = (R*)anp1_set.c[comp];
+  const gptr<const R> an = G((const R*)an_set.c[comp]);
+  const gptr<const R> anm1 = G((const R*)anm1_set.c[comp]);
+  const gptr<R> anp1 = G((R*)anp1_set.c[comp]);
   const double as = comp < 3 ? a4 : a5;
   const long N1N0 = (long)N1 * N0;
   long m = N1N0 * kb + (long)N1 * i + j;
@@ -122,9 +122,9 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
   const int j0 = tj * kTileJ, i0 = 1 + ti * TI;
   const int j = j0 + lane, i = i0 + w;
   const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
-  const R* __restrict__ an = (const R*)an_set.c[comp];
-  const R* __restrict__ anm1 = (const R*)anm1_set.c[comp];
-  R* __restrict__ anp1 = (R*)anp1_set.c[comp];
+  const gptr<const R> an = G((const R*)an_set.c[comp]);
+  const gptr<const R> anm1 = G((const R*)anm1_set.c[comp]);
+  const gptr<R> anp1 = G((R*)anp1_set.c[comp]);
   const double as = comp < 3 ? a4 : a5;
   const long N1N0 = (long)N1 * N0;
   // What this thread stages per plane: its own node, and (wave 0 / wave 1) the halo rows
@@ -336,19 +336,18 @@ __global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid
 // node, o1 and o2 are the two inward offsets and w the offset along the edge.  For a corner,
 // sx, sy, sz are the three inward offsets.
 
-template <typename R>
-__device__ inline double rd(const R* p, long m) { return (double)p[m]; }
+template <typename PT>
+__device__ inline double rd(PT p, long m) { return (double)p[m]; }
 
-template <typename R>
-__device__ inline double mur_face(const double* B, const R* P, const R* N, const R* M, long m, long s, long t, long u) {
+template <typename TP, typename TN>
+__device__ inline double mur_face(const double* B, TP P, TN N, TN M, long m, long s, long t, long u) {
   return B[0] * (rd(M, m) + rd(P, m + s)) + B[1] * rd(M, m + s) + B[2] * (rd(N, m) + rd(N, m + s)) +
          B[3] * (rd(N, m + s + t) + rd(N, m + s - t) + rd(N, m + t) + rd(N, m - t)) +
          B[4] * (rd(N, m + s + u) + rd(N, m + s - u) + rd(N, m + u) + rd(N, m - u));
 }
 
-template <typename R>
-__device__ inline double mur_edge(const double* B, const R* P, const R* N, const R* M, long m, long o1, long o2,
-                                  long w) {
+template <typename TP, typename TN>
+__device__ inline double mur_edge(const double* B, TP P, TN N, TN M, long m, long o1, long o2, long w) {
   const long o3 = o1 + o2;
   return B[0] * (rd(P, m + o1) + rd(M, m + o2)) + B[1] * (rd(M, m + o1) + rd(P, m + o2)) +
          B[2] * (rd(M, m) + rd(P, m + o3)) + B[3] * (rd(N, m) + rd(N, m + o1) + rd(N, m + o2) + rd(N, m + o3)) +
@@ -357,8 +356,8 @@ __device__ inline double mur_edge(const double* B, const R* P, const R* N, const
          rd(M, m + o3);
 }
 
-template <typename R>
-__device__ inline double mur_corner(const double* h, const R* P, const R* N, const R* M, long m, long sx, long sy,
+template <typename TP, typename TN>
+__device__ inline double mur_corner(const double* h, TP P, TN N, TN M, long m, long sx, long sy,
                                     long sz) {
   const long o[7] = {sx, sy, sz, sx + sy, sz + sx, sz + sy, sz + sx + sy};
   double s = rd(N, m) * h[16] + rd(M, m) * h[8];
@@ -375,9 +374,9 @@ __device__ inline void mur_faces_body(FieldSet P, FieldSet N, FieldSet M, int nc
   const long X = N1, Y = 1, Z = N1N0;
   const long nx = (long)(N1 - 2) * (N2 - 2), ny = (long)(N0 - 2) * (N2 - 2), nz = (long)(N0 - 2) * (N1 - 2);
   if (comp >= ncomp) return;
-  const R* pn = (const R*)N.c[comp];
-  const R* pm = (const R*)M.c[comp];
-  R* pp = (R*)P.c[comp];
+  const gptr<const R> pn = G((const R*)N.c[comp]);
+  const gptr<const R> pm = G((const R*)M.c[comp]);
+  const gptr<R> pp = G((R*)P.c[comp]);
   long m, s, tt, u;
   const double* B;
   if (t < 2 * nx) {
@@ -417,9 +416,9 @@ __device__ inline void mur_edges_body(FieldSet P, FieldSet N, FieldSet M, int nc
   const long X = N1, Y = 1, Z = N1N0;
   const long ez = N2 - 2, ex = N0 - 2, ey = N1 - 2;
   if (comp >= ncomp) return;
-  const R* pn = (const R*)N.c[comp];
-  const R* pm = (const R*)M.c[comp];
-  R* pp = (R*)P.c[comp];
+  const gptr<const R> pn = G((const R*)N.c[comp]);
+  const gptr<const R> pm = G((const R*)M.c[comp]);
+  const gptr<R> pp = G((R*)P.c[comp]);
   long m, o1, o2, w;
   const double* B;
   if (t < 4 * ez) {  // along z at the four (x, y) corners
@@ -458,9 +457,9 @@ __device__ inline void mur_corners_body(FieldSet P, FieldSet N, FieldSet M, int 
   const long N1N0 = (long)N1 * N0;
   const bool xh = t & 1, yh = t & 2, zh = t & 4;
   const long m = (xh ? N1N0 - N1 : 0) + (yh ? N1 - 1 : 0) + (zh ? N1N0 * (N2 - 1) : 0);
-  const R* pn = (const R*)N.c[comp];
-  const R* pm = (const R*)M.c[comp];
-  R* pp = (R*)P.c[comp];
+  const gptr<const R> pn = G((const R*)N.c[comp]);
+  const gptr<const R> pm = G((const R*)M.c[comp]);
+  const gptr<R> pp = G((R*)P.c[comp]);
   pp[m] = (R)mur_corner(c.hC, pp, pn, pm, m, xh ? -(long)N1 : N1, yh ? -1L : 1L, zh ? -N1N0 : N1N0);
 }
 
@@ -492,6 +491,7 @@ template <typename R, bool SC, int TI, int K>
 __device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1, int N2,
                                  int tilesJ, int tilesI, int klo, int khi, const EvalConst& q,
                                  const int* __restrict__ zr, int pb0, int pb1, int lb) {
+  const gptr<f4> ebg = G(reinterpret_cast<f4*>(eb));
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
@@ -503,13 +503,13 @@ __device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ 
   if (zr && !lazy_needed(kb, ke, zr, pb0, pb1)) return;
   if (i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
   const long N1N0 = (long)N1 * N0;
-  const R* __restrict__ n0 = (const R*)An.c[0];
-  const R* __restrict__ n1 = (const R*)An.c[1];
-  const R* __restrict__ n2 = (const R*)An.c[2];
-  const R* __restrict__ p0 = (const R*)Ap.c[0];
-  const R* __restrict__ p1 = (const R*)Ap.c[1];
-  const R* __restrict__ p2 = (const R*)Ap.c[2];
-  const R* __restrict__ f = SC ? (const R*)An.c[3] : nullptr;
+  const gptr<const R> n0 = G((const R*)An.c[0]);
+  const gptr<const R> n1 = G((const R*)An.c[1]);
+  const gptr<const R> n2 = G((const R*)An.c[2]);
+  const gptr<const R> p0 = G((const R*)Ap.c[0]);
+  const gptr<const R> p1 = G((const R*)Ap.c[1]);
+  const gptr<const R> p2 = G((const R*)Ap.c[2]);
+  const gptr<const R> f = G(SC ? (const R*)An.c[3] : (const R*)nullptr);
   const double mdt = q.mdt, dx2 = q.dx2, dy2 = q.dy2, dz2 = q.dz2;
   auto dt_ = [&](double x) { return div_rn(x, mdt, q.rmdt); };
   auto dx_ = [&](double x) { return div_rn(x, dx2, q.rdx2); };
@@ -541,17 +541,17 @@ __device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ 
                                     dz_(qp0[2] - qp0[0]) - dx_(rd(p2, m + N1) - rd(p2, m - N1))));
     const float bz = (float)(0.5 * (dx_(rd(n1, m + N1) - rd(n1, m - N1)) - dy_(rd(n0, m + 1) - rd(n0, m - 1)) +
                                     dx_(rd(p1, m + N1) - rd(p1, m - N1)) - dy_(rd(p0, m + 1) - rd(p0, m - 1))));
-    const float4 v0 = make_float4((float)ex, (float)ey, ez, bx);
-    const float4 v1 = make_float4(by, bz, 0.0f, 0.0f);
-    eb[2 * m] = v0;
-    eb[2 * m + 1] = v1;
+    const f4 v0 = {(float)ex, (float)ey, ez, bx};
+    const f4 v1 = {by, bz, 0.0f, 0.0f};
+    ebg[2 * m] = v0;
+    ebg[2 * m + 1] = v1;
     if (k == 1) {
-      eb[2 * (m - N1N0)] = v0;
-      eb[2 * (m - N1N0) + 1] = v1;
+      ebg[2 * (m - N1N0)] = v0;
+      ebg[2 * (m - N1N0) + 1] = v1;
     }
     if (k == N2 - 2) {
-      eb[2 * (m + N1N0)] = v0;
-      eb[2 * (m + N1N0) + 1] = v1;
+      ebg[2 * (m + N1N0)] = v0;
+      ebg[2 * (m + N1N0) + 1] = v1;
     }
     qn0[0] = qn0[1]; qn1[0] = qn1[1]; qp0[0] = qp0[1]; qp1[0] = qp1[1];
     qn0[1] = qn0[2]; qn1[1] = qn1[2]; qp0[1] = qp0[2]; qp1[1] = qp1[2];
@@ -575,6 +575,7 @@ template <typename R, bool SC, int TI, int K>
 __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1, int N2,
                                      int tilesJ, int tilesI, int klo, int khi, const EvalConst& q,
                                      const int* __restrict__ zr, int pb0, int pb1, int lb) {
+  const gptr<f4> ebg = G(reinterpret_cast<f4*>(eb));
   constexpr int NA = SC ? 7 : 6;      // staged arrays: n0 n1 n2 p0 p1 p2 [phi]
   constexpr int NZ = SC ? 5 : 4;      // arrays with z neighbours: n0 n1 p0 p1 [phi]
   constexpr int W = kTileJ + 2, P = (TI + 2) * W;
@@ -592,8 +593,9 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restric
   const int j = j0 + lane, i = i0 + w;
   const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
   const long N1N0 = (long)N1 * N0;
-  const R* A[7] = {(const R*)An.c[0], (const R*)An.c[1], (const R*)An.c[2], (const R*)Ap.c[0],
-                   (const R*)Ap.c[1], (const R*)Ap.c[2], SC ? (const R*)An.c[3] : nullptr};
+  const gptr<const R> A[7] = {G((const R*)An.c[0]), G((const R*)An.c[1]), G((const R*)An.c[2]),
+                              G((const R*)Ap.c[0]), G((const R*)Ap.c[1]), G((const R*)Ap.c[2]),
+                              G(SC ? (const R*)An.c[3] : (const R*)nullptr)};
   constexpr int zarr[5] = {0, 1, 3, 4, 6};  // n0 n1 p0 p1 phi
   const bool own_ok = i <= N0 - 1 && j <= N1 - 1;
   const long own_off = (long)N1 * i + j;
@@ -681,17 +683,17 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restric
                                       dz_(qz[2][2] - qz[2][0]) - dx_(p2l[W] - p2l[-W])));
       const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
                                       dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
-      const float4 v0 = make_float4((float)ex, (float)ey, ez, bx);
-      const float4 v1 = make_float4(by, bz, 0.0f, 0.0f);
-      eb[2 * m] = v0;
-      eb[2 * m + 1] = v1;
+      const f4 v0 = {(float)ex, (float)ey, ez, bx};
+      const f4 v1 = {by, bz, 0.0f, 0.0f};
+      ebg[2 * m] = v0;
+      ebg[2 * m + 1] = v1;
       if (k == 1) {
-        eb[2 * (m - N1N0)] = v0;
-        eb[2 * (m - N1N0) + 1] = v1;
+        ebg[2 * (m - N1N0)] = v0;
+        ebg[2 * (m - N1N0) + 1] = v1;
       }
       if (k == N2 - 2) {
-        eb[2 * (m + N1N0)] = v0;
-        eb[2 * (m + N1N0) + 1] = v1;
+        ebg[2 * (m + N1N0)] = v0;
+        ebg[2 * (m + N1N0) + 1] = v1;
       }
     }
     __syncthreads();

@@ -17,10 +17,11 @@ __device__ inline double pmod(double a, double b) {  // src/stdinclude.cpp:88-93
 
 // Solver::undulatorField for static undulators with Solver::staticUndulator
 // (src/solver.cpp:1798-1824, src/beam.cc:14-76): adds the boosted-frame fields at r.
-__device__ void undulator_field(const PushConst& c, const mithra_undulator* __restrict__ und, double x, double y,
-                                double z, double tb, double et[3], double bt[3]) {
+template <typename UT>
+__device__ void undulator_field(const PushConst& c, UT und, double x, double y, double z, double tb, double et[3],
+                                double bt[3]) {
   for (int u = 0; u < c.n_und; ++u) {
-    const mithra_undulator U = und[u];
+    const mithra_undulator U = *(const mithra_undulator*)(und + u);
     const double b0 = (U.lu != 0.0) ? kEM * c.c0 * 2 * kPI / U.lu * U.k / kEC : 0.0;
     const double ku = (U.lu != 0.0) ? 2 * kPI / U.lu : 0.0;
     const double ct = cos(U.theta), st = sin(U.theta);
@@ -35,13 +36,13 @@ __device__ void undulator_field(const PushConst& c, const mithra_undulator* __re
       const double s = (lz < 0.0) ? lz : lz - L;     // distance into the fringe
       double sz = exp(-pow(ku * s, 2) / 2.0);
       if (lz < 0.0 && u != 0) {
-        const mithra_undulator V = und[u - 1];
+        const mithra_undulator V = *(const mithra_undulator*)(und + (u - 1));
         const double r0 = V.rb + V.length * V.lu - U.rb;
         if (lz < r0 || r0 == 0.0) sz = 0.0;
         else sz *= 0.35875 + 0.48829 * cos(kPI * lz / r0) + 0.14128 * cos(2.0 * kPI * lz / r0) +
                    0.01168 * cos(3.0 * kPI * lz / r0);
       } else if (lz > L && u + 1 != c.n_und) {
-        const mithra_undulator V = und[u + 1];
+        const mithra_undulator V = *(const mithra_undulator*)(und + (u + 1));
         const double r0 = V.rb - U.rb - L;
         if (s > r0 || r0 == 0.0) sz = 0.0;
         else sz *= 0.35875 + 0.48829 * cos(kPI * s / r0) + 0.14128 * cos(2.0 * kPI * s / r0) +
@@ -68,7 +69,9 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
                                  const mithra_undulator* __restrict__ und, const PushConst& c,
                                  unsigned long long* __restrict__ counters, long p) {
   if (p >= n) return;
-  double* X = part;
+  const gptr<double> X = G(part);
+  const gptr<const f4> ebg = G(reinterpret_cast<const f4*>(eb));
+  const gptr<const mithra_undulator> undg = G(und);
   double x = X[PX * cap + p], y = X[PY * cap + p], z = X[PZ * cap + p];
   double gx = X[PGX * cap + p], gy = X[PGY * cap + p], gz = X[PGZ * cap + p];
   double e = X[PE * cap + p];
@@ -87,7 +90,7 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
     const bool b1y = (y < c.ymax - c.dy && y > c.ymin + c.dy);
     const bool b1z = (z < c.zp1 && z >= c.zp0);
     double bt[3] = {0.0, 0.0, 0.0}, et[3] = {0.0, 0.0, 0.0};
-    undulator_field(c, und, x, y, z, tb, et, bt);
+    undulator_field(c, undg, x, y, z, tb, et, bt);
     if (e == 1.0) {
       if (b1x && b1y && b1z) {
         double ip, jp, kp;
@@ -105,11 +108,11 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
                              (1.0 - dxr) * dyr * (1.0 - dzr),         dxr * dyr * (1.0 - dzr),
                              (1.0 - dxr) * (1.0 - dyr) * dzr,         dxr * (1.0 - dyr) * dzr,
                              (1.0 - dxr) * dyr * dzr,                 dxr * dyr * dzr};
-        float4 q0[8], q1[8];
+        f4 q0[8], q1[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-          q0[t] = eb[2 * (m + off[t])];
-          q1[t] = eb[2 * (m + off[t]) + 1];
+          q0[t] = ebg[2 * (m + off[t])];
+          q1[t] = ebg[2 * (m + off[t]) + 1];
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -434,7 +437,8 @@ struct HalfItem {
   bool ok;
 };
 
-__device__ inline void load_item(HalfItem& it, const double* __restrict__ part, long cap, long p) {
+template <typename PT>
+__device__ inline void load_item(HalfItem& it, PT part, long cap, long p) {
   it.rp[0] = part[PX * cap + p];
   it.rp[1] = part[PY * cap + p];
   it.rp[2] = part[PZ * cap + p];
@@ -501,7 +505,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   const long base = N1N0 * k + (long)N1 * i0;
   for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
     const int comp = t / own, o = t - comp * own;
-    tile[comp * tsz + o] = accumulate ? (double)((const R*)J.c[comp])[base + o] : 0.0;
+    tile[comp * tsz + o] = accumulate ? (double)G((const R*)J.c[comp])[base + o] : 0.0;
   }
   // The ranges: (half 0 | 1) x (plane k-1 | k), rows lo..hi of each.
   const int lo = max(i0 - 1, 0), hi = min(i0 + ti - 1, c.N0 - 2);
@@ -511,7 +515,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   if (lo <= hi)
     for (int half = 0; half < 2; ++half)
       for (int kk = max(k - 1, 0); kk <= min(k, c.N2 - 2); ++kk) {
-        const unsigned* row = half ? m_row : p_row;
+        const gptr<const unsigned> row = G(half ? m_row : p_row);
         start[nq] = row[(long)kk * c.N0 + lo];
         len[nq] = row[(long)kk * c.N0 + hi + 1] - start[nq];
         qhalf[nq] = half;
@@ -525,7 +529,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
     while (q < nq - 1 && t >= len[q]) t -= len[q++];
     it.half = qhalf[q];
     it.kk = qk[q];
-    return (long)(qhalf[q] ? m_idx : p_idx)[start[q] + t];
+    return (long)G(qhalf[q] ? m_idx : p_idx)[start[q] + t];
   };
   for (unsigned t = threadIdx.x; t < total; t += 2 * blockDim.x) {
     HalfItem a, b;
@@ -533,15 +537,15 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
     const long pa = locate(t, a);
     b.ok = tb < total;
     const long pb = b.ok ? locate(tb, b) : pa;
-    load_item(a, part, cap, pa);
-    load_item(b, part, cap, pb);
+    load_item(a, G(part), cap, pa);
+    load_item(b, G(part), cap, pb);
     deposit_item<SC>(a, c, tile, TI, k, i0, ti, lo, hi);
     if (b.ok) deposit_item<SC>(b, c, tile, TI, k, i0, ti, lo, hi);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
     const int comp = t / own, o = t - comp * own;
-    ((R*)J.c[comp])[base + o] = (R)tile[comp * tsz + o];
+    G((R*)J.c[comp])[base + o] = (R)tile[comp * tsz + o];
   }
 }
 
@@ -638,14 +642,15 @@ __global__ __launch_bounds__(256) void kb_sort_keys(BatchView* __restrict__ V, B
     if (p < v.np) {
       const DepositConst c = v.dep;
       const int o = use_rm ? PXM : PX;
-      const double x = v.part[(o + 0) * v.cap + p], y = v.part[(o + 1) * v.cap + p], z = v.part[(o + 2) * v.cap + p];
+      const gptr<const double> part = G((const double*)v.part);
+      const double x = part[(o + 0) * v.cap + p], y = part[(o + 1) * v.cap + p], z = part[(o + 2) * v.cap + p];
       k = v.rows;  // the bucket of halves that deposit nothing, last
       if (inside(c, x, y, z)) {
         const int i = (int)floor(div_rn(x - c.xmin, c.dx, c.ix)), j = (int)floor(div_rn(y - c.ymin, c.dy, c.iy)),
                   kk = (int)floor(div_rn(z - c.zmin, c.dz, c.iz));
         if (cell_ok(c, i, j, kk)) k = (unsigned)kk * (unsigned)c.N0 + (unsigned)i;
       }
-      v.ord[batch_slot(v, slot_sel)].key[p] = k;
+      G(v.ord[batch_slot(v, slot_sel)].key)[p] = k;
     }
     const KeyRun run = key_run(k);
     if (k != kNoKey && run.head) atomicAdd(&v.cnt[k], run.len);
@@ -668,7 +673,7 @@ __global__ __launch_bounds__(256) void kb_scan1(BatchView* __restrict__ V, Batch
     unsigned c[4], sum = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      c[q] = (r0 + q <= (long)v.rows) ? v.cnt[r0 + q] : 0u;
+      c[q] = (r0 + q <= (long)v.rows) ? G((const unsigned*)v.cnt)[r0 + q] : 0u;
       sum += c[q];
     }
     s[threadIdx.x] = sum;
@@ -680,13 +685,13 @@ __global__ __launch_bounds__(256) void kb_scan1(BatchView* __restrict__ V, Batch
       __syncthreads();
     }
     unsigned run = s[threadIdx.x] - sum;
-    unsigned* row = v.ord[batch_slot(v, slot_sel)].row;
+    const gptr<unsigned> row = G(v.ord[batch_slot(v, slot_sel)].row);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (r0 + q <= (long)v.rows) row[r0 + q] = run;
       run += c[q];
     }
-    if (threadIdx.x == 255) v.bsum[e.y] = s[255];
+    if (threadIdx.x == 255) G(v.bsum)[e.y] = s[255];
     __syncthreads();
   }
 }
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(256) void kb_scan2(BatchView* __restrict__ V, Batch
     unsigned carry = 0;
     for (int b0 = 0; b0 < nb; b0 += 256) {
       const int b = b0 + threadIdx.x;
-      const unsigned x = b < nb ? v.bsum[b] : 0u;
+      const unsigned x = b < nb ? G((const unsigned*)v.bsum)[b] : 0u;
       s[threadIdx.x] = x;
       __syncthreads();
       for (int h = 1; h < 256; h <<= 1) {
@@ -712,7 +717,7 @@ __global__ __launch_bounds__(256) void kb_scan2(BatchView* __restrict__ V, Batch
         s[threadIdx.x] += add;
         __syncthreads();
       }
-      if (b < nb) v.bsum[b] = carry + s[threadIdx.x] - x;
+      if (b < nb) G(v.bsum)[b] = carry + s[threadIdx.x] - x;
       carry += s[255];
       __syncthreads();
     }
@@ -727,8 +732,8 @@ __global__ __launch_bounds__(256) void kb_scan3(BatchView* __restrict__ V, Batch
     const int2 e = T.t[t];
     const BatchView& v = V[e.x];
     if (!v.active) continue;
-    const unsigned off = v.bsum[e.y];
-    unsigned* row = v.ord[batch_slot(v, slot_sel)].row;
+    const unsigned off = G((const unsigned*)v.bsum)[e.y];
+    const gptr<unsigned> row = G(v.ord[batch_slot(v, slot_sel)].row);
     for (int q = threadIdx.x; q < kScanBlock; q += 256) {
       const long rr = (long)e.y * kScanBlock + q;
       if (rr <= (long)v.rows) row[rr] += off;
@@ -746,14 +751,14 @@ __global__ __launch_bounds__(256) void kb_sort_scatter(BatchView* __restrict__ V
     const long p = (long)e.y * 256 + threadIdx.x;
     const CellOrder o = v.ord[batch_slot(v, slot_sel)];
     // every particle gets a position (k <= rows), so the list is a permutation of all particles
-    const unsigned k = p < v.np ? o.key[p] : kNoKey;
+    const unsigned k = p < v.np ? G((const unsigned*)o.key)[p] : kNoKey;
     const KeyRun run = key_run(k);
     unsigned end = 0;
     if (k != kNoKey && run.head) end = atomicSub(&v.cnt[k], run.len);  // counts back down to zero
     // the run's first lane holds the end of the run's range; lanes of one run are consecutive
     const int lane = __lane_id();
     end = __shfl(end, lane - (int)run.o);
-    if (k != kNoKey) o.idx[o.row[k] + end - run.len + run.o] = (unsigned)p;
+    if (k != kNoKey) G(o.idx)[G((const unsigned*)o.row)[k] + end - run.len + run.o] = (unsigned)p;
   }
 }
 
@@ -788,11 +793,13 @@ __global__ __launch_bounds__(256) void kb_reorder(BatchView* __restrict__ V, Bat
     const BatchView& v = V[e.x];
     const long p = (long)e.y * 256 + threadIdx.x;
     if (!v.active || !reorder_step(v) || p >= v.np) continue;
-    unsigned* perm = v.ord[v.parity].idx;
+    const gptr<unsigned> perm = G(v.ord[v.parity].idx);
+    const gptr<const double> src = G((const double*)v.part);
+    const gptr<double> dst = G(v.part_alt);
     const long s = perm[p];
 #pragma unroll
-    for (int a = 0; a < PCOUNT; ++a) v.part_alt[a * v.cap + p] = v.part[a * v.cap + s];
-    v.pid_alt[p] = v.pid[s];
+    for (int a = 0; a < PCOUNT; ++a) dst[a * v.cap + p] = src[a * v.cap + s];
+    G(v.pid_alt)[p] = G((const unsigned*)v.pid)[s];
     perm[p] = (unsigned)p;
   }
 }
@@ -810,7 +817,8 @@ __global__ __launch_bounds__(256) void kb_bunch1(BatchView* __restrict__ V, Batc
     const long p = (long)e.y * 256 + threadIdx.x;
     double c = 0.0, sn = 0.0, q = 0.0;
     if (p < v.np) {
-      const double w = v.part[PQ * v.cap + p], z = v.part[PZ * v.cap + p];
+      const gptr<const double> part = G((const double*)v.part);
+      const double w = part[PQ * v.cap + p], z = part[PZ * v.cap + p];
       double sv, cv;
       sincos(v.bk * z, &sv, &cv);
       c = w * cv;
@@ -826,7 +834,7 @@ __global__ __launch_bounds__(256) void kb_bunch1(BatchView* __restrict__ V, Batc
         for (int a = 0; a < 3; ++a) s[a][threadIdx.x] += s[a][threadIdx.x + h];
       __syncthreads();
     }
-    if (threadIdx.x < 3) v.bpart[3 * (long)e.y + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < 3) G(v.bpart)[3 * (long)e.y + threadIdx.x] = s[threadIdx.x][0];
     __syncthreads();
   }
 }
@@ -840,12 +848,13 @@ __global__ __launch_bounds__(64) void kb_bunch2(BatchView* __restrict__ V, Batch
     if (threadIdx.x == 0) {
       const long nb = (v.np + 255) / 256;
       double c = 0.0, sn = 0.0, q = 0.0;
+      const gptr<const double> bp = G((const double*)v.bpart);
       for (long b = 0; b < nb; ++b) {
-        c += v.bpart[3 * b];
-        sn += v.bpart[3 * b + 1];
-        q += v.bpart[3 * b + 2];
+        c += bp[3 * b];
+        sn += bp[3 * b + 1];
+        q += bp[3 * b + 2];
       }
-      v.bout[v.step / v.bevery] = q != 0.0 ? sqrt(c * c + sn * sn) / fabs(q) : 0.0;
+      G(v.bout)[v.step / v.bevery] = q != 0.0 ? sqrt(c * c + sn * sn) / fabs(q) : 0.0;
     }
   }
 }

@@ -19,6 +19,9 @@ __device__ inline void power_nodes_body(const float4* __restrict__ eb, double* _
                                         double beta, double c0, double pc, int bx, int nbx, unsigned pz) {
   const long ni_count = (long)(N0 - 4) * (N1 - 4);
   const long t = (long)bx * blockDim.x + threadIdx.x;
+  const gptr<const f4> ebg = G(reinterpret_cast<const f4*>(eb));
+  const gptr<double> ringg = G(ring);
+  const gptr<const double> dftg = G(dft);
   const long N1N0 = (long)N1 * N0;
   const size_t ring_slot = (size_t)4 * Nz * N1N0;  // doubles per slot
   __shared__ double red[kPowerBlock];
@@ -27,26 +30,26 @@ __device__ inline void power_nodes_body(const float4* __restrict__ eb, double* _
     if (t < ni_count) {
       const long i = 2 + t / (N1 - 4), j = 2 + t % (N1 - 4);
       const long node = (long)N1 * i + j;
-      const long mi = (long)plane_k[pz] * N1N0 + node;
+      const long mi = (long)G(plane_k)[pz] * N1N0 + node;
       const size_t base = (size_t)pz * N1N0 + node;
       if (l == 0) {
-        const double dzr = plane_dzr[pz];
-        const float4 a0 = eb[2 * mi], a1 = eb[2 * mi + 1];
-        const float4 b0 = eb[2 * (mi + N1N0)], b1 = eb[2 * (mi + N1N0) + 1];
+        const double dzr = G(plane_dzr)[pz];
+        const f4 a0 = ebg[2 * mi], a1 = ebg[2 * mi + 1];
+        const f4 b0 = ebg[2 * (mi + N1N0)], b1 = ebg[2 * (mi + N1N0) + 1];
         const double et0 = (1.0 - dzr) * (double)a0.x + dzr * (double)b0.x;
         const double et1 = (1.0 - dzr) * (double)a0.y + dzr * (double)b0.y;
         const double bt0 = (1.0 - dzr) * (double)a0.w + dzr * (double)b0.w;
         const double bt1 = (1.0 - dzr) * (double)a1.x + dzr * (double)b1.x;
-        double* r = ring + (size_t)slot * ring_slot;
+        const gptr<double> r = ringg + (size_t)slot * ring_slot;
         r[0 * Nz * N1N0 + base] = gamma * (et0 + c0 * beta * bt1);
         r[1 * Nz * N1N0 + base] = gamma * (et1 - c0 * beta * bt0);
         r[2 * Nz * N1N0 + base] = gamma * (bt0 - beta / c0 * et1);
         r[3 * Nz * N1N0 + base] = gamma * (bt1 + beta / c0 * et0);
       }
       double e1r = 0.0, e1i = 0.0, b1r = 0.0, b1i = 0.0, e2r = 0.0, e2i = 0.0, b2r = 0.0, b2i = 0.0;
-      const double* w = dft + (size_t)l * Nf * 2;
+      const gptr<const double> w = dftg + (size_t)l * Nf * 2;
       for (unsigned m = 0; m < Nf; ++m) {
-        const double* r = ring + (size_t)m * ring_slot;
+        const gptr<const double> r = ringg + (size_t)m * ring_slot;
         const double f0 = r[0 * Nz * N1N0 + base], f1 = r[1 * Nz * N1N0 + base];
         const double f2 = r[2 * Nz * N1N0 + base], f3 = r[3 * Nz * N1N0 + base];
         const double cs = w[2 * m], sn = w[2 * m + 1];
@@ -63,7 +66,7 @@ __device__ inline void power_nodes_body(const float4* __restrict__ eb, double* _
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[((size_t)pz * nbx + bx) * Nl + l] = red[0];
+    if (threadIdx.x == 0) G(partial)[((size_t)pz * nbx + bx) * Nl + l] = red[0];
     __syncthreads();
   }
 }
@@ -80,11 +83,11 @@ __device__ inline void power_reduce_body(const double* __restrict__ partial, con
                                          double* __restrict__ row, unsigned N, unsigned Nl, int blocks, unsigned t) {
   if (t >= N * Nl) return;
   const unsigned k = t / Nl, l = t % Nl;
-  const int pz = plane_of[k];
+  const int pz = G(plane_of)[k];
   double s = 0.0;
   if (pz >= 0)
-    for (int b = 0; b < blocks; ++b) s += partial[((size_t)pz * blocks + b) * Nl + l];
-  row[t] = s;
+    for (int b = 0; b < blocks; ++b) s += G(partial)[((size_t)pz * blocks + b) * Nl + l];
+  G(row)[t] = s;
 }
 
 __global__ void k_power_reduce(const double* __restrict__ partial, const int* __restrict__ plane_of,

@@ -10,8 +10,11 @@ ordinary job file (so any instance can be run, or checked, on its own):
   sets the lab-frame cell length dz = lambda_r / 16;
 * the mesh has n_perp x n_perp x n_z cells, the transverse cell dx = 2 gamma dz (above the
   NSFD solver's dx >= 1.44 gamma dz, so the reference keeps the mesh as written);
-* the bunch is Gaussian with sigma_r and sigma_z one eighth of the mesh, n_particles macro-
-  particles of a 1 kA beam, normalised transverse momentum spread 2e-3;
+* the bunch is Gaussian with sigma_r and sigma_z a fraction of the mesh (one eighth by default),
+  loaded at random or by a quiet start (beamlets lambda_r / 4 apart, optionally with an imposed
+  initial bunching at lambda_r),
+  n_particles macro-particles of a beam of the given peak current (1 kA by default), normalised
+  transverse momentum spread emittance / sigma_r (2e-3 by default);
 * the march runs n_z field steps (with dt = dz/c in the frame, the light-crossing time of the
   window), and P(z) and the bunching factor are sampled at SAMPLES evenly spaced steps, the
   power on one plane ahead of the bunch at the resonant frequency.
@@ -88,6 +91,13 @@ class Instance:
     lambda_u: float
     spread: float
     seed: int
+    emittance: float = 0.0    # normalised emittance (um rad); 0: transverse gamma-beta spread 2e-3
+    current: float = 1000.0   # peak current (A)
+    sr_frac: float = 0.125    # sigma_r / transverse mesh length
+    sz_frac: float = 0.125    # sigma_z / longitudinal mesh length
+    quiet: bool = False       # quiet start: four beamlets lambda_r / 4 apart (no shot noise at lambda_r)
+    bunching: float = 0.0     # imposed initial bunching at lambda_r (amplitude, with `phase`)
+    phase: float = 0.0
 
     @property
     def lambda_r(self):
@@ -128,27 +138,45 @@ def generate(n=64, seed=4, n_perp=(32, 128), n_z=(128, 1024), n_particles=(1e4, 
 
 
 def bunch_records(inst):
-    """Lab-frame records q x y z gbx gby gbz of the instance's Gaussian bunch."""
+    """Lab-frame records q x y z gbx gby gbz of the instance's Gaussian bunch, and the number of
+    electrons."""
     rng = np.random.default_rng(inst.seed)
     n = inst.n_particles
     lx, lz = (inst.n_perp - 1) * inst.dx, (inst.n_z - 1) * inst.dz
-    sr, sz = lx / 8.0, lz / 8.0
+    sr, sz = lx * inst.sr_frac, lz * inst.sz_frac
+    sgb = inst.emittance / sr if inst.emittance > 0.0 else 2e-3
     out = np.empty((n, 7))
-    electrons = 1000.0 * (math.sqrt(2.0 * math.pi) * sz * 1e-6) / 2.99792458e8 / 1.602e-19
+    electrons = inst.current * (math.sqrt(2.0 * math.pi) * sz * 1e-6) / 2.99792458e8 / 1.602e-19
     out[:, 0] = electrons / n
-    out[:, 1] = rng.normal(0.0, sr, n)
-    out[:, 2] = rng.normal(0.0, sr, n)
-    out[:, 3] = rng.normal(0.0, sz, n)
-    out[:, 4] = rng.normal(0.0, 2e-3, n)
-    out[:, 5] = rng.normal(0.0, 2e-3, n)
-    out[:, 6] = math.sqrt(inst.gamma ** 2 - 1.0) * (1.0 + inst.spread * rng.normal(0.0, 1.0, n))
+    if inst.quiet:  # n/4 base particles, each copied at z + j lambda_r / 4 (the reference's beamlets)
+        nb = n // 4
+        base = np.column_stack([rng.normal(0.0, sr, nb), rng.normal(0.0, sr, nb), rng.normal(0.0, sz, nb),
+                                rng.normal(0.0, sgb, nb), rng.normal(0.0, sgb, nb),
+                                math.sqrt(inst.gamma ** 2 - 1.0) * (1.0 + inst.spread * rng.normal(0.0, 1.0, nb))])
+        cols = np.repeat(base, 4, axis=0)
+        cols[:, 2] -= np.tile(np.arange(4) * inst.lambda_r / 4.0, nb)
+        out = out[:4 * nb]
+        out[:, 0] = electrons / (4 * nb)
+        out[:, 1:] = cols
+        n = 4 * nb
+    else:
+        out[:, 1] = rng.normal(0.0, sr, n)
+        out[:, 2] = rng.normal(0.0, sr, n)
+        out[:, 3] = rng.normal(0.0, sz, n)
+        out[:, 4] = rng.normal(0.0, sgb, n)
+        out[:, 5] = rng.normal(0.0, sgb, n)
+        out[:, 6] = math.sqrt(inst.gamma ** 2 - 1.0) * (1.0 + inst.spread * rng.normal(0.0, 1.0, n))
+    if inst.bunching > 0.0:  # displacement lambda_r b / pi sin(k z + phase) gives bunching b
+        k = 2.0 * math.pi / inst.lambda_r
+        out[:, 3] -= inst.lambda_r / math.pi * inst.bunching * np.sin(k * out[:, 3] + inst.phase)
     return out, electrons
 
 
 def job_text(inst, idx, workdir, electrons):
     # the reference makes (int)(L / d) + 2 nodes of a mesh length L (src/solver.cpp:610-612)
     lx, lz = (inst.n_perp - 1.5) * inst.dx, (inst.n_z - 1.5) * inst.dz
-    return JOB.format(lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=inst.n_particles,
+    n = inst.n_particles // 4 * 4 if inst.quiet else inst.n_particles
+    return JOB.format(lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=n,
                       gamma=inst.gamma, K=inst.K, lu=inst.lambda_u, dir=workdir, idx=idx, zp=0.375 * lz)
 
 

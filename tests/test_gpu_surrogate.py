@@ -92,3 +92,18 @@ def test_training_reduces_loss():
         last = mlp.train_step(x, t, 1e-3)
     print("loss", first, "->", last, flush=True)
     assert np.all(last < 0.2 * first)
+
+
+def test_surrogate_workflow_small():
+    """The whole exercise at a small size: batched solves produce the data, the ensemble trains
+    on the GPU, predicts, the spread gate decides, and every error is measured against FP64
+    re-solves of the held-out points."""
+    import tempfile
+    from mithra_amd import surrogate
+    with tempfile.TemporaryDirectory() as w:
+        out = surrogate.run_workflow(w, n_train=256, n_hold=32, epochs=20, n_single=4)
+    print(out, flush=True)
+    assert out["fp32_bulk_p_err_vs_fp64"]["max"] < 1e-4
+    assert out["final_loss"] < 1.0
+    assert 0.0 <= out["points_per_hour"]["accepted_at_limit"] <= 1.0
+    assert out["points_per_hour"]["batched"] > out["points_per_hour"]["single_instance_gpu"]
