@@ -235,13 +235,15 @@ int mithra_dev_set_reorder_interval(mithra_dev* dev, int interval);
 
 /* Tile shapes of the two marched field kernels (interior update, E/B evaluation): variant v
  * has `rows` x-rows of 64 y-nodes per workgroup and marches `planes` z-planes; variants 7-9
- * stage the interior update's A^n planes in LDS.  Every shape
- * computes identical values; only the speed differs. */
+ * stage the interior update's A^n planes in LDS; when no TF/SF seed is injected they also
+ * update the Mur y faces, which the boundary pass then skips.  Every shape computes identical
+ * values; only the speed differs. */
 #define MITHRA_TILE_VARIANTS 10
 int mithra_tile_shape(int variant, int* rows, int* planes);
 int mithra_dev_set_tiles(mithra_dev* dev, int interior, int eval);
-/* Autotuner: times `reps` launches of every tile shape of both kernels on this domain's mesh
- * and keeps the fastest of each.  The domain's state is left as it was (the interior update
+/* Autotuner: times `reps` launches of every tile shape of both kernels (the interior update
+ * together with the Mur boundaries, the E/B evaluation alone) on this domain's mesh and keeps
+ * the fastest of each.  The domain's state is left as it was (the interior update
  * runs on a saved copy of the J it consumes).  chosen[2] (optional) receives the two variant
  * indices, ms[2 * MITHRA_TILE_VARIANTS] (optional) the mean time of each shape.  No
  * counterpart in the reference (a CPU code with no launch shapes). */
@@ -271,7 +273,8 @@ int mithra_batch_instance(mithra_batch* batch, int i, double time, double time_b
 /* Tile shape of the field kernels (a variant of mithra_tile_shape) and whether steps replay as
  * a captured graph (1) or are launched one by one (0); before the first run. */
 int mithra_batch_set_tiles(mithra_batch* batch, int variant, int use_graph);
-/* Times the field kernels of the whole batch for each tile shape and keeps the fastest (the
+/* Times the field kernels (interior, Mur boundaries, E/B) of the whole batch for each tile
+ * shape and keeps the fastest (the
  * domains' state is left as it was); before the first run.  ms[MITHRA_TILE_VARIANTS]
  * (optional): per shape. */
 int mithra_batch_autotune(mithra_batch* batch, int reps, int* chosen, float* ms);

@@ -147,8 +147,7 @@ int build_tables(mithra_batch* b) {
     });
   table(b->t_eval, [&](int i, const BatchView& v) { add(i, (long)v.tilesJ * v.tilesI * v.chunks); });
   table(b->t_faces, [&](int i, const BatchView& v) {
-    const long nf = 2L * ((long)(v.N1 - 2) * (v.N2 - 2) + (long)(v.N0 - 2) * (v.N2 - 2) + (long)(v.N0 - 2) * (v.N1 - 2));
-    add(i, v.ncomp * ((nf + 255) / 256));
+    add(i, v.ncomp * ((mur_face_count(v.N0, v.N1, v.N2, v.yfaces) + 255) / 256));
   });
   table(b->t_edges, [&](int i, const BatchView& v) {
     const long ne = 4L * ((v.N2 - 2) + (v.N0 - 2) + (v.N1 - 2));
@@ -239,6 +238,7 @@ void set_tile_geometry(BatchView& v, int variant) {
   v.tilesJ = (v.N1 + 63) / 64;
   v.tilesI = (v.N0 - 2 + TI - 1) / TI;
   v.chunks = (v.N2 - 2 + K - 1) / K;
+  v.yfaces = yface_mask(variant, v.N1);  // no TF/SF in a batch: the LDS forms take the y faces
 }
 
 int prepare(mithra_batch* b) {
@@ -528,6 +528,7 @@ int mithra_batch_autotune(mithra_batch* b, int reps, int* chosen, float* ms) {
       if (!rc) rc = status(hipEventRecord(e0, b->stream));
       if (!rc) rc = status(launch_batch_interior(b->V, b->t_int[1], v, b->precision, 1, b->stream));
       if (!rc) rc = status(launch_batch_interior(b->V, b->t_int[0], v, b->precision, 0, b->stream));
+      if (!rc) rc = status(launch_batch_mur(b->V, b->t_faces, b->t_edges, b->t_corners, b->precision, b->stream));
       if (!rc) rc = status(launch_batch_eval(b->V, b->t_eval, v, b->precision, b->stream));
       if (!rc) rc = status(hipEventRecord(e1, b->stream));
       if (!rc) rc = status(hipEventSynchronize(e1));

@@ -47,6 +47,32 @@ struct MurCoeffs {
   double bB[5], cB[5], dB[5], eE[5], fE[5], gE[5], hC[17];
 };
 
+// Mur y faces (j = 0 and j = N1-1) that the LDS-staged interior kernels (tile variants 7..9)
+// update together with the interior when no TF/SF write comes between the two: bit 0 the j = 0
+// face, bit 1 the j = N1-1 face, the latter only when node N1-2 shares its 64-node tile row.
+__host__ __device__ inline int yface_mask(int variant, int N1) {
+  if (variant < 7 || N1 < 3) return 0;
+  return 1 | ((N1 - 1) % 64 != 0 ? 2 : 0);
+}
+
+// Face nodes the Mur face kernel updates: both x faces, the y faces not in `ymask`, both z faces.
+__host__ __device__ inline long mur_face_count(int N0, int N1, int N2, int ymask) {
+  const long ny = (long)(N0 - 2) * (N2 - 2);
+  return 2L * (long)(N1 - 2) * (N2 - 2) + ((ymask & 1) ? 0 : ny) + ((ymask & 2) ? 0 : ny) +
+         2L * (long)(N0 - 2) * (N1 - 2);
+}
+
+// Component c of a FieldSet by selects: indexing the pointer array with a runtime value would
+// place the struct in scratch memory.
+__host__ __device__ inline void* component(const FieldSet& f, int c) {
+  return c == 0 ? f.c[0] : c == 1 ? f.c[1] : c == 2 ? f.c[2] : f.c[3];
+}
+
+struct MurYFace {  // coefficients of the y faces (MurCoeffs::cB) and the faces to update
+  double B[5];
+  int mask;
+};
+
 struct EvalConst {  // -dt, twice the steps, and their correctly rounded reciprocals (div_rn)
   double mdt, dx2, dy2, dz2, rmdt, rdx2, rdy2, rdz2;
 };
@@ -117,6 +143,7 @@ struct Domain {
   int zrange_valid;         // zrange describes the current particle positions
   int pk_lo, pk_hi;         // power planes' cells (pk_lo > pk_hi: none)
   int zero_pending;         // the A^{n+1} buffer is logically zero (current_reset), not yet written
+  int yfaces_pending;       // Mur y faces the last interior launch updated (yface_mask bits)
   int sort_interval;        // deposits between reorders of the particle arrays (0: never)
   long sort_count;          // deposits since the particles were set
   unsigned long long* counters;   // [0] transverse escapes, [1] z-boundary crossings
@@ -160,6 +187,7 @@ struct BatchView {
   FieldSet lvl[3];
   float4* eb;
   int tilesJ, tilesI, chunks;  // interior / E-B tiles of the batch's tile shape
+  int yfaces;               // Mur y faces the interior kernels update (yface_mask of the tile shape)
   double a[6], alpha, beta;
   MurCoeffs mur;
   EvalConst ev;
@@ -204,7 +232,9 @@ struct BatchView {
 inline double* particle_array(Domain* d, int which) { return d->part + (size_t)which * d->cap; }
 
 // Launch helpers implemented in field.hip, particles.hip and power.hip.
-hipError_t launch_interior(Domain* d);
+// fuse_yfaces: no TF/SF write follows, so the LDS-staged variants may update the Mur y faces
+// too; launch_boundaries then leaves them alone.
+hipError_t launch_interior(Domain* d, bool fuse_yfaces);
 hipError_t launch_tfsf(Domain* d, double time, const mithra_seed* seed);
 hipError_t launch_boundaries(Domain* d);
 hipError_t launch_eval_eb(Domain* d);

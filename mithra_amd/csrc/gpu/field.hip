@@ -61,9 +61,9 @@ __device__ inline void interior_body(FieldSet an_set, FieldSet anm1_set, FieldSe
   const int kb = 1 + kc * K;
   const int ke = min(kb + K, N2 - 1);
   if (comp >= ncomp || i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
-  const gptr<const R> an = G((const R*)an_set.c[comp]);
-  const gptr<const R> anm1 = G((const R*)anm1_set.c[comp]);
-  const gptr<R> anp1 = G((R*)anp1_set.c[comp]);
+  const gptr<const R> an = G((const R*)component(an_set, comp));
+  const gptr<const R> anm1 = G((const R*)component(anm1_set, comp));
+  const gptr<R> anp1 = G((R*)component(anp1_set, comp));
   const double as = comp < 3 ? a4 : a5;
   const long N1N0 = (long)N1 * N0;
   long m = N1N0 * kb + (long)N1 * i + j;
@@ -101,11 +101,17 @@ __global__ __launch_bounds__(kTileJ* TI) void k_interior(
 // tile plus a one-node halo ring is fetched once, coalesced, into a three-plane LDS ring, and
 // every neighbour of the stencil is read from LDS.  Same arithmetic, in the same order, as
 // k_interior.
-template <typename R, bool NSFD, int TI, int K>
+//
+// FACES: the workgroups of the first and last tile columns also update the Mur y faces in
+// `yf.mask` (mur_face below with s = +-1 along y, t along x, u along z): the lane of face node
+// j = 0 (N1-1) takes the new value and the A^{n-1} value of node j = 1 (N1-2) from the
+// neighbouring lane and the A^n values from the LDS ring, so the faces cost no extra HBM pass.
+// Valid only when nothing writes A^{n+1} between the interior update and the boundaries.
+template <typename R, bool NSFD, int TI, int K, bool FACES = false>
 __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0,
                                          int N1, int N2, int tilesJ, int tilesI, int chunks, double a0, double a1,
                                          double a2, double a3, double a4, double a5, double alpha, double beta,
-                                         int lb) {
+                                         int lb, const MurYFace& yf = MurYFace{}) {
   constexpr int W = kTileJ + 2;           // LDS row: y-halo, 64 nodes, y-halo
   constexpr int P = (TI + 2) * W;         // one plane of the tile with its halo ring
   __shared__ double L[3 * P];
@@ -122,9 +128,9 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
   const int j0 = tj * kTileJ, i0 = 1 + ti * TI;
   const int j = j0 + lane, i = i0 + w;
   const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
-  const gptr<const R> an = G((const R*)an_set.c[comp]);
-  const gptr<const R> anm1 = G((const R*)anm1_set.c[comp]);
-  const gptr<R> anp1 = G((R*)anp1_set.c[comp]);
+  const gptr<const R> an = G((const R*)component(an_set, comp));
+  const gptr<const R> anm1 = G((const R*)component(anm1_set, comp));
+  const gptr<R> anp1 = G((R*)component(anp1_set, comp));
   const double as = comp < 3 ? a4 : a5;
   const long N1N0 = (long)N1 * N0;
   // What this thread stages per plane: its own node, and (wave 0 / wave 1) the halo rows
@@ -144,6 +150,12 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
   }
   const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
   const long hcol_off = (long)N1 * hcol_i + hcol_j;
+  // y faces: tile columns that hold a face node (uniform over the workgroup), the face lanes,
+  // and the lane each reads its inward neighbour from
+  const bool face_tile = FACES && (tj == 0 || tj == (N1 - 1) / kTileJ);
+  const int fs = j == 0 ? 1 : -1;
+  const bool face = FACES && i <= N0 - 2 && (((yf.mask & 1) && j == 0) || ((yf.mask & 2) && j == N1 - 1));
+  const int nb_lane = max(lane + fs, 0);
   double r_own = 0.0, r_row = 0.0, r_col = 0.0;
   auto fetch = [&](int k) {
     const long b = N1N0 * k;
@@ -168,13 +180,13 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
     stage(sp);
     __syncthreads();
     if (k + 2 <= ke) fetch(k + 2);
+    const double* c0 = L + s0 * P + own_slot;
+    const double* cp = L + sp * P + own_slot;
+    const double* cm = L + sm * P + own_slot;
+    double r = 0.0, v1 = 0.0;
     if (active) {
-      const double v1 = (double)anm1[m];
+      v1 = (double)anm1[m];
       const double src = (double)anp1[m];
-      const double* c0 = L + s0 * P + own_slot;
-      const double* cp = L + sp * P + own_slot;
-      const double* cm = L + sm * P + own_slot;
-      double r;
       if (NSFD) {
         double tx = c0[W] + c0[-W] + beta * (cp[W] + cm[W] + cp[-W] + cm[-W]);
         double ty = c0[1] + c0[-1] + beta * (cp[1] + cm[1] + cp[-1] + cm[-1]);
@@ -184,17 +196,29 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
       }
       anp1[m] = (R)r;
     }
+    if (face_tile) {
+      // the inward neighbour's A^{n+1} as stored, and its A^{n-1}
+      const double pn = __shfl((double)(R)r, nb_lane);
+      const double mn = __shfl(v1, nb_lane);
+      if (face) {
+        const double* B = yf.B;
+        anp1[m] = (R)(B[0] * ((double)anm1[m] + pn) + B[1] * mn + B[2] * (c0[0] + c0[fs]) +
+                      B[3] * (c0[fs + W] + c0[fs - W] + c0[W] + c0[-W]) +
+                      B[4] * (cp[fs] + cm[fs] + cp[0] + cm[0]));
+      }
+    }
     __syncthreads();
   }
 }
 
-template <typename R, bool NSFD, int TI, int K>
+template <typename R, bool NSFD, int TI, int K, bool FACES>
 __global__ __launch_bounds__(kTileJ* TI) void k_interior_lds(
     FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
     int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
-    double alpha, double beta) {
-  interior_lds_body<R, NSFD, TI, K>(an_set, anm1_set, anp1_set, ncomp, N0, N1, N2, tilesJ, tilesI, chunks, a0, a1,
-                                    a2, a3, a4, a5, alpha, beta, xcd_logical_block(blockIdx.x, gridDim.x));
+    double alpha, double beta, MurYFace yf) {
+  interior_lds_body<R, NSFD, TI, K, FACES>(an_set, anm1_set, anp1_set, ncomp, N0, N1, N2, tilesJ, tilesI, chunks,
+                                           a0, a1, a2, a3, a4, a5, alpha, beta,
+                                           xcd_logical_block(blockIdx.x, gridDim.x), yf);
 }
 
 // ---- analytic seed (Signal::self src/classes.cpp:534-575, Seed::fields :740-855) --------
@@ -367,45 +391,53 @@ __device__ inline double mur_corner(const double* h, TP P, TN N, TN M, long m, l
 
 
 // All six faces in one launch: x faces (j, k), y faces (i, k), z faces (i, j), each min/max.
+// The y faces in `ymask` (yface_mask) were updated by the interior kernel and are skipped.
 template <typename R>
 __device__ inline void mur_faces_body(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2,
-                                      const MurCoeffs& c, long t, int comp) {
+                                      const MurCoeffs& c, long t, int comp, int ymask) {
   const long N1N0 = (long)N1 * N0;
   const long X = N1, Y = 1, Z = N1N0;
   const long nx = (long)(N1 - 2) * (N2 - 2), ny = (long)(N0 - 2) * (N2 - 2), nz = (long)(N0 - 2) * (N1 - 2);
+  const long nylo = (ymask & 1) ? 0 : ny, nyhi = (ymask & 2) ? 0 : ny;
   if (comp >= ncomp) return;
-  const gptr<const R> pn = G((const R*)N.c[comp]);
-  const gptr<const R> pm = G((const R*)M.c[comp]);
-  const gptr<R> pp = G((R*)P.c[comp]);
+  const gptr<const R> pn = G((const R*)component(N, comp));
+  const gptr<const R> pm = G((const R*)component(M, comp));
+  const gptr<R> pp = G((R*)component(P, comp));
   long m, s, tt, u;
-  const double* B;
+  int axis;
   if (t < 2 * nx) {
     const bool hi = t >= nx;
     t -= hi ? nx : 0;
     const long j = 1 + t % (N1 - 2), k = 1 + t / (N1 - 2);
     m = N1N0 * k + (hi ? N1N0 - N1 : 0) + j;
-    s = hi ? -X : X; tt = Y; u = Z; B = c.bB;
-  } else if ((t -= 2 * nx) < 2 * ny) {
-    const bool hi = t >= ny;
-    t -= hi ? ny : 0;
+    s = hi ? -X : X; tt = Y; u = Z; axis = 0;
+  } else if ((t -= 2 * nx) < nylo + nyhi) {
+    const bool hi = t >= nylo;
+    t -= hi ? nylo : 0;
     const long i = 1 + t % (N0 - 2), k = 1 + t / (N0 - 2);
     m = N1N0 * k + (long)N1 * i + (hi ? N1 - 1 : 0);
-    s = hi ? -Y : Y; tt = X; u = Z; B = c.cB;
-  } else if ((t -= 2 * ny) < 2 * nz) {
+    s = hi ? -Y : Y; tt = X; u = Z; axis = 1;
+  } else if ((t -= nylo + nyhi) < 2 * nz) {
     const bool hi = t >= nz;
     t -= hi ? nz : 0;
     const long j = 1 + t % (N1 - 2), i = 1 + t / (N1 - 2);
     m = (hi ? N1N0 * (N2 - 1) : 0) + (long)N1 * i + j;
-    s = hi ? -Z : Z; tt = X; u = Y; B = c.dB;
+    s = hi ? -Z : Z; tt = X; u = Y; axis = 2;
   } else {
     return;
   }
+  // The coefficients by selects into registers: a pointer into the coefficient struct would
+  // copy the whole struct to scratch memory in every thread.
+  double B[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) B[q] = axis == 0 ? c.bB[q] : axis == 1 ? c.cB[q] : c.dB[q];
   pp[m] = (R)mur_face(B, pp, pn, pm, m, s, tt, u);
 }
 
 template <typename R>
-__global__ void k_mur_faces(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c) {
-  mur_faces_body<R>(P, N, M, ncomp, N0, N1, N2, c, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+__global__ void k_mur_faces(FieldSet P, FieldSet N, FieldSet M, int ncomp, int N0, int N1, int N2, MurCoeffs c,
+                           int ymask) {
+  mur_faces_body<R>(P, N, M, ncomp, N0, N1, N2, c, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, ymask);
 }
 
 // The twelve edges (second-order Mur only).
@@ -416,32 +448,35 @@ __device__ inline void mur_edges_body(FieldSet P, FieldSet N, FieldSet M, int nc
   const long X = N1, Y = 1, Z = N1N0;
   const long ez = N2 - 2, ex = N0 - 2, ey = N1 - 2;
   if (comp >= ncomp) return;
-  const gptr<const R> pn = G((const R*)N.c[comp]);
-  const gptr<const R> pm = G((const R*)M.c[comp]);
-  const gptr<R> pp = G((R*)P.c[comp]);
+  const gptr<const R> pn = G((const R*)component(N, comp));
+  const gptr<const R> pm = G((const R*)component(M, comp));
+  const gptr<R> pp = G((R*)component(P, comp));
   long m, o1, o2, w;
-  const double* B;
+  int axis;
   if (t < 4 * ez) {  // along z at the four (x, y) corners
     const int which = t / ez;
     const long k = 1 + t % ez;
     const bool xh = which & 1, yh = which & 2;
     m = N1N0 * k + (xh ? (long)N1 * (N0 - 1) : 0) + (yh ? N1 - 1 : 0);
-    o1 = xh ? -X : X; o2 = yh ? -Y : Y; w = Z; B = c.eE;
+    o1 = xh ? -X : X; o2 = yh ? -Y : Y; w = Z; axis = 0;
   } else if ((t -= 4 * ez) < 4 * ex) {  // along x at the four (y, z) corners
     const int which = t / ex;
     const long i = 1 + t % ex;
     const bool yh = which & 1, zh = which & 2;
     m = (long)N1 * i + (yh ? N1 - 1 : 0) + (zh ? N1N0 * (N2 - 1) : 0);
-    o1 = yh ? -Y : Y; o2 = zh ? -Z : Z; w = X; B = c.fE;
+    o1 = yh ? -Y : Y; o2 = zh ? -Z : Z; w = X; axis = 1;
   } else if ((t -= 4 * ex) < 4 * ey) {  // along y at the four (z, x) corners
     const int which = t / ey;
     const long j = 1 + t % ey;
     const bool zh = which & 1, xh = which & 2;
     m = j + (zh ? N1N0 * (N2 - 1) : 0) + (xh ? N1N0 - N1 : 0);
-    o1 = zh ? -Z : Z; o2 = xh ? -X : X; w = Y; B = c.gE;
+    o1 = zh ? -Z : Z; o2 = xh ? -X : X; w = Y; axis = 2;
   } else {
     return;
   }
+  double B[5];  // by selects, as in mur_faces_body
+#pragma unroll
+  for (int q = 0; q < 5; ++q) B[q] = axis == 0 ? c.eE[q] : axis == 1 ? c.fE[q] : c.gE[q];
   pp[m] = (R)mur_edge(B, pp, pn, pm, m, o1, o2, w);
 }
 
@@ -457,9 +492,9 @@ __device__ inline void mur_corners_body(FieldSet P, FieldSet N, FieldSet M, int 
   const long N1N0 = (long)N1 * N0;
   const bool xh = t & 1, yh = t & 2, zh = t & 4;
   const long m = (xh ? N1N0 - N1 : 0) + (yh ? N1 - 1 : 0) + (zh ? N1N0 * (N2 - 1) : 0);
-  const gptr<const R> pn = G((const R*)N.c[comp]);
-  const gptr<const R> pm = G((const R*)M.c[comp]);
-  const gptr<R> pp = G((R*)P.c[comp]);
+  const gptr<const R> pn = G((const R*)component(N, comp));
+  const gptr<const R> pm = G((const R*)component(M, comp));
+  const gptr<R> pp = G((R*)component(P, comp));
   pp[m] = (R)mur_corner(c.hC, pp, pn, pm, m, xh ? -(long)N1 : N1, yh ? -1L : 1L, zh ? -N1N0 : N1N0);
 }
 
@@ -727,11 +762,14 @@ __global__ __launch_bounds__(kTileJ* TI) void kb_interior(const BatchView* __res
     const int2 e = T.t[t];
     const BatchView& v = V[e.x];
     if (v.active) {
-      if constexpr (LDS)
-        interior_lds_body<R, NSFD, TI, K>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1, v.N2,
-                                          v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3], v.a[4],
-                                          v.a[5], v.alpha, v.beta, e.y);
-      else
+      if constexpr (LDS) {
+        MurYFace yf;  // a local copy: stores to the potentials cannot alias it
+        for (int q = 0; q < 5; ++q) yf.B[q] = v.mur.cB[q];
+        yf.mask = v.yfaces;
+        interior_lds_body<R, NSFD, TI, K, true>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1,
+                                                v.N2, v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3],
+                                                v.a[4], v.a[5], v.alpha, v.beta, e.y, yf);
+      } else
         interior_body<R, NSFD, TI, K>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1, v.N2,
                                       v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3], v.a[4], v.a[5],
                                       v.alpha, v.beta, e.y);
@@ -771,9 +809,10 @@ __global__ __launch_bounds__(256) void kb_mur(const BatchView* __restrict__ V, B
     if (!v.active) continue;
     const FieldSet P = v.lvl[v.i_np1], N = v.lvl[v.i_n], M = v.lvl[v.i_nm1];
     if (WHICH == 0) {
-      const long nf = 2L * ((long)(v.N1 - 2) * (v.N2 - 2) + (long)(v.N0 - 2) * (v.N2 - 2) + (long)(v.N0 - 2) * (v.N1 - 2));
+      const long nf = mur_face_count(v.N0, v.N1, v.N2, v.yfaces);
       const int nb = (int)((nf + 255) / 256);
-      mur_faces_body<R>(P, N, M, v.ncomp, v.N0, v.N1, v.N2, v.mur, (long)(e.y % nb) * 256 + threadIdx.x, e.y / nb);
+      mur_faces_body<R>(P, N, M, v.ncomp, v.N0, v.N1, v.N2, v.mur, (long)(e.y % nb) * 256 + threadIdx.x, e.y / nb,
+                        v.yfaces);
     } else if (WHICH == 1) {
       const long ne = 4L * ((v.N2 - 2) + (v.N0 - 2) + (v.N1 - 2));
       const int nb = (int)((ne + 255) / 256);
@@ -798,18 +837,26 @@ __global__ void k_fill_zero_bytes(unsigned char* p, size_t n) {
 // nodes per row) and z planes per workgroup.  Every shape computes the same values; the
 // autotuner (capi) picks the fastest per domain.
 template <typename R, bool NSFD, int TI, int K, bool LDS = false>
-hipError_t interior_launch(Domain* d) {
+hipError_t interior_launch(Domain* d, int ymask) {
   const mithra_grid& g = d->g;
   const int tilesJ = (g.N1 + kTileJ - 1) / kTileJ;
   const int tilesI = (g.N0 - 2 + TI - 1) / TI;
   const int chunks = (g.N2 - 2 + K - 1) / K;
   const long nblk = (long)tilesJ * tilesI * chunks * d->ncomp;
   const double* a = d->co.a;
-  if constexpr (LDS)
-    hipLaunchKernelGGL((k_interior_lds<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
-                       d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
-                       chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
-  else
+  MurYFace yf;
+  for (int q = 0; q < 5; ++q) yf.B[q] = d->co.cB[q];
+  yf.mask = ymask;
+  if constexpr (LDS) {
+    if (ymask)
+      hipLaunchKernelGGL((k_interior_lds<R, NSFD, TI, K, true>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0,
+                         d->stream, d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2,
+                         tilesJ, tilesI, chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta, yf);
+    else
+      hipLaunchKernelGGL((k_interior_lds<R, NSFD, TI, K, false>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0,
+                         d->stream, d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2,
+                         tilesJ, tilesI, chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta, yf);
+  } else
     hipLaunchKernelGGL((k_interior<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
                        d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
                        chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
@@ -837,11 +884,11 @@ hipError_t eval_launch(Domain* d, int klo, int khi, const int* zr, int pb0, int 
 
 template <typename R, bool FLAG>
 hipError_t field_variant(Domain* d, int kernel, int v, int klo, int khi, const int* zr = nullptr, int pb0 = 0,
-                         int pb1 = -1) {
+                         int pb1 = -1, int ymask = 0) {
   // kernel 0: interior (FLAG = NSFD), kernel 1: E/B evaluation (FLAG = space charge)
 #define MITHRA_VARIANT(idx, TI, K)                                                            \
   case idx:                                                                                    \
-    return kernel == 0 ? interior_launch<R, FLAG, TI, K>(d) : eval_launch<R, FLAG, TI, K>(d, klo, khi, zr, pb0, pb1);
+    return kernel == 0 ? interior_launch<R, FLAG, TI, K>(d, 0) : eval_launch<R, FLAG, TI, K>(d, klo, khi, zr, pb0, pb1);
   switch (v) {
     MITHRA_VARIANT(0, 4, 64)
     MITHRA_VARIANT(1, 2, 64)
@@ -854,18 +901,22 @@ hipError_t field_variant(Domain* d, int kernel, int v, int klo, int khi, const i
   static_assert(kTileVariants == 10, "variant table and switch disagree");
   // variants 7..9: the LDS-staged (2.5-D blocked) forms of both kernels
   switch (v) {
-    case 7: return kernel == 0 ? interior_launch<R, FLAG, 4, 64, true>(d) : eval_launch<R, FLAG, 4, 64, true>(d, klo, khi, zr, pb0, pb1);
-    case 8: return kernel == 0 ? interior_launch<R, FLAG, 8, 64, true>(d) : eval_launch<R, FLAG, 8, 64, true>(d, klo, khi, zr, pb0, pb1);
-    case 9: return kernel == 0 ? interior_launch<R, FLAG, 16, 64, true>(d) : eval_launch<R, FLAG, 16, 64, true>(d, klo, khi, zr, pb0, pb1);
+    case 7: return kernel == 0 ? interior_launch<R, FLAG, 4, 64, true>(d, ymask) : eval_launch<R, FLAG, 4, 64, true>(d, klo, khi, zr, pb0, pb1);
+    case 8: return kernel == 0 ? interior_launch<R, FLAG, 8, 64, true>(d, ymask) : eval_launch<R, FLAG, 8, 64, true>(d, klo, khi, zr, pb0, pb1);
+    case 9: return kernel == 0 ? interior_launch<R, FLAG, 16, 64, true>(d, ymask) : eval_launch<R, FLAG, 16, 64, true>(d, klo, khi, zr, pb0, pb1);
   }
 #undef MITHRA_VARIANT
   return hipErrorInvalidValue;
 }
 
 template <typename R>
-hipError_t interior_impl(Domain* d) {
-  return d->g.solver == MITHRA_SOLVER_NSFD ? field_variant<R, true>(d, 0, d->tile_interior, 0, 0)
-                                           : field_variant<R, false>(d, 0, d->tile_interior, 0, 0);
+hipError_t interior_impl(Domain* d, bool fuse_yfaces) {
+  const int ymask = fuse_yfaces ? yface_mask(d->tile_interior, d->g.N1) : 0;
+  const hipError_t e = d->g.solver == MITHRA_SOLVER_NSFD
+                           ? field_variant<R, true>(d, 0, d->tile_interior, 0, 0, nullptr, 0, -1, ymask)
+                           : field_variant<R, false>(d, 0, d->tile_interior, 0, 0, nullptr, 0, -1, ymask);
+  d->yfaces_pending = e == hipSuccess ? ymask : 0;
+  return e;
 }
 
 template <typename R>
@@ -893,9 +944,11 @@ hipError_t boundaries_impl(Domain* d) {
   }
   for (int t = 0; t < 17; ++t) c.hC[t] = d->co.hC[t];
   FieldSet P = d->lvl[d->i_np1], N = d->lvl[d->i_n], M = d->lvl[d->i_nm1];
-  const long nf = 2L * ((long)(g.N1 - 2) * (g.N2 - 2) + (long)(g.N0 - 2) * (g.N2 - 2) + (long)(g.N0 - 2) * (g.N1 - 2));
+  const int ymask = d->yfaces_pending;  // y faces the interior launch already updated
+  d->yfaces_pending = 0;
+  const long nf = mur_face_count(g.N0, g.N1, g.N2, ymask);
   hipLaunchKernelGGL((k_mur_faces<R>), dim3((unsigned)((nf + 255) / 256), d->ncomp), dim3(256), 0, d->stream, P, N, M,
-                     d->ncomp, g.N0, g.N1, g.N2, c);
+                     d->ncomp, g.N0, g.N1, g.N2, c, ymask);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || g.truncation_order != 2) return e;
   const long ne = 4L * ((g.N2 - 2) + (g.N0 - 2) + (g.N1 - 2));
@@ -921,8 +974,9 @@ hipError_t eval_impl(Domain* d, int klo, int khi, const int* zr = nullptr, int p
 extern const int kTileShapes[kTileVariants][2] = {{4, 64}, {2, 64}, {4, 32}, {4, 128}, {8, 32},
                                                   {8, 64}, {16, 32}, {4, 64}, {8, 64}, {16, 64}};
 
-hipError_t launch_interior(Domain* d) {
-  return d->precision == MITHRA_PRECISION_FP64 ? interior_impl<double>(d) : interior_impl<float>(d);
+hipError_t launch_interior(Domain* d, bool fuse_yfaces) {
+  return d->precision == MITHRA_PRECISION_FP64 ? interior_impl<double>(d, fuse_yfaces)
+                                               : interior_impl<float>(d, fuse_yfaces);
 }
 
 hipError_t launch_tfsf(Domain* d, double time, const mithra_seed* seed) {

@@ -505,31 +505,33 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   const long base = N1N0 * k + (long)N1 * i0;
   for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
     const int comp = t / own, o = t - comp * own;
-    tile[comp * tsz + o] = accumulate ? (double)G((const R*)J.c[comp])[base + o] : 0.0;
+    tile[comp * tsz + o] = accumulate ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
   }
-  // The ranges: (half 0 | 1) x (plane k-1 | k), rows lo..hi of each.
+  // The ranges: q = 2 half + (kk - k + 1) for half 0 | 1 and plane kk = k-1 | k, rows lo..hi
+  // of each (empty where the plane or the rows do not exist).  Every index below is a
+  // constant after unrolling, so the bounds stay in registers (a dynamically indexed array
+  // would live in scratch memory and cost a memory round trip per item).
   const int lo = max(i0 - 1, 0), hi = min(i0 + ti - 1, c.N0 - 2);
   unsigned start[4], len[4];
-  int qhalf[4], qk[4], nq = 0;
-  unsigned total = 0;
-  if (lo <= hi)
-    for (int half = 0; half < 2; ++half)
-      for (int kk = max(k - 1, 0); kk <= min(k, c.N2 - 2); ++kk) {
-        const gptr<const unsigned> row = G(half ? m_row : p_row);
-        start[nq] = row[(long)kk * c.N0 + lo];
-        len[nq] = row[(long)kk * c.N0 + hi + 1] - start[nq];
-        qhalf[nq] = half;
-        qk[nq] = kk;
-        total += len[nq];
-        ++nq;
-      }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int kk = k - 1 + (q & 1);
+    start[q] = len[q] = 0;
+    if (lo <= hi && kk >= 0 && kk <= c.N2 - 2) {
+      const gptr<const unsigned> row = G((q >> 1) ? m_row : p_row);
+      start[q] = row[(long)kk * c.N0 + lo];
+      len[q] = row[(long)kk * c.N0 + hi + 1] - start[q];
+    }
+  }
+  const unsigned e0 = len[0], e1 = e0 + len[1], e2 = e1 + len[2], total = e2 + len[3];
   __syncthreads();
   auto locate = [&](unsigned t, HalfItem& it) {
-    int q = 0;
-    while (q < nq - 1 && t >= len[q]) t -= len[q++];
-    it.half = qhalf[q];
-    it.kk = qk[q];
-    return (long)G(qhalf[q] ? m_idx : p_idx)[start[q] + t];
+    const int q = (t >= e0) + (t >= e1) + (t >= e2);
+    const unsigned off = t - (q == 0 ? 0u : q == 1 ? e0 : q == 2 ? e1 : e2);
+    const unsigned st = q == 0 ? start[0] : q == 1 ? start[1] : q == 2 ? start[2] : start[3];
+    it.half = q >> 1;
+    it.kk = k - 1 + (q & 1);
+    return (long)G((q >> 1) ? m_idx : p_idx)[st + off];
   };
   for (unsigned t = threadIdx.x; t < total; t += 2 * blockDim.x) {
     HalfItem a, b;
@@ -545,7 +547,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   __syncthreads();
   for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
     const int comp = t / own, o = t - comp * own;
-    G((R*)J.c[comp])[base + o] = (R)tile[comp * tsz + o];
+    G((R*)component(J, comp))[base + o] = (R)tile[comp * tsz + o];
   }
 }
 

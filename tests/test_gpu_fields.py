@@ -51,3 +51,36 @@ def test_field_evaluate_bit_exact(sc):
     for v, (e, b) in enumerate(out[1:]):
         assert np.array_equal(e, e_o), f"E, tile variant {v}"
         assert np.array_equal(b, b_o), f"B, tile variant {v}"
+
+
+@pytest.mark.parametrize("ny", [65, 70, 130])
+def test_field_update_bit_exact_every_shape(ny):
+    """Interior update, Mur faces/edges/corners and E/B of one unseeded field update against the
+    oracle, bit for bit, for every tile shape (FdTd::fieldUpdate, src/fdtd.cpp:270-303 and
+    377-725).  Without a TF/SF seed the LDS-staged shapes update the Mur y faces inside the
+    interior kernel; N1 = 65 puts node N1-1 in a 64-node tile column of its own, so that face
+    stays with the boundary pass, and 130 spreads the mesh over three tile columns."""
+    g = grid(20, ny, 40, 0)
+    rng = np.random.default_rng(ny)
+    c = abi.Coeffs()
+    for name, n in (("a", 6), ("bB", 5), ("cB", 5), ("dB", 5), ("eE", 5), ("fE", 5), ("gE", 5), ("hC", 17)):
+        getattr(c, name)[:] = rng.uniform(-1.0, 1.0, n)
+    c.hC[0] = 2.5  # the corner update divides by hC[0]
+    c.alpha, c.beta = 1.1, 0.07
+    n = g.nodes
+    levels = [rng.standard_normal(3 * n) * 1e-6 for _ in range(3)]
+    doms = [abi.Domain(g, c, lib=load_oracle(), prefix="oracle_")]
+    for v in range(abi.TILE_VARIANTS):
+        d = abi.Domain(g, c)
+        d.set_tiles(v, v)
+        doms.append(d)
+    out = []
+    for d in doms:
+        for lvl, a in enumerate(levels):
+            d.set_potentials(lvl, a)
+        d.field_update(0.0, None)
+        out.append((d.get_potentials(2)[0], *d.get_fields()))
+    ref = out[0]
+    for v, got in enumerate(out[1:]):
+        for name, a, b in zip(("A", "E", "B"), got, ref):
+            assert np.array_equal(a, b), f"{name}, tile variant {v} {abi.tile_shape(v)}"
