@@ -159,7 +159,7 @@ int build_tables(mithra_batch* b) {
   table(b->t_part, [&](int i, const BatchView& v) { add(i, (v.np + 255) / 256); });
   table(b->t_rows, [&](int i, const BatchView& v) { add(i, ((long)v.rows + 1 + 1023) / 1024); });
   table(b->t_inst, [&](int i, const BatchView&) { add(i, 1); });
-  table(b->t_dep, [&](int i, const BatchView& v) { add(i, (long)v.N2 * v.dtilesI); });
+  table(b->t_dep, [&](int i, const BatchView& v) { add(i, deposit_blocks(v.N2, v.dtilesI, deposit_chunk())); });
   table(b->t_pnodes, [&](int i, const BatchView& v) {
     if (v.pNz > 0 && v.N0 > 4 && v.N1 > 4) add(i, (long)v.pNz * v.pblocks);
   });
@@ -247,7 +247,7 @@ int prepare(mithra_batch* b) {
     if (!b->ready[i]) return MITHRA_E_STATE;
     set_tile_geometry(b->hv[i], b->variant);
     const BatchView& v = b->hv[i];
-    b->dep_lds = std::max(b->dep_lds, sizeof(double) * v.ncomp * v.dTI * v.N1);
+    b->dep_lds = std::max(b->dep_lds, deposit_lds_bytes(v.ncomp, v.dTI, v.N1));
   }
   if (int rc = upload_views(b)) return rc;
   if (int rc = build_tables(b)) return rc;

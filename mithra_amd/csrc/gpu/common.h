@@ -62,6 +62,17 @@ __host__ __device__ inline long mur_face_count(int N0, int N1, int N2, int ymask
          2L * (long)(N0 - 2) * (N1 - 2);
 }
 
+// Streaming accesses: values read or written once per launch, marked nontemporal so they do
+// not displace reused lines (the stencil's A^n planes) from the caches.
+template <typename T>
+__device__ inline T ld_stream(gptr<const T> p, long i) {
+  return __builtin_nontemporal_load(p + i);
+}
+template <typename T>
+__device__ inline void st_stream(gptr<T> p, long i, T v) {
+  __builtin_nontemporal_store(v, p + i);
+}
+
 // Component c of a FieldSet by selects: indexing the pointer array with a runtime value would
 // place the struct in scratch memory.
 __host__ __device__ inline void* component(const FieldSet& f, int c) {
@@ -280,6 +291,9 @@ hipError_t launch_batch_power(BatchView* V, BatchTiles nodes, BatchTiles reduce,
 int batch_grid(int ntiles, int per_cu);  // persistent grid size for a table of ntiles entries
 
 int deposit_tile_rows(const Domain* d);      // rows per tile of the row-tiled deposit (0: atomic path)
+long deposit_blocks(int N2, int tilesI, int KC);  // workgroups of the row-tiled deposit (row tiles x z chunks)
+int deposit_chunk();                         // z planes per chunk of the batched deposit
+size_t deposit_lds_bytes(int ncomp, int TI, int N1);  // its LDS per workgroup
 DepositConst make_deposit_const(const Domain* d);
 
 }  // namespace mithra

@@ -71,8 +71,8 @@ __device__ inline void interior_body(FieldSet an_set, FieldSet anm1_set, FieldSe
   Plane5 p0 = load_plane(an, m, N1);
   for (int k = kb; k < ke; ++k, m += N1N0) {
     Plane5 pp = load_plane(an, m + N1N0, N1);
-    double v1 = (double)anm1[m];
-    double src = (double)anp1[m];
+    double v1 = (double)ld_stream(anm1, m);
+    double src = (double)ld_stream(G((const R*)component(anp1_set, comp)), m);
     double r;
     if (NSFD) {
       double tx = p0.xp + p0.xm + beta * (pp.xp + pm.xp + pp.xm + pm.xm);
@@ -81,7 +81,7 @@ __device__ inline void interior_body(FieldSet an_set, FieldSet anm1_set, FieldSe
     } else {
       r = a0 * p0.c - v1 + a1 * (p0.xp + p0.xm) + a2 * (p0.yp + p0.ym) + a3 * (pp.c + pm.c) + as * src;
     }
-    anp1[m] = (R)r;
+    st_stream(anp1, m, (R)r);
     pm = p0;
     p0 = pp;
   }
@@ -169,24 +169,34 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
     if (hrow_ok) p[hrow_slot] = r_row;
     if (hcol_ok) p[hcol_slot] = r_col;
   };
+  // A^{n-1} and J of the next plane are loaded one plane ahead, like the A^n planes, so their
+  // latency overlaps the current plane's work.
+  const gptr<const R> jsrc = G((const R*)component(anp1_set, comp));
+  const bool need_m1 = active || face;
+  double v1_next = 0.0, src_next = 0.0;
   fetch(kb - 1);
   stage(0);
   fetch(kb);
   stage(1);
   fetch(kb + 1);
   long m = N1N0 * kb + own_off;
+  if (need_m1) v1_next = (double)anm1[m];
+  if (active) src_next = (double)jsrc[m];
   for (int k = kb; k < ke; ++k, m += N1N0) {
     const int s0 = (k - kb + 1) % 3, sp = (k - kb + 2) % 3, sm = (k - kb) % 3;
+    const double v1 = v1_next, src = src_next;
     stage(sp);
     __syncthreads();
     if (k + 2 <= ke) fetch(k + 2);
+    if (k + 1 < ke) {
+      if (need_m1) v1_next = (double)anm1[m + N1N0];
+      if (active) src_next = (double)jsrc[m + N1N0];
+    }
     const double* c0 = L + s0 * P + own_slot;
     const double* cp = L + sp * P + own_slot;
     const double* cm = L + sm * P + own_slot;
-    double r = 0.0, v1 = 0.0;
+    double r = 0.0;
     if (active) {
-      v1 = (double)anm1[m];
-      const double src = (double)anp1[m];
       if (NSFD) {
         double tx = c0[W] + c0[-W] + beta * (cp[W] + cm[W] + cp[-W] + cm[-W]);
         double ty = c0[1] + c0[-1] + beta * (cp[1] + cm[1] + cp[-1] + cm[-1]);
@@ -202,7 +212,7 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
       const double mn = __shfl(v1, nb_lane);
       if (face) {
         const double* B = yf.B;
-        anp1[m] = (R)(B[0] * ((double)anm1[m] + pn) + B[1] * mn + B[2] * (c0[0] + c0[fs]) +
+        anp1[m] = (R)(B[0] * (v1 + pn) + B[1] * mn + B[2] * (c0[0] + c0[fs]) +
                       B[3] * (c0[fs + W] + c0[fs - W] + c0[W] + c0[-W]) +
                       B[4] * (cp[fs] + cm[fs] + cp[0] + cm[0]));
       }

@@ -412,21 +412,44 @@ __global__ __launch_bounds__(256) void k_row_keys(const double* __restrict__ par
   val[p] = (unsigned)p;
 }
 
-// Adds one half's contributions to the nodes of the tile (plane k, rows i0 .. i0+ti-1, all j).
+// Adds one half's contributions to the four nodes of its cell in one node plane of a tile
+// (rows i0 .. i0+ti-1, all j): the cell's lower node plane (up = false) or its upper one
+// (up = true).  Target t = 4 up + tt, tt = 0..3 over (row, column) offsets ((tt & 1), (tt >> 1)),
+// with the value layout of half_weights / rho_weights.  The cell's row ic lies in
+// i0-1 .. i0+ti-1, so row offset 1 is always inside the tile from below and row offset 0 is out
+// only for ic = i0-1.
 template <bool RHO>
-__device__ inline void add_owned(double* tile, int TI, int N1, int k, int i0, int ti, int ic, int jc, int kc,
-                                 const double* v, int) {
+__device__ inline void add_plane(double* tile, int TI, int N1, bool up, int i0, int ti, int ic, int jc,
+                                 const double* v) {
   const int tsz = TI * N1;
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    if (kc + (t >> 2) != k) continue;
-    const int row = ic + (t & 1) - i0;
-    if (row < 0 || row >= ti) continue;
-    double* at = tile + row * N1 + jc + ((t >> 1) & 1);
-    atomicAdd(at, v[t >> 1]);
-    atomicAdd(at + tsz, v[4 + (t & 1) + 2 * (t >> 2)]);
-    atomicAdd(at + 2 * tsz, v[8 + (t & 3)]);
-    if (RHO) atomicAdd(at + 3 * tsz, v[12 + t]);
+  const int r0 = ic - i0;
+  const double x0 = up ? v[2] : v[0], x1 = up ? v[3] : v[1];
+  const double y0 = up ? v[6] : v[4], y1 = up ? v[7] : v[5];
+  if (r0 >= 0) {
+    double* at = tile + r0 * N1 + jc;
+    atomicAdd(at, x0);
+    atomicAdd(at + tsz, y0);
+    atomicAdd(at + 2 * tsz, v[8]);
+    atomicAdd(at + 1, x1);
+    atomicAdd(at + 1 + tsz, y0);
+    atomicAdd(at + 1 + 2 * tsz, v[10]);
+    if (RHO) {
+      atomicAdd(at + 3 * tsz, up ? v[16] : v[12]);
+      atomicAdd(at + 1 + 3 * tsz, up ? v[18] : v[14]);
+    }
+  }
+  if (r0 + 1 < ti) {
+    double* at = tile + (r0 + 1) * N1 + jc;
+    atomicAdd(at, x0);
+    atomicAdd(at + tsz, y1);
+    atomicAdd(at + 2 * tsz, v[9]);
+    atomicAdd(at + 1, x1);
+    atomicAdd(at + 1 + tsz, y1);
+    atomicAdd(at + 1 + 2 * tsz, v[11]);
+    if (RHO) {
+      atomicAdd(at + 3 * tsz, up ? v[17] : v[13]);
+      atomicAdd(at + 1 + 3 * tsz, up ? v[19] : v[15]);
+    }
   }
 }
 
@@ -448,9 +471,12 @@ __device__ inline void load_item(HalfItem& it, PT part, long cap, long p) {
   it.q = part[PQ * cap + p];
 }
 
+// Deposits one listed half into the two-plane tile ring of the marching deposit: its lower
+// node plane kc when kc >= kb, its upper one kc+1 when kc+1 < ke (slot = plane & 1, `psz`
+// doubles per slot).
 template <bool SC>
-__device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, double* tile, int TI, int k, int i0,
-                                    int ti, int lo, int hi) {
+__device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, double* tile, int psz, int TI, int kb,
+                                    int ke, int i0, int ti, int lo, int hi) {
   const double* rp = it.rp;
   const double* rm = it.rm;
   const int ip = (int)floor(div_rn(rp[0] - c.xmin, c.dx, c.ix)), jp = (int)floor(div_rn(rp[1] - c.ymin, c.dy, c.iy)),
@@ -465,89 +491,102 @@ __device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, d
   r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
   r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
   double v[SC ? 20 : 12];
+  const bool rho = SC && it.half == 0;
   if (it.half == 0) {
     const double jc3[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
     half_weights(c, it.q, rp, r, jc3, v);
-    if (SC) {
-      rho_weights(c, it.q, rp, v + 12);
-      add_owned<SC>(tile, TI, c.N1, k, i0, ti, ic, jc, kc, v, 0);
-      return;
-    }
+    if (SC) rho_weights(c, it.q, rp, v + 12);
   } else {
     const double jc3[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
     half_weights(c, it.q, rm, r, jc3, v);
   }
-  add_owned<false>(tile, TI, c.N1, k, i0, ti, ic, jc, kc, v, 0);
+  for (int up = 0; up < 2; ++up) {
+    const int k = kc + up;
+    if (k < kb || k >= ke) continue;
+    double* t = tile + (k & 1) * psz;
+    if (rho) add_plane<SC>(t, TI, c.N1, up, i0, ti, ic, jc, v);
+    else add_plane<false>(t, TI, c.N1, up, i0, ti, ic, jc, v);
+  }
 }
 
-// Row-tiled current deposition (same arithmetic per particle as k_deposit).  Workgroup b owns
-// the J (and rho) values of plane k and rows i0 .. i0+ti-1, all j: it adds, in LDS, the halves
-// whose cell lies in planes k-1..k and rows i0-1 .. i0+ti-1 (up to four contiguous ranges of
-// the two row orders, walked as one index space, two halves in flight per thread), then
-// writes its nodes once.  Every node of the mesh belongs to exactly one workgroup, so no
-// global atomics are needed and the J buffer needs no separate reset (`accumulate` = 0 writes
-// the deposit alone, 1 adds it to what the buffer holds).
+// Row-tiled current deposition, marching along z (same arithmetic per particle as k_deposit).
+// Workgroup b owns the J (and rho) values of rows i0 .. i0+ti-1, all j, of the node planes
+// kb .. ke-1 of its chunk of KC planes.  It walks the cell planes kc = kb-1 .. ke-1 in turn; the halves
+// listed under plane kc and rows i0-1 .. i0+ti-1 (two contiguous ranges, one per row order,
+// two halves in flight per thread) add their lower nodes to plane kc and their upper nodes to
+// plane kc+1 of a two-plane LDS ring, so each half is weighed once per workgroup.  Once plane
+// kc is complete it is written out and its slot starts plane kc+2.  Every node of the mesh
+// belongs to exactly one workgroup, so no global atomics are needed and the J buffer needs no
+// separate reset (`accumulate` = 0 writes the deposit alone, 1 adds it to what the buffer holds).
 constexpr int kDepositThreads = 512;
+constexpr int kDepositChunk = 16;  // node planes per workgroup (at most)
 
 template <typename R, bool SC>
 __device__ inline void deposit_rows_body(const double* __restrict__ part, long cap, FieldSet J, const DepositConst& c,
                                          const unsigned* __restrict__ p_row, const unsigned* __restrict__ p_idx,
                                          const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-                                         int TI, int tilesI, int accumulate, int block) {
-  extern __shared__ double tile[];  // [ncomp][TI][N1]
+                                         int TI, int tilesI, int KC, int accumulate, int block) {
+  extern __shared__ double tile[];  // [2][ncomp][TI][N1]
   constexpr int ncomp = SC ? 4 : 3;
-  const int k = block / tilesI;
-  const int i0 = (block - k * tilesI) * TI;
+  const int chunk = block / tilesI;
+  const int i0 = (block - chunk * tilesI) * TI;
   const int ti = min(TI, c.N0 - i0);
+  const int kb = chunk * KC, ke = min(kb + KC, c.N2);
   const int N1 = c.N1;
   const long N1N0 = (long)N1 * c.N0;
-  const int tsz = TI * N1, own = ti * N1;
-  const long base = N1N0 * k + (long)N1 * i0;
-  for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
-    const int comp = t / own, o = t - comp * own;
-    tile[comp * tsz + o] = accumulate ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
-  }
-  // The ranges: q = 2 half + (kk - k + 1) for half 0 | 1 and plane kk = k-1 | k, rows lo..hi
-  // of each (empty where the plane or the rows do not exist).  Every index below is a
-  // constant after unrolling, so the bounds stay in registers (a dynamically indexed array
-  // would live in scratch memory and cost a memory round trip per item).
+  const int tsz = TI * N1, psz = ncomp * tsz, own = ti * N1;
   const int lo = max(i0 - 1, 0), hi = min(i0 + ti - 1, c.N0 - 2);
-  unsigned start[4], len[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int kk = k - 1 + (q & 1);
-    start[q] = len[q] = 0;
-    if (lo <= hi && kk >= 0 && kk <= c.N2 - 2) {
-      const gptr<const unsigned> row = G((q >> 1) ? m_row : p_row);
-      start[q] = row[(long)kk * c.N0 + lo];
-      len[q] = row[(long)kk * c.N0 + hi + 1] - start[q];
+  // slot k & 1 := node plane k (the buffer's values with `accumulate`, else zero)
+  auto start_plane = [&](int k) {
+    double* p = tile + (k & 1) * psz;
+    const long base = N1N0 * k + (long)N1 * i0;
+    for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
+      const int comp = t / own, o = t - comp * own;
+      p[comp * tsz + o] = accumulate ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
     }
-  }
-  const unsigned e0 = len[0], e1 = e0 + len[1], e2 = e1 + len[2], total = e2 + len[3];
-  __syncthreads();
-  auto locate = [&](unsigned t, HalfItem& it) {
-    const int q = (t >= e0) + (t >= e1) + (t >= e2);
-    const unsigned off = t - (q == 0 ? 0u : q == 1 ? e0 : q == 2 ? e1 : e2);
-    const unsigned st = q == 0 ? start[0] : q == 1 ? start[1] : q == 2 ? start[2] : start[3];
-    it.half = q >> 1;
-    it.kk = k - 1 + (q & 1);
-    return (long)G((q >> 1) ? m_idx : p_idx)[st + off];
   };
-  for (unsigned t = threadIdx.x; t < total; t += 2 * blockDim.x) {
-    HalfItem a, b;
-    const unsigned tb = t + blockDim.x;
-    const long pa = locate(t, a);
-    b.ok = tb < total;
-    const long pb = b.ok ? locate(tb, b) : pa;
-    load_item(a, G(part), cap, pa);
-    load_item(b, G(part), cap, pb);
-    deposit_item<SC>(a, c, tile, TI, k, i0, ti, lo, hi);
-    if (b.ok) deposit_item<SC>(b, c, tile, TI, k, i0, ti, lo, hi);
-  }
+  start_plane(kb);
+  if (kb + 1 < ke) start_plane(kb + 1);
   __syncthreads();
-  for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
-    const int comp = t / own, o = t - comp * own;
-    G((R*)component(J, comp))[base + o] = (R)tile[comp * tsz + o];
+  for (int kc = kb - 1; kc < ke; ++kc) {
+    // the two ranges of cell plane kc: halves 0 (rnp order) and 1 (rnm order), rows lo..hi
+    unsigned s0 = 0, n0 = 0, s1 = 0, n1 = 0;
+    if (lo <= hi && kc >= 0 && kc <= c.N2 - 2) {
+      const long r = (long)kc * c.N0;
+      s0 = G(p_row)[r + lo];
+      n0 = G(p_row)[r + hi + 1] - s0;
+      s1 = G(m_row)[r + lo];
+      n1 = G(m_row)[r + hi + 1] - s1;
+    }
+    const unsigned total = n0 + n1;
+    auto locate = [&](unsigned t, HalfItem& it) {
+      it.half = t >= n0;
+      it.kk = kc;
+      return (long)(it.half ? G(m_idx)[s1 + (t - n0)] : G(p_idx)[s0 + t]);
+    };
+    for (unsigned t = threadIdx.x; t < total; t += 2 * blockDim.x) {
+      HalfItem a, b;
+      const unsigned tb = t + blockDim.x;
+      const long pa = locate(t, a);
+      b.ok = tb < total;
+      const long pb = b.ok ? locate(tb, b) : pa;
+      load_item(a, G(part), cap, pa);
+      load_item(b, G(part), cap, pb);
+      deposit_item<SC>(a, c, tile, psz, TI, kb, ke, i0, ti, lo, hi);
+      if (b.ok) deposit_item<SC>(b, c, tile, psz, TI, kb, ke, i0, ti, lo, hi);
+    }
+    __syncthreads();
+    if (kc >= kb) {  // node plane kc is complete: write it, and its slot becomes plane kc+2
+      double* p = tile + (kc & 1) * psz;
+      const long base = N1N0 * kc + (long)N1 * i0;
+      const bool next = kc + 2 < ke;
+      for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
+        const int comp = t / own, o = t - comp * own;
+        G((R*)component(J, comp))[base + o] = (R)p[comp * tsz + o];
+        if (next) p[comp * tsz + o] = accumulate ? (double)G((const R*)component(J, comp))[base + 2 * N1N0 + o] : 0.0;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -555,8 +594,8 @@ template <typename R, bool SC>
 __global__ __launch_bounds__(kDepositThreads) void k_deposit_rows(
     const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
     const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-    int TI, int tilesI, int accumulate) {
-  deposit_rows_body<R, SC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, accumulate, blockIdx.x);
+    int TI, int tilesI, int KC, int accumulate) {
+  deposit_rows_body<R, SC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, KC, accumulate, blockIdx.x);
 }
 
 // Bunching factor partial sums: per workgroup sum of q cos(k z), q sin(k z), q over its particles.
@@ -776,7 +815,7 @@ __global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* _
       const CellOrder mo = v.ord[1 - v.parity];
       const DepositConst c = v.dep;  // a local copy: the LDS atomics cannot alias it
       deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
-                                  v.dtilesI, 0, e.y);
+                                  v.dtilesI, kDepositChunk, 0, e.y);
     }
     __syncthreads();  // the LDS tile is reused by the next entry
   }
@@ -936,16 +975,15 @@ hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows) {
   return sort_order(d, o, d->np, rows);
 }
 
-// Rows per workgroup tile of the row-tiled deposit: the tile's LDS (ncomp x TI x N1 doubles)
-// stays within 48 KiB (three workgroups per CU); 0 when even one row does not fit.
+// Rows per workgroup tile of the row-tiled deposit: its two-plane LDS ring
+// (2 x ncomp x TI x N1 doubles) stays within 48 KiB (three workgroups per CU); 0 when even one
+// row does not fit.
 int tile_rows(const Domain* d) {
-  const long per_row = (long)d->ncomp * d->g.N1 * (long)sizeof(double);
+  const long per_row = 2L * d->ncomp * d->g.N1 * (long)sizeof(double);
   const long fit = (48L << 10) / per_row;
-  if (fit < 1) return 0;
-  int ti = 1;
-  while (ti * 2 <= fit && ti * 2 <= 16) ti *= 2;
-  return ti;
+  return (int)std::min(fit, 16L);
 }
+
 
 hipError_t deposit_atomic(Domain* d) {
   const DepositConst c = deposit_const(d);
@@ -966,6 +1004,12 @@ hipError_t deposit_atomic(Domain* d) {
 }  // namespace
 
 int deposit_tile_rows(const Domain* d) { return tile_rows(d); }
+
+long deposit_blocks(int N2, int tilesI, int KC) { return (long)tilesI * ((N2 + KC - 1) / KC); }
+
+int deposit_chunk() { return kDepositChunk; }
+
+size_t deposit_lds_bytes(int ncomp, int TI, int N1) { return sizeof(double) * 2 * ncomp * TI * N1; }
 
 DepositConst make_deposit_const(const Domain* d) { return deposit_const(d); }
 
@@ -996,26 +1040,33 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   if ((e = ensure_orders(d, rows)) != hipSuccess) return e;
   if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows)) != hipSuccess) return e;
   if ((e = build_order(d, d->ord[0], 0, rows)) != hipSuccess) return e;
-  const int TI = tile_rows(d);
+  // A single domain must fill the GPU by itself: shorter chunks, then fewer rows per tile,
+  // until there are some 2048 workgroups (each costs a little redundant work at its edges).
+  int TI = tile_rows(d), KC = kDepositChunk;
+  while (deposit_blocks(g.N2, (g.N0 + TI - 1) / TI, KC) < 2048) {
+    if (KC > 2) KC /= 2;
+    else if (TI > 1) TI = (TI + 1) / 2;
+    else break;
+  }
   const int tilesI = (g.N0 + TI - 1) / TI;
   const DepositConst c = deposit_const(d);
   const FieldSet J = d->lvl[d->i_np1];
-  const size_t lds = sizeof(double) * d->ncomp * TI * g.N1;
-  const dim3 grid((unsigned)((long)g.N2 * tilesI));
+  const size_t lds = deposit_lds_bytes(d->ncomp, TI, g.N1);
+  const dim3 grid((unsigned)deposit_blocks(g.N2, tilesI, KC));
   const unsigned *pr = d->ord[0].row, *pi = d->ord[0].idx, *mr = d->ord[1].row, *mi = d->ord[1].idx;
   const bool f64 = d->precision == MITHRA_PRECISION_FP64;
   if (f64 && c.sc)
     hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate);
   else if (f64)
     hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate);
   else if (c.sc)
     hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate);
   else
     hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // Periodic reorder of the particle arrays into the rnp row order.
   if (d->sort_interval > 0 && d->sort_count % d->sort_interval == 0)
