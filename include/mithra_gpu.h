@@ -332,6 +332,10 @@ int mithra_dev_step_timed(mithra_dev* dev, double time, const mithra_seed* seed,
 int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps);
 /* The same with elem_bytes (8, 16 or 32) bytes moved per lane and load. */
 int mithra_copy_bandwidth_elem(unsigned long bytes, int reps, int elem_bytes, double* gbps);
+/* Rate (GB/s, all bytes moved) of the interior stencil's stream pattern without its arithmetic:
+ * three buffers of `bytes` read and one written per launch.  The measured ceiling of the
+ * stencil's access pattern.  No counterpart in the reference. */
+int mithra_stream_bandwidth(unsigned long bytes, int reps, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -102,6 +102,19 @@ def copy_bandwidth(nbytes=2 << 30, reps=10, elem_bytes=32):
     return out.value
 
 
+def stream_bandwidth(nbytes=2 << 30, reps=10):
+    """Measured rate (GB/s, all bytes moved) of three streams read and one written, the interior
+    stencil's access pattern without its arithmetic."""
+    lib = load_gpu_library()
+    f = lib.mithra_stream_bandwidth
+    f.restype, f.argtypes = C.c_int, [C.c_ulong, C.c_int, C.POINTER(C.c_double)]
+    out = C.c_double()
+    rc = f(nbytes, reps, C.byref(out))
+    if rc != 0:
+        raise MithraError(f"mithra_stream_bandwidth: status {rc}")
+    return out.value
+
+
 def _declare(lib, prefix):
     vp = C.c_void_p
     sig = {

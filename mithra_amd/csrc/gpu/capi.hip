@@ -649,9 +649,52 @@ int copy_rate(unsigned long bytes, int reps, double* gbps) {
   (void)hipFree(b);
   return MITHRA_OK;
 }
+// The interior stencil's stream pattern without its arithmetic: three arrays read, one written,
+// 16 bytes per thread per array, the read-once streams and the written one nontemporal.
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(512) void k_stream3r1w(const d2v* __restrict__ a, const d2v* __restrict__ b,
+                                                    const d2v* __restrict__ c, d2v* __restrict__ d, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(a[t] + __builtin_nontemporal_load(b + t) + __builtin_nontemporal_load(c + t), d + t);
+}
+
+int stream_rate(unsigned long bytes, int reps, double* gbps) {
+  const size_t n = bytes / sizeof(d2v);
+  d2v* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MITHRA_OK;
+  for (int i = 0; i < 4 && !rc; ++i)
+    if (hipMalloc(&buf[i], n * sizeof(d2v)) != hipSuccess) rc = MITHRA_E_NOMEM;
+  for (int i = 0; i < 3 && !rc; ++i) rc = hip_status(hipMemset(buf[i], 0, n * sizeof(d2v)));
+  if (!rc) rc = hip_status(hipEventCreate(&e0));
+  if (!rc) rc = hip_status(hipEventCreate(&e1));
+  const unsigned blocks = 65536;
+  float ms = 0.0f;
+  if (!rc) {
+    hipLaunchKernelGGL(k_stream3r1w, dim3(blocks), dim3(512), 0, 0, buf[0], buf[1], buf[2], buf[3], n);  // warm-up
+    rc = hip_status(hipEventRecord(e0, 0));
+  }
+  for (int r = 0; r < reps && !rc; ++r)
+    hipLaunchKernelGGL(k_stream3r1w, dim3(blocks), dim3(512), 0, 0, buf[0], buf[1], buf[2], buf[3], n);
+  if (!rc) rc = hip_status(hipGetLastError());
+  if (!rc) rc = hip_status(hipEventRecord(e1, 0));
+  if (!rc) rc = hip_status(hipEventSynchronize(e1));
+  if (!rc) rc = hip_status(hipEventElapsedTime(&ms, e0, e1));
+  if (!rc) *gbps = 4.0 * n * sizeof(d2v) * reps / (ms * 1e-3) / 1e9;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (d2v* p : buf) (void)hipFree(p);
+  return rc;
+}
 }  // namespace
 
 extern "C" {
+
+int mithra_stream_bandwidth(unsigned long bytes, int reps, double* gbps) {
+  if (!gbps || reps < 1 || bytes < 4096) return MITHRA_E_ARG;
+  return stream_rate(bytes, reps, gbps);
+}
 
 int mithra_copy_bandwidth(unsigned long bytes, int reps, double* gbps) {
   return mithra_copy_bandwidth_elem(bytes, reps, 32, gbps);

@@ -284,3 +284,40 @@ def run_batch(prepared, variant=None, use_graph=True, autotune=True):
     res = [p.result(b.bunching(i)) for i, p in enumerate(prepared)]
     b.close()
     return res, seconds, tuned
+
+
+def main():
+    """`python -m mithra_amd.scan [--instances N] [--mode batch|single] [--seed S]`: sets up the
+    seeded scan through the drop-in host and marches it, printing one JSON line of timings."""
+    import argparse
+    import json
+    import sys
+    import tempfile
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--mode", choices=("batch", "single"), default="batch")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp32")
+    args = ap.parse_args()
+    precision = abi.PRECISION_FP64 if args.precision == "fp64" else abi.PRECISION_FP32_BULK
+    insts = generate(args.instances, seed=args.seed)
+    with tempfile.TemporaryDirectory(prefix="mithra_scan_") as work:
+        t0 = time.perf_counter()
+        prep = prepare(insts, work, precision, log=lambda s: print(f"[scan] {s}", file=sys.stderr, flush=True))
+        t_setup = time.perf_counter() - t0
+        if args.mode == "batch":
+            _, t_march, tuned = run_batch(prep)
+        else:
+            _, t_march = run_single(prep)
+            tuned = None
+        for p in prep:
+            p.dom.close()
+            p.job.close()
+    print(json.dumps({"instances": args.instances, "seed": args.seed, "mode": args.mode,
+                      "precision": args.precision, "setup_s": t_setup, "march_s": t_march,
+                      "points_per_hour": 3600.0 * args.instances / (t_setup + t_march),
+                      "tile_variant": tuned[0] if tuned else None}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

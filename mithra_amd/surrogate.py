@@ -212,3 +212,26 @@ def run_workflow(workdir, n_train=4096, n_hold=512, epochs=600, train_seed=5, ho
                               "surrogate_with_fallback": 3600.0 * n_hold / t_sur,
                               "accepted_at_limit": float(accept.mean()), "spread_limit": SPREAD_LIMIT}
     return out
+
+
+def main():
+    """`python -m mithra_amd.surrogate [--train N] [--hold M] [--epochs E]`: the whole surrogate
+    exercise on one GPU (see run_workflow), printing its summary as one JSON line."""
+    import argparse
+    import json
+    import sys
+    import tempfile
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--train", type=int, default=4096)
+    ap.add_argument("--hold", type=int, default=512)
+    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--single", type=int, default=32, help="held-out points timed on the single-instance path")
+    args = ap.parse_args()
+    with tempfile.TemporaryDirectory(prefix="mithra_surrogate_") as work:
+        out = run_workflow(work, n_train=args.train, n_hold=args.hold, epochs=args.epochs, n_single=args.single,
+                           log=lambda s: print(f"[surrogate] {s}", file=sys.stderr, flush=True))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
