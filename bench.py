@@ -268,11 +268,14 @@ def run_cpu_baseline(job, steps, threads):
 
 def traffic_from_profiles(precision):
     """HBM bytes per launch of the interior stencil from the committed rocprofv3 --pmc
-    summary of this workload, when present."""
-    path = os.path.join(ROOT, "profiles", f"traffic_interior_production_{precision}.json")
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on the stream probe),
+    when present."""
+    path = os.path.join(ROOT, "profiles", f"interior_production_{precision}_traffic.md")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
     return None
 
 

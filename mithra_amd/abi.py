@@ -7,6 +7,7 @@ oracle with prefix ``oracle_``), so both can be driven by one call sequence.
 """
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -164,20 +165,39 @@ def _declare(lib, prefix):
     return fns
 
 
+_DECLARED = {}
+_DECLARE_LOCK = threading.Lock()
+
+
+def _functions(lib, prefix):
+    """The library's entry points under `prefix`, declared once: ctypes function objects are
+    shared, and re-assigning their argument types while another thread calls them is a race."""
+    key = (id(lib), prefix)
+    with _DECLARE_LOCK:
+        if key not in _DECLARED:
+            _DECLARED[key] = (lib, _declare(lib, prefix))
+        return _DECLARED[key][1]
+
+
 _GPU = None
 
 
 def load_gpu_library(path=GPU_LIB):
     """Load libmithra_gpu.so; raises MithraError when it is not built (there is no fallback)."""
     global _GPU
-    if _GPU is None:
-        if not os.path.exists(path):
-            raise MithraError(f"{path} is missing: run __graft_entry__.build() first")
-        lib = C.CDLL(path)
-        lib.mithra_strerror.restype = C.c_char_p
-        lib.mithra_strerror.argtypes = [C.c_int]
-        _GPU = lib
+    with _DECLARE_LOCK:
+        if _GPU is None:
+            _GPU = _open_gpu_library(path)
     return _GPU
+
+
+def _open_gpu_library(path):
+    if not os.path.exists(path):
+        raise MithraError(f"{path} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(path)
+    lib.mithra_strerror.restype = C.c_char_p
+    lib.mithra_strerror.argtypes = [C.c_int]
+    return lib
 
 
 def _ptr(a, ctype=C.c_double):
@@ -190,7 +210,7 @@ class Domain:
     def __init__(self, grid, coeffs, lib=None, prefix="mithra_dev_", precision=PRECISION_FP64):
         self.lib = lib if lib is not None else load_gpu_library()
         self.prefix = prefix
-        self.f = _declare(self.lib, prefix)
+        self.f = _functions(self.lib, prefix)
         self.grid = grid
         self.h = C.c_void_p()
         if prefix == "mithra_dev_":

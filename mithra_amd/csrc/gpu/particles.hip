@@ -103,28 +103,29 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
         // reference too), so such a particle skips the grid field instead.
         if (ci < 0 || ci > c.N0 - 2 || cj < 0 || cj > c.N1 - 2 || ck < 0 || ck > c.N2 - 2) goto boris;
         const long m = (long)ck * N1N0 + (long)ci * N1 + cj;
-        const long off[8] = {0, N1, 1, N1 + 1, N1N0, N1N0 + N1, N1N0 + 1, N1N0 + N1 + 1};
-        const double w[8] = {(1.0 - dxr) * (1.0 - dyr) * (1.0 - dzr), dxr * (1.0 - dyr) * (1.0 - dzr),
-                             (1.0 - dxr) * dyr * (1.0 - dzr),         dxr * dyr * (1.0 - dzr),
-                             (1.0 - dxr) * (1.0 - dyr) * dzr,         dxr * (1.0 - dyr) * dzr,
-                             (1.0 - dxr) * dyr * dzr,                 dxr * dyr * dzr};
-        f4 q0[8], q1[8];
+        // Corner t = (x offset t & 1, y offset (t >> 1) & 1, z offset t >> 2) with weight
+        // w_x w_y w_z in the reference's product order; every component sums its corners in
+        // the order t = 0..7.  The corners come in two z layers of four, so only one layer's
+        // E/B is held in registers at a time.
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          q0[t] = ebg[2 * (m + off[t])];
-          q1[t] = ebg[2 * (m + off[t]) + 1];
-        }
+        for (int h = 0; h < 2; ++h) {
+          f4 q0[4], q1[4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          et[0] += w[t] * (double)q0[t].x;
-          et[1] += w[t] * (double)q0[t].y;
-          et[2] += w[t] * (double)q0[t].z;
-        }
+          for (int u = 0; u < 4; ++u) {
+            const long o = m + (h ? N1N0 : 0) + ((u & 1) ? N1 : 0) + (u >> 1);
+            q0[u] = ebg[2 * o];
+            q1[u] = ebg[2 * o + 1];
+          }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          bt[0] += w[t] * (double)q0[t].w;
-          bt[1] += w[t] * (double)q1[t].x;
-          bt[2] += w[t] * (double)q1[t].y;
+          for (int u = 0; u < 4; ++u) {
+            const double w = ((u & 1) ? dxr : 1.0 - dxr) * ((u >> 1) ? dyr : 1.0 - dyr) * (h ? dzr : 1.0 - dzr);
+            et[0] += w * (double)q0[u].x;
+            et[1] += w * (double)q0[u].y;
+            et[2] += w * (double)q0[u].z;
+            bt[0] += w * (double)q0[u].w;
+            bt[1] += w * (double)q1[u].x;
+            bt[2] += w * (double)q1[u].y;
+          }
         }
       } else if (!b1x && !b1y && b1z) {
         ++nt;

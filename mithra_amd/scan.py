@@ -211,15 +211,22 @@ def bunching_wavenumber(job):
 
 
 class Prepared:
-    """One instance set up on the device: its job, domain and march constants."""
+    """One instance set up on the device: its job, domain and march constants.  The host set-up
+    (job) and the device set-up (open) are separate steps, so that many jobs can be set up on
+    host threads while the device calls stay on one thread."""
 
-    def __init__(self, inst, idx, workdir, precision):
-        self.inst, self.idx = inst, idx
+    def __init__(self, inst, idx, workdir, precision, open_now=True):
+        self.inst, self.idx, self.precision = inst, idx, precision
         self.job = set_up(inst, idx, workdir)
         self.t = self.job.times()
         self.bp, self.und, self.seed = self.job.bunch_params(), self.job.undulators(), self.job.seed()
         self.k = bunching_wavenumber(self.job)
-        self.dom = open_domain(self.job, precision, inst.n_steps)
+        self.dom = None
+        if open_now:
+            self.open()
+
+    def open(self):
+        self.dom = open_domain(self.job, self.precision, self.inst.n_steps)
 
     def result(self, bunching):
         """P(z) and b(z) at the SAMPLES sample steps."""
@@ -229,12 +236,26 @@ class Prepared:
         return P[idx], np.asarray(bunching)[:len(idx)]
 
 
-def prepare(instances, workdir, precision=abi.PRECISION_FP32_BULK, log=None):
-    out = []
-    for i, inst in enumerate(instances):
-        out.append(Prepared(inst, i, workdir, precision))
-        if log and (i + 1) % 8 == 0:
-            log(f"set up {i + 1}/{len(instances)} instances")
+def prepare(instances, workdir, precision=abi.PRECISION_FP32_BULK, log=None, workers=8):
+    """Sets the instances up: the host set-up of the jobs on `workers` host threads (the host
+    library's set-up of one job shares no mutable state with another's, and ctypes releases
+    the interpreter lock during its calls), then each domain's creation and upload on this
+    thread, in order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def host(item):
+        i, inst = item
+        return Prepared(inst, i, workdir, precision, open_now=False)
+
+    if workers <= 1:
+        out = [host(item) for item in enumerate(instances)]
+    else:
+        with ThreadPoolExecutor(max_workers=min(workers, max(1, len(instances)))) as ex:
+            out = list(ex.map(host, enumerate(instances)))
+    for n, p in enumerate(out, 1):
+        p.open()
+        if log and n % 8 == 0:
+            log(f"set up {n}/{len(instances)} instances")
     return out
 
 

@@ -29,9 +29,11 @@ DIMS = [8 + PROFILE_BINS, HIDDEN, HIDDEN, HIDDEN, 2 * scan.SAMPLES]
 BASE_SAMPLE = 7          # seed of the one base sample every surrogate point scales
 
 
-def generate(n, seed, n_perp=(24, 40), n_z=128, n_particles=16384):
-    """Scan points for the surrogate: every beam/undulator scalar varies; the mesh keeps
-    n_z = SAMPLES cells, so P(z) and b(z) are sampled at every step.  The bunch is loaded by a
+def generate(n, seed, n_perp=32, n_z=128, n_particles=16384):
+    """Scan points for the surrogate: every beam/undulator scalar varies on one mesh shape,
+    n_perp x n_perp x n_z cells with n_z = SAMPLES, so P(z) and b(z) are sampled at every step.
+    (The mesh is not among the surrogate's inputs, so it must not vary: P sampled over a
+    transverse window depends on the window's size.)  The bunch is loaded by a
     quiet start with an imposed initial bunching (amplitude 0.5-5 %, random phase), which the
     current profile shows: P(z) is then a function of the inputs rather than of shot noise.  The
     bunch (sigma_z at most 1/16 of the window) stays clear of the power plane 3/8 of the window
@@ -46,7 +48,7 @@ def generate(n, seed, n_perp=(24, 40), n_z=128, n_particles=16384):
     out = []
     for _ in range(n):
         out.append(scan.Instance(
-            n_perp=int(rng.integers(n_perp[0], n_perp[1] + 1)), n_z=n_z,
+            n_perp=n_perp, n_z=n_z,
             n_particles=n_particles, gamma=rng.uniform(100.0, 2000.0),
             K=rng.uniform(0.5, 2.0), lambda_u=rng.uniform(1.0e4, 3.0e4), spread=logu(1e-4, 1e-2),
             seed=BASE_SAMPLE, emittance=logu(0.5, 5.0), current=rng.uniform(200.0, 3000.0),
