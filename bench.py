@@ -169,9 +169,10 @@ class March:
         self.n += 1
 
 
-def run_gpu(job, steps, warmup, precision):
+def run_gpu(job, steps, warmup, precision, reorder=16):
     dom = open_domain(job, precision)
     dom.set_lazy_fields(1)   # E/B where the bunch and the sampler read them (the reference's rule)
+    dom.set_reorder_interval(reorder)
     (ci, ce), tune_ms = dom.autotune(reps=2)
     log("autotune (ms per launch, tile shapes rows x planes): " + "; ".join(
         f"{abi.tile_shape(v)[0]}x{abi.tile_shape(v)[1]} interior {tune_ms[0, v]:.3f} eval {tune_ms[1, v]:.3f}"
@@ -287,6 +288,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--reorder", type=int, default=16,
+                    help="deposits between reorders of the particle arrays into cell-row order (0: never)")
     ap.add_argument("--scan", type=int, default=64,
                     help="instances of the batched parameter scan measured for scan points/hour (0 skips it)")
     args = ap.parse_args()
@@ -305,7 +308,7 @@ def main():
         n_part = job.particles().shape[0]
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
-        elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision)
+        elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision, args.reorder)
         log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
         ms_step = 1e3 * elapsed / args.steps
         cell_rate = nodes * args.steps / elapsed
@@ -342,7 +345,8 @@ def main():
         "config": {"workload": "production single instance (256x256x2048 nodes, 16.8M macro-particles, "
                                "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)",
                    "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
-                   "precision": args.precision, "parallelism": "single GPU, whole domain resident"},
+                   "precision": args.precision, "parallelism": "single GPU, whole domain resident",
+                   "reorder_interval": args.reorder},
         "particle_pushes_per_s": push_rate,
         "scan_points_per_hour": scan_out["points_per_hour_batched"] if scan_out else None,
         "scan": scan_out,

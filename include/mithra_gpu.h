@@ -235,9 +235,9 @@ int mithra_dev_set_reorder_interval(mithra_dev* dev, int interval);
 
 /* Tile shapes of the two marched field kernels (interior update, E/B evaluation): variant v
  * has `rows` x-rows of 64 y-nodes per workgroup and marches `planes` z-planes; variants 7-9
- * stage the interior update's A^n planes in LDS; when no TF/SF seed is injected they also
- * update the Mur y faces, which the boundary pass then skips.  Every shape computes identical
- * values; only the speed differs. */
+ * stage the interior update's A^n planes in LDS.  When no TF/SF seed is injected, the interior
+ * update of every shape also updates the Mur y faces, which the boundary pass then skips.
+ * Every shape computes identical values; only the speed differs. */
 #define MITHRA_TILE_VARIANTS 10
 int mithra_tile_shape(int variant, int* rows, int* planes);
 int mithra_dev_set_tiles(mithra_dev* dev, int interior, int eval);

@@ -238,7 +238,7 @@ void set_tile_geometry(BatchView& v, int variant) {
   v.tilesJ = (v.N1 + 63) / 64;
   v.tilesI = (v.N0 - 2 + TI - 1) / TI;
   v.chunks = (v.N2 - 2 + K - 1) / K;
-  v.yfaces = yface_mask(variant, v.N1);  // no TF/SF in a batch: the LDS forms take the y faces
+  v.yfaces = yface_mask(variant, v.N1);  // no TF/SF in a batch: the interior kernels take the y faces
 }
 
 int prepare(mithra_batch* b) {

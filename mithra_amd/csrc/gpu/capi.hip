@@ -125,7 +125,7 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
   // 256-byte aligned component arrays inside one allocation
   const size_t stride = (per + 255) / 256 * 256;
   if (hipMalloc(&d->pot_block, stride * 3 * d->ncomp) != hipSuccess ||
-      hipMalloc(&d->eb, sizeof(float4) * 2 * (size_t)d->nodes) != hipSuccess ||
+      hipMalloc(&d->eb, sizeof(float) * 2 * kEBPairs * (size_t)d->nodes) != hipSuccess ||
       hipMalloc(&d->counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&d->zrange, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc(&d->zpart, 1024 * 2 * sizeof(int)) != hipSuccess) {
@@ -136,7 +136,7 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
     for (int c = 0; c < 4; ++c)
       d->lvl[l].c[c] = c < d->ncomp ? (char*)d->pot_block + stride * (l * d->ncomp + c) : nullptr;
   int rc = hip_status(launch_fill_zero(d, d->pot_block, stride * 3 * d->ncomp));
-  if (!rc) rc = hip_status(launch_fill_zero(d, d->eb, sizeof(float4) * 2 * (size_t)d->nodes));
+  if (!rc) rc = hip_status(launch_fill_zero(d, d->eb, sizeof(float) * 2 * kEBPairs * (size_t)d->nodes));
   if (!rc) rc = hip_status(launch_fill_zero(d, d->counters, 2 * sizeof(unsigned long long)));
   if (!rc) rc = hip_status(hipStreamSynchronize(d->stream));
   if (rc) { mithra_dev_destroy(d); return rc; }
@@ -225,11 +225,11 @@ int mithra_dev_get_potentials(mithra_dev* d, int level, double* a, double* phi) 
 int mithra_dev_get_fields(mithra_dev* d, float* e, float* b) {
   if (!d) return MITHRA_E_ARG;
   MITHRA_TRY(hipStreamSynchronize(d->stream));
-  std::vector<float4> h(2 * (size_t)d->nodes);
-  MITHRA_TRY(hipMemcpy(h.data(), d->eb, sizeof(float4) * h.size(), hipMemcpyDeviceToHost));
+  std::vector<float> h(2 * kEBPairs * (size_t)d->nodes);  // per node: Ex Ey Ez Bx By Bz
+  MITHRA_TRY(hipMemcpy(h.data(), d->eb, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
   for (size_t m = 0; m < (size_t)d->nodes; ++m) {
-    if (e) { e[3 * m] = h[2 * m].x; e[3 * m + 1] = h[2 * m].y; e[3 * m + 2] = h[2 * m].z; }
-    if (b) { b[3 * m] = h[2 * m].w; b[3 * m + 1] = h[2 * m + 1].x; b[3 * m + 2] = h[2 * m + 1].y; }
+    if (e) { e[3 * m] = h[6 * m]; e[3 * m + 1] = h[6 * m + 1]; e[3 * m + 2] = h[6 * m + 2]; }
+    if (b) { b[3 * m] = h[6 * m + 3]; b[3 * m + 1] = h[6 * m + 4]; b[3 * m + 2] = h[6 * m + 5]; }
   }
   return MITHRA_OK;
 }

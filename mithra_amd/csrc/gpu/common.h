@@ -26,6 +26,11 @@ template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
 // Four floats as a builtin vector (float4's member operators cannot act on address space 1).
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+
+// E/B of node m: six floats {Ex, Ey | Ez, Bx | By, Bz}, 24 bytes, read and written as three
+// 8-byte pairs at eb[3 m], eb[3 m + 1], eb[3 m + 2] (f2 units).
+constexpr int kEBPairs = 3;
 template <typename T>
 __device__ inline gptr<T> G(T* p) {
   return (gptr<T>)p;
@@ -47,11 +52,11 @@ struct MurCoeffs {
   double bB[5], cB[5], dB[5], eE[5], fE[5], gE[5], hC[17];
 };
 
-// Mur y faces (j = 0 and j = N1-1) that the LDS-staged interior kernels (tile variants 7..9)
-// update together with the interior when no TF/SF write comes between the two: bit 0 the j = 0
-// face, bit 1 the j = N1-1 face, the latter only when node N1-2 shares its 64-node tile row.
-__host__ __device__ inline int yface_mask(int variant, int N1) {
-  if (variant < 7 || N1 < 3) return 0;
+// Mur y faces (j = 0 and j = N1-1) that the interior kernels (every tile shape) update together
+// with the interior when no TF/SF write comes between the two: bit 0 the j = 0 face, bit 1 the
+// j = N1-1 face, the latter only when node N1-2 shares its 64-node tile row.
+__host__ __device__ inline int yface_mask(int, int N1) {
+  if (N1 < 3) return 0;
   return 1 | ((N1 - 1) % 64 != 0 ? 2 : 0);
 }
 
@@ -124,7 +129,7 @@ struct Domain {
   void* pot_block;          // one allocation: 3 levels x ncomp x nodes
   FieldSet lvl[3];          // physical buffers
   int i_np1, i_n, i_nm1;    // which physical buffer is A^{n+1}, A^n, A^{n-1}
-  float4* eb;               // 2 per node
+  float* eb;                // E/B, 6 floats per node (kEBPairs pairs)
   // particles
   long np;                  // particle count
   long cap;                 // allocated capacity
@@ -196,7 +201,7 @@ struct BatchView {
   // mesh, potentials, E/B
   int N0, N1, N2, ncomp, nsfd, trunc2;
   FieldSet lvl[3];
-  float4* eb;
+  float* eb;
   int tilesJ, tilesI, chunks;  // interior / E-B tiles of the batch's tile shape
   int yfaces;               // Mur y faces the interior kernels update (yface_mask of the tile shape)
   double a[6], alpha, beta;

@@ -46,54 +46,85 @@ __device__ inline Plane5 load_plane(PT a, long m, long N1) {
 // and marches kChunkK planes along z with the three planes it needs held in registers; the
 // x/y neighbours it reads are the centres of neighbouring lanes and rows, served by L1/L2.
 // J (or rho) lives in the A^{n+1} buffer and is overwritten in place, as in the reference.
-template <typename R, bool NSFD, int TI, int K>
+template <typename R, bool NSFD, int TI, int K, bool FACES = false>
 __device__ inline void interior_body(FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0,
                                      int N1, int N2, int tilesJ, int tilesI, int chunks, double a0, double a1,
-                                     double a2, double a3, double a4, double a5, double alpha, double beta, int lb) {
+                                     double a2, double a3, double a4, double a5, double alpha, double beta, int lb,
+                                     const MurYFace& yf = MurYFace{}) {
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
   lb /= tilesI;
   const int kc = lb % chunks;
   const int comp = lb / chunks;
-  const int j = tj * kTileJ + (threadIdx.x % kTileJ);
+  const int lane = threadIdx.x % kTileJ;
+  const int j = tj * kTileJ + lane;
   const int i = 1 + ti * TI + (threadIdx.x / kTileJ);
   const int kb = 1 + kc * K;
   const int ke = min(kb + K, N2 - 1);
-  if (comp >= ncomp || i > N0 - 2 || j < 1 || j > N1 - 2 || kb >= ke) return;
+  if (comp >= ncomp || kb >= ke) return;  // uniform over the workgroup
+  // FACES: as in interior_lds_body, the lanes of the face nodes j = 0 and N1-1 update the Mur y
+  // faces in `yf.mask`; they take the inward neighbour's new value, its A^{n-1} and its x
+  // neighbours of A^n by lane shuffle, and their own A^n column from their register queue.
+  const bool face_tile = FACES && (tj == 0 || tj == (N1 - 1) / kTileJ);
+  const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
+  const bool face = FACES && i <= N0 - 2 && (((yf.mask & 1) && j == 0) || ((yf.mask & 2) && j == N1 - 1));
+  if (!active && !face_tile) return;
+  const bool load = active || face;
+  const int fs = j == 0 ? 1 : -1;
+  const int nb_lane = max(lane + fs, 0);
   const gptr<const R> an = G((const R*)component(an_set, comp));
   const gptr<const R> anm1 = G((const R*)component(anm1_set, comp));
   const gptr<R> anp1 = G((R*)component(anp1_set, comp));
   const double as = comp < 3 ? a4 : a5;
   const long N1N0 = (long)N1 * N0;
   long m = N1N0 * kb + (long)N1 * i + j;
-  Plane5 pm = load_plane(an, m - N1N0, N1);
-  Plane5 p0 = load_plane(an, m, N1);
+  Plane5 pm = {}, p0 = {};
+  if (load) {
+    pm = load_plane(an, m - N1N0, N1);
+    p0 = load_plane(an, m, N1);
+  }
   for (int k = kb; k < ke; ++k, m += N1N0) {
-    Plane5 pp = load_plane(an, m + N1N0, N1);
-    double v1 = (double)ld_stream(anm1, m);
-    double src = (double)ld_stream(G((const R*)component(anp1_set, comp)), m);
-    double r;
-    if (NSFD) {
-      double tx = p0.xp + p0.xm + beta * (pp.xp + pm.xp + pp.xm + pm.xm);
-      double ty = p0.yp + p0.ym + beta * (pp.yp + pm.yp + pp.ym + pm.ym);
-      r = a0 * p0.c - v1 + alpha * (a1 * tx + a2 * ty) + a3 * (pp.c + pm.c) + as * src;
-    } else {
-      r = a0 * p0.c - v1 + a1 * (p0.xp + p0.xm) + a2 * (p0.yp + p0.ym) + a3 * (pp.c + pm.c) + as * src;
+    Plane5 pp = {};
+    double v1 = 0.0, r = 0.0;
+    if (load) {
+      pp = load_plane(an, m + N1N0, N1);
+      v1 = (double)ld_stream(anm1, m);
     }
-    st_stream(anp1, m, (R)r);
+    if (active) {
+      const double src = (double)ld_stream(G((const R*)component(anp1_set, comp)), m);
+      if (NSFD) {
+        double tx = p0.xp + p0.xm + beta * (pp.xp + pm.xp + pp.xm + pm.xm);
+        double ty = p0.yp + p0.ym + beta * (pp.yp + pm.yp + pp.ym + pm.ym);
+        r = a0 * p0.c - v1 + alpha * (a1 * tx + a2 * ty) + a3 * (pp.c + pm.c) + as * src;
+      } else {
+        r = a0 * p0.c - v1 + a1 * (p0.xp + p0.xm) + a2 * (p0.yp + p0.ym) + a3 * (pp.c + pm.c) + as * src;
+      }
+      st_stream(anp1, m, (R)r);
+    }
+    if (face_tile) {
+      const double pn = __shfl((double)(R)r, nb_lane);
+      const double mn = __shfl(v1, nb_lane);
+      const double nxp = __shfl(p0.xp, nb_lane), nxm = __shfl(p0.xm, nb_lane);
+      if (face) {
+        const double* B = yf.B;
+        const double ns = fs > 0 ? p0.yp : p0.ym, nsp = fs > 0 ? pp.yp : pp.ym, nsm = fs > 0 ? pm.yp : pm.ym;
+        anp1[m] = (R)(B[0] * (v1 + pn) + B[1] * mn + B[2] * (p0.c + ns) + B[3] * (nxp + nxm + p0.xp + p0.xm) +
+                      B[4] * (nsp + nsm + pp.c + pm.c));
+      }
+    }
     pm = p0;
     p0 = pp;
   }
 }
 
-template <typename R, bool NSFD, int TI, int K>
+template <typename R, bool NSFD, int TI, int K, bool FACES>
 __global__ __launch_bounds__(kTileJ* TI) void k_interior(
     FieldSet an_set, FieldSet anm1_set, FieldSet anp1_set, int ncomp, int N0, int N1, int N2,
     int tilesJ, int tilesI, int chunks, double a0, double a1, double a2, double a3, double a4, double a5,
-    double alpha, double beta) {
-  interior_body<R, NSFD, TI, K>(an_set, anm1_set, anp1_set, ncomp, N0, N1, N2, tilesJ, tilesI, chunks, a0, a1, a2,
-                                a3, a4, a5, alpha, beta, xcd_logical_block(blockIdx.x, gridDim.x));
+    double alpha, double beta, MurYFace yf) {
+  interior_body<R, NSFD, TI, K, FACES>(an_set, anm1_set, anp1_set, ncomp, N0, N1, N2, tilesJ, tilesI, chunks, a0,
+                                       a1, a2, a3, a4, a5, alpha, beta, xcd_logical_block(blockIdx.x, gridDim.x), yf);
 }
 
 // The interior update with the A^n planes staged in LDS (2.5-D blocking).  The workgroup (TI
@@ -533,10 +564,10 @@ __device__ inline bool lazy_needed(int kb, int ke, const int* zr, int pb0, int p
 // phi) at both time levels ride in a three-plane register queue, the transverse neighbours are
 // the centres of neighbouring lanes and rows.  Planes outside [klo, khi] are skipped.
 template <typename R, bool SC, int TI, int K>
-__device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1, int N2,
+__device__ inline void eval_body(FieldSet An, FieldSet Ap, float* __restrict__ eb, int N0, int N1, int N2,
                                  int tilesJ, int tilesI, int klo, int khi, const EvalConst& q,
                                  const int* __restrict__ zr, int pb0, int pb1, int lb) {
-  const gptr<f4> ebg = G(reinterpret_cast<f4*>(eb));
+  const gptr<f2> ebg = G(reinterpret_cast<f2*>(eb));
   const int tj = lb % tilesJ;
   lb /= tilesJ;
   const int ti = lb % tilesI;
@@ -586,18 +617,15 @@ __device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ 
                                     dz_(qp0[2] - qp0[0]) - dx_(rd(p2, m + N1) - rd(p2, m - N1))));
     const float bz = (float)(0.5 * (dx_(rd(n1, m + N1) - rd(n1, m - N1)) - dy_(rd(n0, m + 1) - rd(n0, m - 1)) +
                                     dx_(rd(p1, m + N1) - rd(p1, m - N1)) - dy_(rd(p0, m + 1) - rd(p0, m - 1))));
-    const f4 v0 = {(float)ex, (float)ey, ez, bx};
-    const f4 v1 = {by, bz, 0.0f, 0.0f};
-    ebg[2 * m] = v0;
-    ebg[2 * m + 1] = v1;
-    if (k == 1) {
-      ebg[2 * (m - N1N0)] = v0;
-      ebg[2 * (m - N1N0) + 1] = v1;
-    }
-    if (k == N2 - 2) {
-      ebg[2 * (m + N1N0)] = v0;
-      ebg[2 * (m + N1N0) + 1] = v1;
-    }
+    const f2 v0 = {(float)ex, (float)ey}, v1 = {ez, bx}, v2 = {by, bz};
+    auto put = [&](long node) {
+      ebg[kEBPairs * node] = v0;
+      ebg[kEBPairs * node + 1] = v1;
+      ebg[kEBPairs * node + 2] = v2;
+    };
+    put(m);
+    if (k == 1) put(m - N1N0);
+    if (k == N2 - 2) put(m + N1N0);
     qn0[0] = qn0[1]; qn1[0] = qn1[1]; qp0[0] = qp0[1]; qp1[0] = qp1[1];
     qn0[1] = qn0[2]; qn1[1] = qn1[2]; qp0[1] = qp0[2]; qp1[1] = qp1[2];
     if (SC) { qf[0] = qf[1]; qf[1] = qf[2]; }
@@ -605,7 +633,7 @@ __device__ inline void eval_body(FieldSet An, FieldSet Ap, float4* __restrict__ 
 }
 
 template <typename R, bool SC, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
+__global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap, float* __restrict__ eb, int N0,
                                                         int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
                                                         EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
   eval_body<R, SC, TI, K>(An, Ap, eb, N0, N1, N2, tilesJ, tilesI, klo, khi, q, zr, pb0, pb1,
@@ -617,10 +645,10 @@ __global__ __launch_bounds__(kTileJ* TI) void k_eval_eb(FieldSet An, FieldSet Ap
 // read from LDS, the z neighbours of A_x, A_y (and phi) ride in a register queue, and the
 // next plane is fetched while the current one is computed.  Same expressions as k_eval_eb.
 template <typename R, bool SC, int TI, int K>
-__device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0, int N1, int N2,
+__device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict__ eb, int N0, int N1, int N2,
                                      int tilesJ, int tilesI, int klo, int khi, const EvalConst& q,
                                      const int* __restrict__ zr, int pb0, int pb1, int lb) {
-  const gptr<f4> ebg = G(reinterpret_cast<f4*>(eb));
+  const gptr<f2> ebg = G(reinterpret_cast<f2*>(eb));
   constexpr int NA = SC ? 7 : 6;      // staged arrays: n0 n1 n2 p0 p1 p2 [phi]
   constexpr int NZ = SC ? 5 : 4;      // arrays with z neighbours: n0 n1 p0 p1 [phi]
   constexpr int W = kTileJ + 2, P = (TI + 2) * W;
@@ -728,18 +756,15 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restric
                                       dz_(qz[2][2] - qz[2][0]) - dx_(p2l[W] - p2l[-W])));
       const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
                                       dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
-      const f4 v0 = {(float)ex, (float)ey, ez, bx};
-      const f4 v1 = {by, bz, 0.0f, 0.0f};
-      ebg[2 * m] = v0;
-      ebg[2 * m + 1] = v1;
-      if (k == 1) {
-        ebg[2 * (m - N1N0)] = v0;
-        ebg[2 * (m - N1N0) + 1] = v1;
-      }
-      if (k == N2 - 2) {
-        ebg[2 * (m + N1N0)] = v0;
-        ebg[2 * (m + N1N0) + 1] = v1;
-      }
+      const f2 v0 = {(float)ex, (float)ey}, v1 = {ez, bx}, v2 = {by, bz};
+      auto put = [&](long node) {
+        ebg[kEBPairs * node] = v0;
+        ebg[kEBPairs * node + 1] = v1;
+        ebg[kEBPairs * node + 2] = v2;
+      };
+      put(m);
+      if (k == 1) put(m - N1N0);
+      if (k == N2 - 2) put(m + N1N0);
     }
     __syncthreads();
 #pragma unroll
@@ -752,7 +777,7 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float4* __restric
 }
 
 template <typename R, bool SC, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet Ap, float4* __restrict__ eb, int N0,
+__global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet Ap, float* __restrict__ eb, int N0,
                                                          int N1, int N2, int tilesJ, int tilesI, int klo, int khi,
                                                          EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
   eval_lds_body<R, SC, TI, K>(An, Ap, eb, N0, N1, N2, tilesJ, tilesI, klo, khi, q, zr, pb0, pb1,
@@ -779,10 +804,14 @@ __global__ __launch_bounds__(kTileJ* TI) void kb_interior(const BatchView* __res
         interior_lds_body<R, NSFD, TI, K, true>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1,
                                                 v.N2, v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3],
                                                 v.a[4], v.a[5], v.alpha, v.beta, e.y, yf);
-      } else
-        interior_body<R, NSFD, TI, K>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1, v.N2,
-                                      v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3], v.a[4], v.a[5],
-                                      v.alpha, v.beta, e.y);
+      } else {
+        MurYFace yf;
+        for (int q = 0; q < 5; ++q) yf.B[q] = v.mur.cB[q];
+        yf.mask = v.yfaces;
+        interior_body<R, NSFD, TI, K, true>(v.lvl[v.i_n], v.lvl[v.i_nm1], v.lvl[v.i_np1], v.ncomp, v.N0, v.N1, v.N2,
+                                            v.tilesJ, v.tilesI, v.chunks, v.a[0], v.a[1], v.a[2], v.a[3], v.a[4],
+                                            v.a[5], v.alpha, v.beta, e.y, yf);
+      }
     }
     if constexpr (LDS) __syncthreads();  // the LDS ring is reused by the next tile
   }
@@ -866,10 +895,15 @@ hipError_t interior_launch(Domain* d, int ymask) {
       hipLaunchKernelGGL((k_interior_lds<R, NSFD, TI, K, false>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0,
                          d->stream, d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2,
                          tilesJ, tilesI, chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta, yf);
-  } else
-    hipLaunchKernelGGL((k_interior<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
+  } else if (ymask) {
+    hipLaunchKernelGGL((k_interior<R, NSFD, TI, K, true>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
                        d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
-                       chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta);
+                       chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta, yf);
+  } else {
+    hipLaunchKernelGGL((k_interior<R, NSFD, TI, K, false>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
+                       d->lvl[d->i_n], d->lvl[d->i_nm1], d->lvl[d->i_np1], d->ncomp, g.N0, g.N1, g.N2, tilesJ, tilesI,
+                       chunks, a[0], a[1], a[2], a[3], a[4], a[5], d->co.alpha, d->co.beta, yf);
+  }
   return hipGetLastError();
 }
 
@@ -898,7 +932,7 @@ hipError_t field_variant(Domain* d, int kernel, int v, int klo, int khi, const i
   // kernel 0: interior (FLAG = NSFD), kernel 1: E/B evaluation (FLAG = space charge)
 #define MITHRA_VARIANT(idx, TI, K)                                                            \
   case idx:                                                                                    \
-    return kernel == 0 ? interior_launch<R, FLAG, TI, K>(d, 0) : eval_launch<R, FLAG, TI, K>(d, klo, khi, zr, pb0, pb1);
+    return kernel == 0 ? interior_launch<R, FLAG, TI, K>(d, ymask) : eval_launch<R, FLAG, TI, K>(d, klo, khi, zr, pb0, pb1);
   switch (v) {
     MITHRA_VARIANT(0, 4, 64)
     MITHRA_VARIANT(1, 2, 64)

@@ -65,12 +65,12 @@ __device__ void undulator_field(const PushConst& c, UT und, double x, double y, 
 // E/B grid does not change between sub-steps, so a particle is read from HBM once per field
 // step and advanced n_sub times in registers; the time of each sub-step is accumulated exactly
 // as Solver::solve() accumulates timeBunch_ (src/solver.cpp:1316-1321).
-__device__ inline void push_body(double* __restrict__ part, long cap, long n, const float4* __restrict__ eb,
+__device__ inline void push_body(double* __restrict__ part, long cap, long n, const float* __restrict__ eb,
                                  const mithra_undulator* __restrict__ und, const PushConst& c,
                                  unsigned long long* __restrict__ counters, long p) {
   if (p >= n) return;
   const gptr<double> X = G(part);
-  const gptr<const f4> ebg = G(reinterpret_cast<const f4*>(eb));
+  const gptr<const f2> ebg = G(reinterpret_cast<const f2*>(eb));
   const gptr<const mithra_undulator> undg = G(und);
   double x = X[PX * cap + p], y = X[PY * cap + p], z = X[PZ * cap + p];
   double gx = X[PGX * cap + p], gy = X[PGY * cap + p], gz = X[PGZ * cap + p];
@@ -109,22 +109,23 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
         // E/B is held in registers at a time.
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          f4 q0[4], q1[4];
+          f2 q0[4], q1[4], q2[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const long o = m + (h ? N1N0 : 0) + ((u & 1) ? N1 : 0) + (u >> 1);
-            q0[u] = ebg[2 * o];
-            q1[u] = ebg[2 * o + 1];
+            q0[u] = ebg[kEBPairs * o];
+            q1[u] = ebg[kEBPairs * o + 1];
+            q2[u] = ebg[kEBPairs * o + 2];
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const double w = ((u & 1) ? dxr : 1.0 - dxr) * ((u >> 1) ? dyr : 1.0 - dyr) * (h ? dzr : 1.0 - dzr);
             et[0] += w * (double)q0[u].x;
             et[1] += w * (double)q0[u].y;
-            et[2] += w * (double)q0[u].z;
-            bt[0] += w * (double)q0[u].w;
-            bt[1] += w * (double)q1[u].x;
-            bt[2] += w * (double)q1[u].y;
+            et[2] += w * (double)q1[u].x;
+            bt[0] += w * (double)q1[u].y;
+            bt[1] += w * (double)q2[u].x;
+            bt[2] += w * (double)q2[u].y;
           }
         }
       } else if (!b1x && !b1y && b1z) {
@@ -179,7 +180,7 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
 }
 
 __global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long cap, long n,
-                                              const float4* __restrict__ eb, const mithra_undulator* __restrict__ und,
+                                              const float* __restrict__ eb, const mithra_undulator* __restrict__ und,
                                               PushConst c, unsigned long long* __restrict__ counters) {
   push_body(part, cap, n, eb, und, c, counters, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }

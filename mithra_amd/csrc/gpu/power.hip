@@ -12,14 +12,14 @@ namespace {
 
 constexpr int kPowerBlock = 256;
 
-__device__ inline void power_nodes_body(const float4* __restrict__ eb, double* __restrict__ ring,
+__device__ inline void power_nodes_body(const float* __restrict__ eb, double* __restrict__ ring,
                                         const double* __restrict__ dft, const int* __restrict__ plane_k,
                                         const double* __restrict__ plane_dzr, double* __restrict__ partial, int N0,
                                         int N1, unsigned Nz, unsigned Nl, unsigned Nf, unsigned slot, double gamma,
                                         double beta, double c0, double pc, int bx, int nbx, unsigned pz) {
   const long ni_count = (long)(N0 - 4) * (N1 - 4);
   const long t = (long)bx * blockDim.x + threadIdx.x;
-  const gptr<const f4> ebg = G(reinterpret_cast<const f4*>(eb));
+  const gptr<const f2> ebg = G(reinterpret_cast<const f2*>(eb));
   const gptr<double> ringg = G(ring);
   const gptr<const double> dftg = G(dft);
   const long N1N0 = (long)N1 * N0;
@@ -34,12 +34,13 @@ __device__ inline void power_nodes_body(const float4* __restrict__ eb, double* _
       const size_t base = (size_t)pz * N1N0 + node;
       if (l == 0) {
         const double dzr = G(plane_dzr)[pz];
-        const f4 a0 = ebg[2 * mi], a1 = ebg[2 * mi + 1];
-        const f4 b0 = ebg[2 * (mi + N1N0)], b1 = ebg[2 * (mi + N1N0) + 1];
+        const long na = kEBPairs * mi, nb = kEBPairs * (mi + N1N0);
+        const f2 a0 = ebg[na], a1 = ebg[na + 1], a2 = ebg[na + 2];  // {Ex, Ey}, {Ez, Bx}, {By, Bz}
+        const f2 b0 = ebg[nb], b1 = ebg[nb + 1], b2 = ebg[nb + 2];
         const double et0 = (1.0 - dzr) * (double)a0.x + dzr * (double)b0.x;
         const double et1 = (1.0 - dzr) * (double)a0.y + dzr * (double)b0.y;
-        const double bt0 = (1.0 - dzr) * (double)a0.w + dzr * (double)b0.w;
-        const double bt1 = (1.0 - dzr) * (double)a1.x + dzr * (double)b1.x;
+        const double bt0 = (1.0 - dzr) * (double)a1.y + dzr * (double)b1.y;
+        const double bt1 = (1.0 - dzr) * (double)a2.x + dzr * (double)b2.x;
         const gptr<double> r = ringg + (size_t)slot * ring_slot;
         r[0 * Nz * N1N0 + base] = gamma * (et0 + c0 * beta * bt1);
         r[1 * Nz * N1N0 + base] = gamma * (et1 - c0 * beta * bt0);
@@ -72,7 +73,7 @@ __device__ inline void power_nodes_body(const float4* __restrict__ eb, double* _
 }
 
 __global__ __launch_bounds__(kPowerBlock) void k_power_nodes(
-    const float4* __restrict__ eb, double* __restrict__ ring, const double* __restrict__ dft,
+    const float* __restrict__ eb, double* __restrict__ ring, const double* __restrict__ dft,
     const int* __restrict__ plane_k, const double* __restrict__ plane_dzr, double* __restrict__ partial, int N0,
     int N1, unsigned Nz, unsigned Nl, unsigned Nf, unsigned slot, double gamma, double beta, double c0, double pc) {
   power_nodes_body(eb, ring, dft, plane_k, plane_dzr, partial, N0, N1, Nz, Nl, Nf, slot, gamma, beta, c0, pc,

@@ -110,7 +110,7 @@ class Surrogate:
         self.mlp = abi.MLP(DIMS, members=MEMBERS, seed=seed)
         self.x_mean = self.x_std = self.y_mean = self.y_std = None
 
-    def fit(self, X, Y, epochs=300, batch=256, lr=1e-3, seed=0, log=None):
+    def fit(self, X, Y, epochs=2000, batch=256, lr=1e-3, seed=0, log=None):
         self.x_mean, self.x_std = X.mean(0), X.std(0) + 1e-6
         self.x_mean[8:], self.x_std[8:] = 0.0, 1.0          # the profile stays as it is
         self.y_mean, self.y_std = Y.mean(0), Y.std(0) + 1e-6
@@ -153,8 +153,8 @@ def p_error(P, P_ref):
     return float(np.sum(np.abs(P - P_ref)) / max(np.sum(np.abs(P_ref)), 1e-300))
 
 
-def run_workflow(workdir, n_train=4096, n_hold=512, epochs=600, train_seed=5, hold_seed=6, n_single=32,
-                 log=None):
+def run_workflow(workdir, n_train=4096, n_hold=512, epochs=2000, train_seed=5, hold_seed=6, n_single=32,
+                 log=None, save=None):
     """The whole surrogate exercise on one GPU: batched FP32-bulk solves of n_train points, the
     held-out points solved in FP32-bulk and re-solved in FP64 (the reference for every error),
     training, the spread-gated decision, and scan points/hour of three ways to evaluate the
@@ -172,6 +172,8 @@ def run_workflow(workdir, n_train=4096, n_hold=512, epochs=600, train_seed=5, ho
     out["fp32_bulk_p_err_vs_fp64"] = {"max": max(p_error(P, r) for (P, _), r in zip(res_h32, P64)),
                                       "median": float(np.median([p_error(P, r) for (P, _), r in zip(res_h32, P64)]))}
     Xtr, Ytr, Xh = features(train), targets(res_tr), features(hold)
+    if save:  # the generated data set: features, targets, FP64 held-out P(z) and b(z)
+        np.savez(save, Xtr=Xtr, Ytr=Ytr, Xh=Xh, Ph64=P64, bh64=b64)
     model = Surrogate(seed=1)
     t0 = time.perf_counter()
     hist = model.fit(Xtr, Ytr, epochs=epochs, log=log)
@@ -226,12 +228,13 @@ def main():
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--train", type=int, default=4096)
     ap.add_argument("--hold", type=int, default=512)
-    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--epochs", type=int, default=2000)
     ap.add_argument("--single", type=int, default=32, help="held-out points timed on the single-instance path")
+    ap.add_argument("--save", default=None, help="write the generated data set to this .npz file")
     args = ap.parse_args()
     with tempfile.TemporaryDirectory(prefix="mithra_surrogate_") as work:
         out = run_workflow(work, n_train=args.train, n_hold=args.hold, epochs=args.epochs, n_single=args.single,
-                           log=lambda s: print(f"[surrogate] {s}", file=sys.stderr, flush=True))
+                           log=lambda s: print(f"[surrogate] {s}", file=sys.stderr, flush=True), save=args.save)
     print(json.dumps(out), flush=True)
 
 
