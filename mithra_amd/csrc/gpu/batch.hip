@@ -334,7 +334,7 @@ int mithra_batch_instance(mithra_batch* b, int i, double time, double time_bunch
   mithra_dev* d = b->dev[i];
   BATCH_TRY(hipStreamSynchronize(d->stream));
   if (int rc = settle(d)) return rc;
-  if (int rc = upload_undulators(d, und, n_und)) return rc;
+  if (int rc = upload_undulators(d, und, n_und, p->c0)) return rc;
   const mithra_grid& g = d->g;
   const unsigned rows = (unsigned)((g.N2 - 1) * (long)g.N0);
   if (d->np > 0) {

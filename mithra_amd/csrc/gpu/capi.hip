@@ -35,23 +35,37 @@ bool valid_grid(const mithra_grid* g) {
 
 }  // namespace
 
-int mithra::upload_undulators(mithra_dev* d, const mithra_undulator* und, int n) {
+int mithra::upload_undulators(mithra_dev* d, const mithra_undulator* und, int n, double c0) {
   if (n < 0 || (n > 0 && !und)) return MITHRA_E_ARG;
   for (int u = 0; u < n; ++u)
     if (und[u].type != MITHRA_UNDULATOR_STATIC) return MITHRA_E_UNSUPPORTED;
-  if ((int)d->und_host.size() == n && (n == 0 || std::memcmp(d->und_host.data(), und, sizeof(*und) * n) == 0))
+  if ((int)d->und_host.size() == n && d->und_c0 == c0 &&
+      (n == 0 || std::memcmp(d->und_host.data(), und, sizeof(*und) * n) == 0))
     return MITHRA_OK;
   if (n > d->und_cap) {
     if (d->und_dev) MITHRA_TRY(hipFree(d->und_dev));
     d->und_dev = nullptr;
-    MITHRA_TRY(hipMalloc(&d->und_dev, sizeof(mithra_undulator) * n));
+    MITHRA_TRY(hipMalloc(&d->und_dev, sizeof(UndDev) * n));
     d->und_cap = n;
+  }
+  std::vector<UndDev> t(n);
+  for (int u = 0; u < n; ++u) {  // the push's expressions, src/solver.cpp:1805-1812
+    const mithra_undulator& U = und[u];
+    UndDev& v = t[u];
+    v.k = U.k; v.lu = U.lu; v.rb = U.rb; v.dist = U.dist; v.theta = U.theta; v.length = U.length;
+    v.b0 = (U.lu != 0.0) ? kEM * c0 * 2 * kPI / U.lu * U.k / kEC : 0.0;
+    v.ku = (U.lu != 0.0) ? 2 * kPI / U.lu : 0.0;
+    v.ct = std::cos(U.theta);
+    v.st = std::sin(U.theta);
+    v.L = U.length * U.lu;
   }
   if (n) {
     MITHRA_TRY(hipStreamSynchronize(d->stream));
-    MITHRA_TRY(hipMemcpy(d->und_dev, und, sizeof(*und) * n, hipMemcpyHostToDevice));
+    MITHRA_TRY(hipMemcpy(d->und_dev, t.data(), sizeof(UndDev) * n, hipMemcpyHostToDevice));
   }
   d->und_host.assign(und, und + n);
+  d->und_c0 = c0;
+  d->und_dist0 = n > 0 ? und[0].dist : 0.0;
   return MITHRA_OK;
 }
 
@@ -318,7 +332,7 @@ int mithra_dev_bunch_begin(mithra_dev* d) {
 int mithra_dev_bunch_update(mithra_dev* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
                             double time_bunch) {
   if (!d || !p) return MITHRA_E_ARG;
-  int rc = upload_undulators(d, und, n_und);
+  int rc = upload_undulators(d, und, n_und, p->c0);
   if (rc) return rc;
   return hip_status(launch_push(d, p, d->und_dev, n_und, time_bunch, 0.0, 1, 0));
 }
@@ -326,7 +340,7 @@ int mithra_dev_bunch_update(mithra_dev* d, const mithra_bunch_params* p, const m
 int mithra_dev_bunch_advance(mithra_dev* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
                              double time_bunch, double bunch_time_step, int n_sub) {
   if (!d || !p || n_sub < 0) return MITHRA_E_ARG;
-  int rc = upload_undulators(d, und, n_und);
+  int rc = upload_undulators(d, und, n_und, p->c0);
   if (rc) return rc;
   return hip_status(launch_push(d, p, d->und_dev, n_und, time_bunch, bunch_time_step, n_sub, 1));
 }
@@ -581,7 +595,7 @@ int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, c
   if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
       seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
     return MITHRA_E_UNSUPPORTED;
-  int rc = upload_undulators(d, und, n_und);
+  int rc = upload_undulators(d, und, n_und, p->c0);
   if (!rc) rc = settle(d);
   if (rc) return rc;
   hipStream_t s = d->stream;

@@ -93,6 +93,15 @@ struct EvalConst {  // -dt, twice the steps, and their correctly rounded recipro
   double mdt, dx2, dy2, dz2, rmdt, rdx2, rdy2, rdz2;
 };
 
+// Device copy of one static undulator with its per-undulator constants, computed once at upload
+// in the push's operation order (Solver::undulatorField, src/solver.cpp:1798-1824): b0, the
+// wavenumber ku, the polarisation cos/sin and the length L in lengths.
+struct UndDev {
+  double k, lu, rb, dist, theta;
+  unsigned length;
+  double b0, ku, ct, st, L;
+};
+
 struct PushConst {
   // mesh
   double xmin, xmax, ymin, ymax, zmin, zp0, zp1, dx, dy, dz, Lz;
@@ -160,6 +169,7 @@ struct Domain {
   int pk_lo, pk_hi;         // power planes' cells (pk_lo > pk_hi: none)
   int zero_pending;         // the A^{n+1} buffer is logically zero (current_reset), not yet written
   int yfaces_pending;       // Mur y faces the last interior launch updated (yface_mask bits)
+  double und_dist0;         // undulator[0].dist of the uploaded table (the emission flag's entry test)
   int sort_interval;        // deposits between reorders of the particle arrays (0: never)
   long sort_count;          // deposits since the particles were set
   unsigned long long* counters;   // [0] transverse escapes, [1] z-boundary crossings
@@ -216,7 +226,7 @@ struct BatchView {
   int reorder;              // steps between reorders of the particle arrays (0: never)
   long cap, np;
   PushConst push;           // tb0 is taken from time_bunch each step
-  const mithra_undulator* und;
+  const UndDev* und;
   unsigned long long* counters;
   int n_sub;
   double dtb;
@@ -262,7 +272,7 @@ constexpr int kTileVariants = 10;  // 7..9: LDS-staged interior update
 extern const int kTileShapes[kTileVariants][2];
 hipError_t launch_fill_zero(Domain* d, void* ptr, size_t bytes);
 hipError_t launch_bunch_begin(Domain* d);
-hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const UndDev* und, int n_und,
                        double time_bunch, double bunch_time_step, int n_sub, int save_rnm);
 // Current deposition into the A^{n+1} buffer: accumulate = 0 replaces its contents with the
 // deposit, 1 adds the deposit to them.
@@ -305,7 +315,8 @@ DepositConst make_deposit_const(const Domain* d);
 
 // The C-ABI handle of one domain (include/mithra_gpu.h).
 struct mithra_dev : mithra::Domain {
-  mithra_undulator* und_dev = nullptr;      // device copy of the undulator table
+  mithra::UndDev* und_dev = nullptr;                // device copy of the undulator table
+  double und_c0 = 0.0;                      // the speed of light its b0 were computed with
   int und_cap = 0;
   std::vector<mithra_undulator> und_host;   // what und_dev holds
   hipEvent_t ev[MITHRA_STEP_PHASES + 1] = {};  // phase timers of mithra_dev_step_timed
@@ -313,6 +324,6 @@ struct mithra_dev : mithra::Domain {
 };
 
 namespace mithra {
-int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n);  // capi.hip
+int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n, double c0);  // capi.hip
 int settle(mithra_dev* d);  // writes the zeros current_reset left pending
 }  // namespace mithra

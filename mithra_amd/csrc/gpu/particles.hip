@@ -21,13 +21,10 @@ template <typename UT>
 __device__ void undulator_field(const PushConst& c, UT und, double x, double y, double z, double tb, double et[3],
                                 double bt[3]) {
   for (int u = 0; u < c.n_und; ++u) {
-    const mithra_undulator U = *(const mithra_undulator*)(und + u);
-    const double b0 = (U.lu != 0.0) ? kEM * c.c0 * 2 * kPI / U.lu * U.k / kEC : 0.0;
-    const double ku = (U.lu != 0.0) ? 2 * kPI / U.lu : 0.0;
-    const double ct = cos(U.theta), st = sin(U.theta);
+    const UndDev U = *(const UndDev*)(und + u);
+    const double b0 = U.b0, ku = U.ku, ct = U.ct, st = U.st, L = U.L;
     const double lz = c.gamma * (z + c.beta * c.c0 * (tb + c.dt_shift)) - U.rb;
     const double ly = x * ct + y * st;
-    const double L = U.length * U.lu;
     double d1, bz;
     if (lz >= 0.0 && lz <= L) {
       d1 = b0 * cosh(ku * ly) * sin(ku * lz) * c.gamma;
@@ -36,13 +33,13 @@ __device__ void undulator_field(const PushConst& c, UT und, double x, double y, 
       const double s = (lz < 0.0) ? lz : lz - L;     // distance into the fringe
       double sz = exp(-pow(ku * s, 2) / 2.0);
       if (lz < 0.0 && u != 0) {
-        const mithra_undulator V = *(const mithra_undulator*)(und + (u - 1));
-        const double r0 = V.rb + V.length * V.lu - U.rb;
+        const UndDev V = *(const UndDev*)(und + (u - 1));
+        const double r0 = V.rb + V.L - U.rb;
         if (lz < r0 || r0 == 0.0) sz = 0.0;
         else sz *= 0.35875 + 0.48829 * cos(kPI * lz / r0) + 0.14128 * cos(2.0 * kPI * lz / r0) +
                    0.01168 * cos(3.0 * kPI * lz / r0);
       } else if (lz > L && u + 1 != c.n_und) {
-        const mithra_undulator V = *(const mithra_undulator*)(und + (u + 1));
+        const UndDev V = *(const UndDev*)(und + (u + 1));
         const double r0 = V.rb - U.rb - L;
         if (s > r0 || r0 == 0.0) sz = 0.0;
         else sz *= 0.35875 + 0.48829 * cos(kPI * s / r0) + 0.14128 * cos(2.0 * kPI * s / r0) +
@@ -66,12 +63,12 @@ __device__ void undulator_field(const PushConst& c, UT und, double x, double y, 
 // step and advanced n_sub times in registers; the time of each sub-step is accumulated exactly
 // as Solver::solve() accumulates timeBunch_ (src/solver.cpp:1316-1321).
 __device__ inline void push_body(double* __restrict__ part, long cap, long n, const float* __restrict__ eb,
-                                 const mithra_undulator* __restrict__ und, const PushConst& c,
+                                 const UndDev* __restrict__ und, const PushConst& c,
                                  unsigned long long* __restrict__ counters, long p) {
   if (p >= n) return;
   const gptr<double> X = G(part);
   const gptr<const f2> ebg = G(reinterpret_cast<const f2*>(eb));
-  const gptr<const mithra_undulator> undg = G(und);
+  const gptr<const UndDev> undg = G(und);
   double x = X[PX * cap + p], y = X[PY * cap + p], z = X[PZ * cap + p];
   double gx = X[PGX * cap + p], gy = X[PGY * cap + p], gz = X[PGZ * cap + p];
   double e = X[PE * cap + p];
@@ -180,7 +177,7 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
 }
 
 __global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long cap, long n,
-                                              const float* __restrict__ eb, const mithra_undulator* __restrict__ und,
+                                              const float* __restrict__ eb, const UndDev* __restrict__ und,
                                               PushConst c, unsigned long long* __restrict__ counters) {
   push_body(part, cap, n, eb, und, c, counters, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
@@ -933,7 +930,7 @@ hipError_t launch_bunch_begin(Domain* d) {
   return hipGetLastError();
 }
 
-hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const UndDev* und, int n_und,
                        double time_bunch, double bunch_time_step, int n_sub, int save_rnm) {
   if (save_rnm) d->rm_order_valid = d->order1_is_rnp;  // rnm := rnp
   if (n_sub > 0) d->order1_is_rnp = 0;                  // rnp moves
@@ -948,9 +945,9 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const mithra_und
   c.r1 = p->r1; c.r2 = p->r2; c.dtb = p->dtb;
   c.tb0 = time_bunch; c.dtb_time = bunch_time_step;
   c.n_sub = n_sub; c.n_und = n_und; c.save_rnm = save_rnm;
-  c.dist0 = n_und > 0 ? und[0].dist : 0.0;
+  c.dist0 = n_und > 0 ? d->und_dist0 : 0.0;  // kept at upload: `und` is device memory
   hipLaunchKernelGGL(k_push, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap, d->np,
-                     d->eb, (const mithra_undulator*)und, c, d->counters);
+                     d->eb, und, c, d->counters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_zrange(d);

@@ -5,6 +5,7 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cstring>
 #include <iostream>
 #include <sstream>
@@ -28,7 +29,8 @@ void createDirectory(const std::string& filename) {  // src/stdinclude.cpp:22-35
   std::string path = filename.substr(0, found + 1);
   struct stat st;
   if (path.empty() || stat(path.c_str(), &st) == 0) return;
-  if (mkdir(path.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH) == -1)
+  // EEXIST: another job set up at the same time created it between the test and here
+  if (mkdir(path.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH) == -1 && errno != EEXIST)
     throw JobError("Could not create the directory " + path + ". Probably the given address does not exist.");
 }
 
