@@ -590,7 +590,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
 }
 
 template <typename R, bool SC>
-__global__ __launch_bounds__(kDepositThreads) void k_deposit_rows(
+__global__ __launch_bounds__(kDepositThreads, SC ? 4 : 8) void k_deposit_rows(
     const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
     const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
     int TI, int tilesI, int KC, int accumulate) {
@@ -975,11 +975,12 @@ hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows) {
 }
 
 // Rows per workgroup tile of the row-tiled deposit: its two-plane LDS ring
-// (2 x ncomp x TI x N1 doubles) stays within 48 KiB (three workgroups per CU); 0 when even one
-// row does not fit.
+// (2 x ncomp x TI x N1 doubles) stays within 40 KiB, so four 512-thread workgroups (eight
+// waves per SIMD, the kernel's register budget) share a CU's 160 KiB; 0 when even one row does
+// not fit.
 int tile_rows(const Domain* d) {
   const long per_row = 2L * d->ncomp * d->g.N1 * (long)sizeof(double);
-  const long fit = (48L << 10) / per_row;
+  const long fit = (40L << 10) / per_row;
   return (int)std::min(fit, 16L);
 }
 
