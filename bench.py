@@ -169,7 +169,11 @@ class March:
         self.n += 1
 
 
-def run_gpu(job, steps, warmup, precision, reorder=16):
+def run_gpu(job, steps, warmup, precision, reorder=16, pipelined=True):
+    """Warm-up, then `steps` timed steps, then a few steps with per-phase HIP events.  Pipelined:
+    each group of steps is one complete pipelined run (mithra_dev_step_pipelined, more = 1 on all
+    but its last step), so the timed region starts and ends in the reference's between-step state
+    and includes the run's first interior update and last separate E/B pass."""
     dom = open_domain(job, precision)
     dom.set_lazy_fields(1)   # E/B where the bunch and the sampler read them (the reference's rule)
     dom.set_reorder_interval(reorder)
@@ -178,26 +182,41 @@ def run_gpu(job, steps, warmup, precision, reorder=16):
         f"{abi.tile_shape(v)[0]}x{abi.tile_shape(v)[1]} interior {tune_ms[0, v]:.3f} eval {tune_ms[1, v]:.3f}"
         for v in range(abi.TILE_VARIANTS)))
     tiles = {"interior": list(abi.tile_shape(ci)), "eval": list(abi.tile_shape(ce))}
+    if pipelined:
+        cf, fused_ms = dom.autotune_fused(reps=2)
+        log("autotune of the fused stencil + E/B kernel (ms per launch): " + "; ".join(
+            f"{abi.fused_tile_shape(v)[0]}x{abi.fused_tile_shape(v)[1]} {fused_ms[v]:.3f}"
+            for v in range(abi.FUSED_TILE_VARIANTS)))
+        tiles["fused"] = list(abi.fused_tile_shape(cf))
     m = March(job)
     n_sub, dtb = m.t["n_sub"], m.t["bunch_time_step"]
     rows = 64
-    for _ in range(warmup):
-        dom.step(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows)
+
+    def step(more, timed=False):
+        r = None
+        if pipelined:
+            r = dom.step_pipelined(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows,
+                                   more=more, timed=timed)
+        elif timed:
+            r = dom.step_timed(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows)
+        else:
+            dom.step(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows)
         m.advance()
+        return r
+
+    for i in range(warmup):
+        step(i < warmup - 1)
     dom.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        dom.step(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows)
-        m.advance()
+    for i in range(steps):
+        step(i < steps - 1)
     dom.synchronize()
     elapsed = time.perf_counter() - t0
-    # per-phase device times, HIP events on the domain's stream, over further real steps
-    phases = np.zeros(len(abi.PHASES))
-    n_timed = max(3, min(steps, 10))
-    for _ in range(n_timed):
-        phases += np.array(dom.step_timed(m.time, m.seed, m.bp, m.und, m.tb, dtb, n_sub, m.n, m.n % rows))
-        m.advance()
-    phases /= n_timed
+    # per-phase device times, HIP events on the domain's stream, over a further run of real
+    # steps; pipelined, its first and last steps (interior update, separate E/B) are left out
+    n_timed = max(4, min(steps, 10))
+    rs = np.array([step(i < n_timed - 1, timed=True) for i in range(n_timed)])
+    phases = (rs[1:-1] if pipelined else rs).mean(axis=0)
     counters = dom.bunch_counters()
     P = dom.power_fetch(0, min(rows, warmup + steps), 1, 1)[:, 0, 0]
     dom.close()
@@ -267,11 +286,11 @@ def run_cpu_baseline(job, steps, threads):
     return elapsed, used
 
 
-def traffic_from_profiles(precision):
-    """HBM bytes per launch of the interior stencil from the committed rocprofv3 --pmc
+def traffic_from_profiles(kernel, precision):
+    """HBM bytes per launch of `kernel` (interior, stencil_eb) from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on the stream probe),
     when present."""
-    path = os.path.join(ROOT, "profiles", f"interior_production_{precision}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_production_{precision}_traffic.md")
     if os.path.exists(path):
         with open(path) as f:
             for line in f:
@@ -290,6 +309,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--reorder", type=int, default=16,
                     help="deposits between reorders of the particle arrays into cell-row order (0: never)")
+    ap.add_argument("--plain", action="store_true",
+                    help="time the plain step (mithra_dev_step) instead of the pipelined one")
     ap.add_argument("--scan", type=int, default=64,
                     help="instances of the batched parameter scan measured for scan points/hour (0 skips it)")
     args = ap.parse_args()
@@ -308,14 +329,23 @@ def main():
         n_part = job.particles().shape[0]
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
-        elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision, args.reorder)
+        pipelined = not args.plain
+        elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision, args.reorder,
+                                                      pipelined)
         log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
         ms_step = 1e3 * elapsed / args.steps
         cell_rate = nodes * args.steps / elapsed
         push_rate = n_part * n_sub * args.steps / elapsed
-        bytes_per_node = 96 if precision == abi.PRECISION_FP64 else 48
-        t_int = phases[abi.PHASES.index("interior")] * 1e-3
-        achieved = bytes_per_node * interior / t_int / 1e9
+        # the dominant field kernel and its algorithmic bytes per interior node
+        real = 8 if precision == abi.PRECISION_FP64 else 4
+        if pipelined:   # reads A^{n+1}, A^n; writes X and E/B (6 floats)
+            kernel, phase, bytes_per_node = "stencil_eb", "eval_eb", 9 * real + 24
+            kernel_name = "k_stencil_eb (E/B fused with the next NSFD update's stencil)"
+        else:           # reads A^n, A^{n-1}, J; writes A^{n+1}
+            kernel, phase, bytes_per_node = "interior", "interior", 12 * real
+            kernel_name = "k_interior (NSFD stencil)"
+        t_k = phases[abi.PHASES.index(phase)] * 1e-3
+        achieved = bytes_per_node * interior / t_k / 1e9
         scan_out = run_scan(args.scan, work) if args.scan > 0 else None
         copy_gbs = abi.copy_bandwidth()
         stream_gbs = abi.stream_bandwidth()
@@ -346,16 +376,17 @@ def main():
                                "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)",
                    "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
                    "precision": args.precision, "parallelism": "single GPU, whole domain resident",
-                   "reorder_interval": args.reorder},
+                   "reorder_interval": args.reorder,
+                   "step": "pipelined (E/B fused with the next update's stencil)" if pipelined else "plain"},
         "particle_pushes_per_s": push_rate,
         "scan_points_per_hour": scan_out["points_per_hour_batched"] if scan_out else None,
         "scan": scan_out,
         "phase_ms": {k: round(float(v), 4) for k, v in zip(abi.PHASES, phases)},
-        "roofline": {"bound": "hbm", "kernel": "k_interior (NSFD stencil)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kernel_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "frac_of_measured_stream": achieved / stream_gbs, "measured_stream_3r1w_GBps": stream_gbs,
                      "measured_copy_GBps": copy_gbs,
-                     "traffic": traffic_from_profiles(args.precision),
+                     "traffic": traffic_from_profiles(kernel, args.precision),
                      "algorithmic_bytes_per_launch": bytes_per_node * interior},
         "cpu_baseline": cpu,
         "setup_s": t_setup,

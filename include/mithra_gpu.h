@@ -326,6 +326,37 @@ int mithra_dev_step_timed(mithra_dev* dev, double time, const mithra_seed* seed,
                           const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
                           int n_sub, unsigned n_time, int power_row, float* ms);
 
+/* mithra_dev_step pipelined across calls.  The interior update sums its terms left to right
+ * with the source a4 J last (src/database.cpp:44-134), so A^{n+2} = X + a4 J^{n+1} splits
+ * exactly, and X reads the same two levels (A^{n+1}, A^n) as the E/B evaluation of step n.
+ * more = 1: after the seed injection and Mur boundaries of A^{n+1}, one kernel evaluates E/B and
+ * writes X into the free level; the deposition then writes X + a4 J at the interior nodes
+ * instead of J, so the next call starts from the marched A^{n+2} and skips its interior update.
+ * Per node and step this saves one read each of A^n, A^{n+1} and J (72 B in FP64) for one read
+ * of X (24 B).  more = 0 ends the run: that step evaluates E/B alone and deposits J, leaving
+ * exactly the state mithra_dev_step leaves.  While a run is pending the potentials are not in
+ * the reference's between-step state (J is not held), so mithra_dev_get/set_potentials,
+ * field_update, field_evaluate, field_shift, current_reset, current_update, autotune and
+ * batch_add return MITHRA_E_STATE; particles, E/B, power samples and counters stay readable.
+ * Pipelining needs FP64, no space charge, particles and the row-tiled deposit; otherwise
+ * more = 1 runs as more = 0 (and nothing is pending).  Its results are bit-identical to
+ * mithra_dev_step's (the deposit's summation order aside).  ms: NULL, or MITHRA_STEP_PHASES durations as
+ * mithra_dev_step_timed gives them (phase 0 is zero after a pipelined step; phase 3 is the
+ * fused kernel when more = 1).  No counterpart in the reference: its loop body reordered. */
+int mithra_dev_step_pipelined(mithra_dev* dev, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+                              const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
+                              int n_sub, unsigned n_time, int power_row, int more, float* ms);
+/* 1 while a pipelined run is pending (the last step had more = 1 and was pipelined), else 0. */
+int mithra_dev_pipeline_pending(const mithra_dev* dev);
+/* Tile shapes of the pipelined step's fused kernel: `rows` x-rows of 64 y-nodes, `planes`
+ * z-planes per workgroup; every shape computes identical values.  autotune_fused times each
+ * (reps launches after one warm-up, ms[v] the mean) on the handle's current levels, keeps the
+ * fastest and restores the level and the E/B array it writes.  No counterpart in the reference. */
+#define MITHRA_FUSED_TILE_VARIANTS 4
+int mithra_fused_tile_shape(int variant, int* rows, int* planes);
+int mithra_dev_set_fused_tile(mithra_dev* dev, int variant);
+int mithra_dev_autotune_fused(mithra_dev* dev, int reps, int* chosen, float* ms);
+
 /* Device-to-device copy rate of a plain streaming kernel over two buffers of `bytes` each
  * (read + write bytes per second, GB/s), the measured HBM ceiling the roofline fractions are
  * also reported against.  No counterpart in the reference. */

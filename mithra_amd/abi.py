@@ -77,6 +77,7 @@ PHASES = ("interior", "tfsf", "boundaries", "eval_eb", "push", "power", "current
 
 
 TILE_VARIANTS = 10
+FUSED_TILE_VARIANTS = 4   # shapes of the pipelined step's fused kernel (MITHRA_FUSED_TILE_VARIANTS)
 
 
 def tile_shape(variant):
@@ -101,6 +102,17 @@ def copy_bandwidth(nbytes=2 << 30, reps=10, elem_bytes=32):
     if rc != 0:
         raise MithraError(f"mithra_copy_bandwidth: status {rc}")
     return out.value
+
+
+def fused_tile_shape(variant):
+    """(rows, planes) of a tile variant of the pipelined step's fused kernel."""
+    lib = load_gpu_library()
+    f = lib.mithra_fused_tile_shape
+    f.restype, f.argtypes = C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    r, k = C.c_int(), C.c_int()
+    if f(variant, C.byref(r), C.byref(k)) != 0:
+        raise MithraError(f"no fused tile variant {variant}")
+    return r.value, k.value
 
 
 def stream_bandwidth(nbytes=2 << 30, reps=10):
@@ -155,6 +167,13 @@ def _declare(lib, prefix):
         sig["step"] = (C.c_int, [vp, C.c_double, C.POINTER(Seed), C.POINTER(BunchParams),
                                  C.POINTER(Undulator), C.c_int, C.c_double, C.c_double, C.c_int,
                                  C.c_uint, C.c_int])
+        sig["step_pipelined"] = (C.c_int, [vp, C.c_double, C.POINTER(Seed), C.POINTER(BunchParams),
+                                           C.POINTER(Undulator), C.c_int, C.c_double, C.c_double,
+                                           C.c_int, C.c_uint, C.c_int, C.c_int,
+                                           C.POINTER(C.c_float)])
+        sig["pipeline_pending"] = (C.c_int, [vp])
+        sig["set_fused_tile"] = (C.c_int, [vp, C.c_int])
+        sig["autotune_fused"] = (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)])
     fns = {}
     for name, (res, args) in sig.items():
         f = getattr(lib, prefix + name)
@@ -307,6 +326,40 @@ class Domain:
                                          C.byref(params), arr, len(undulators), time_bunch,
                                          bunch_time_step, n_sub, n_time, power_row, ms))
         return list(ms)
+
+    def step_pipelined(self, time, seed, params, undulators, time_bunch, bunch_time_step, n_sub,
+                       n_time, power_row=-1, more=True, timed=False):
+        """step() pipelined across calls (device library only): with more=True the E/B pass is
+        fused with the next interior update and the deposit adds its J to that update, so the
+        next call skips the interior update; the potentials are unavailable until a call with
+        more=False ends the run.  FP64 without space charge only (elsewhere more=True runs the
+        plain step).  Returns the phase times in ms when timed, else None."""
+        arr = (Undulator * max(1, len(undulators)))(*undulators)
+        ms = (C.c_float * 8)() if timed else None
+        self._check(self.f["step_pipelined"](self.h, time, None if seed is None else C.byref(seed),
+                                             C.byref(params), arr, len(undulators), time_bunch,
+                                             bunch_time_step, n_sub, n_time, power_row, int(more),
+                                             ms))
+        return list(ms) if timed else None
+
+    def pipeline_pending(self):
+        """True while a pipelined run awaits its last (more=False) step."""
+        r = self.f["pipeline_pending"](self.h)
+        if r < 0:
+            self._check(r)
+        return bool(r)
+
+    def set_fused_tile(self, variant):
+        """Tile shape variant of the pipelined step's fused kernel."""
+        self._check(self.f["set_fused_tile"](self.h, variant))
+
+    def autotune_fused(self, reps=3):
+        """Times every tile shape of the pipelined step's fused kernel and keeps the fastest.
+        Returns (chosen, ms[variants])."""
+        chosen = C.c_int()
+        ms = (C.c_float * FUSED_TILE_VARIANTS)()
+        self._check(self.f["autotune_fused"](self.h, reps, C.byref(chosen), ms))
+        return chosen.value, np.array(ms[:])
 
     def synchronize(self):
         self._check(self.f["synchronize"](self.h))

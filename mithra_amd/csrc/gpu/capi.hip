@@ -76,6 +76,7 @@ namespace {
 }  // namespace
 
 int mithra::settle(mithra_dev* d) {
+  if (d->pipe) return MITHRA_E_STATE;  // mid-way through a pipelined run: J is not held
   if (!d->zero_pending) return MITHRA_OK;
   d->zero_pending = 0;
   for (int c = 0; c < d->ncomp; ++c)
@@ -84,6 +85,66 @@ int mithra::settle(mithra_dev* d) {
 }
 
 namespace {
+
+// One iteration of Solver::solve()'s loop (src/solver.cpp:1303-1399) on the device: the interior
+// update (skipped when the previous pipelined step already marched A^{n+1}), seed injection, Mur
+// boundaries, E/B (fused with the next update's stencil when `more` keeps the run pipelined),
+// bunch sub-steps, power sample, field shift and deposition (of J, or, pipelined, of X + a4 J).
+// ms (optional) receives the phase durations of mithra_dev_step_timed.
+int run_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+             const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step, int n_sub,
+             unsigned n_time, int power_row, int more, float* ms) {
+  if (!p || n_sub < 0) return MITHRA_E_ARG;
+  const bool seeded = seed && std::fabs(seed->amplitude) > 1.0e-50;
+  if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
+      seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
+    return MITHRA_E_UNSUPPORTED;
+  if (power_row >= 0 && (!d->prows || (unsigned)power_row >= d->prows)) return MITHRA_E_STATE;
+  int rc = upload_undulators(d, und, n_und, p->c0);
+  if (rc) return rc;
+  const bool pipe_in = d->pipe != 0;
+  if (!pipe_in && (rc = settle(d))) return rc;
+  // FP64 only: in FP32 the LDS-staged field kernels run at half the speed of the register
+  // forms, and the fused kernel stages in LDS
+  const bool pipe_out = more && d->precision == MITHRA_PRECISION_FP64 && d->ncomp == 3 && d->np > 0 &&
+                        deposit_tiled(d);
+  if (ms && !d->ev_ready) {
+    for (auto& e : d->ev) MITHRA_TRY(hipEventCreate(&e));
+    d->ev_ready = true;
+  }
+  hipStream_t s = d->stream;
+  auto mark = [&](int i) { return ms ? hipEventRecord(d->ev[i], s) : hipSuccess; };
+  const bool tfsf = seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4;
+  MITHRA_TRY(mark(0));
+  if (!pipe_in) MITHRA_TRY(launch_interior(d, !tfsf));
+  MITHRA_TRY(mark(1));
+  if (tfsf) MITHRA_TRY(launch_tfsf(d, time, seed));
+  MITHRA_TRY(mark(2));
+  MITHRA_TRY(launch_boundaries(d));
+  MITHRA_TRY(mark(3));
+  MITHRA_TRY(pipe_out ? launch_stencil_eb(d) : launch_eval_eb_step(d));
+  MITHRA_TRY(mark(4));
+  MITHRA_TRY(launch_push(d, p, d->und_dev, n_und, time_bunch, bunch_time_step, n_sub, 1));
+  MITHRA_TRY(mark(5));
+  if (power_row >= 0) MITHRA_TRY(launch_power(d, n_time, (unsigned)power_row));
+  MITHRA_TRY(mark(6));
+  const int t = d->i_nm1;  // field shift; the free level then holds X when pipelined
+  d->i_nm1 = d->i_n;
+  d->i_n = d->i_np1;
+  d->i_np1 = t;
+  d->zero_pending = 0;
+  MITHRA_TRY(mark(7));
+  const hipError_t e = launch_deposit(d, pipe_out ? kDepositSource : 0);
+  if (e == hipErrorOutOfMemory) return MITHRA_E_NOMEM;
+  MITHRA_TRY(e);
+  d->pipe = pipe_out ? 1 : 0;
+  MITHRA_TRY(mark(8));
+  if (ms) {
+    MITHRA_TRY(hipEventSynchronize(d->ev[8]));
+    for (int i = 0; i < MITHRA_STEP_PHASES; ++i) MITHRA_TRY(hipEventElapsedTime(&ms[i], d->ev[i], d->ev[i + 1]));
+  }
+  return MITHRA_OK;
+}
 
 int level_of(const mithra_dev* d, int level) {
   return level == 0 ? d->i_n : level == 1 ? d->i_nm1 : level == 2 ? d->i_np1 : -1;
@@ -451,12 +512,14 @@ int mithra_dev_field_shift(mithra_dev* d) {
 
 int mithra_dev_current_reset(mithra_dev* d) {
   if (!d) return MITHRA_E_ARG;
+  if (d->pipe) return MITHRA_E_STATE;
   d->zero_pending = 1;
   return MITHRA_OK;
 }
 
 int mithra_dev_current_update(mithra_dev* d) {
   if (!d) return MITHRA_E_ARG;
+  if (d->pipe) return MITHRA_E_STATE;
   const int accumulate = d->zero_pending ? 0 : 1;
   d->zero_pending = 0;
   const hipError_t e = launch_deposit(d, accumulate);
@@ -480,14 +543,18 @@ int mithra_dev_set_reorder_interval(mithra_dev* d, int interval) {
 int mithra_dev_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
                     const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step, int n_sub,
                     unsigned n_time, int power_row) {
-  int rc = mithra_dev_field_update(d, time, seed);
-  if (!rc) rc = mithra_dev_bunch_advance(d, p, und, n_und, time_bunch, bunch_time_step, n_sub);
-  if (!rc && power_row >= 0) rc = mithra_dev_power_sample(d, n_time, (unsigned)power_row);
-  if (!rc) rc = mithra_dev_field_shift(d);
-  if (!rc) rc = mithra_dev_current_reset(d);
-  if (!rc) rc = mithra_dev_current_update(d);
-  return rc;
+  if (!d) return MITHRA_E_ARG;
+  return run_step(d, time, seed, p, und, n_und, time_bunch, bunch_time_step, n_sub, n_time, power_row, 0, nullptr);
 }
+
+int mithra_dev_step_pipelined(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
+                              const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step,
+                              int n_sub, unsigned n_time, int power_row, int more, float* ms) {
+  if (!d || (more != 0 && more != 1)) return MITHRA_E_ARG;
+  return run_step(d, time, seed, p, und, n_und, time_bunch, bunch_time_step, n_sub, n_time, power_row, more, ms);
+}
+
+int mithra_dev_pipeline_pending(const mithra_dev* d) { return d ? d->pipe : MITHRA_E_ARG; }
 
 int mithra_tile_shape(int variant, int* rows, int* planes) {
   if (variant < 0 || variant >= kTileVariants) return MITHRA_E_ARG;
@@ -586,46 +653,71 @@ int mithra_dev_bunching(mithra_dev* d, double k, double* b_out) {
 int mithra_dev_step_timed(mithra_dev* d, double time, const mithra_seed* seed, const mithra_bunch_params* p,
                           const mithra_undulator* und, int n_und, double time_bunch, double bunch_time_step, int n_sub,
                           unsigned n_time, int power_row, float* ms) {
-  if (!d || !p || !ms) return MITHRA_E_ARG;
-  if (!d->ev_ready) {
-    for (auto& e : d->ev) MITHRA_TRY(hipEventCreate(&e));
-    d->ev_ready = true;
-  }
-  const bool seeded = seed && std::fabs(seed->amplitude) > 1.0e-50;
-  if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
-      seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
-    return MITHRA_E_UNSUPPORTED;
-  int rc = upload_undulators(d, und, n_und, p->c0);
-  if (!rc) rc = settle(d);
-  if (rc) return rc;
-  hipStream_t s = d->stream;
-  const bool tfsf = seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4;
-  MITHRA_TRY(hipEventRecord(d->ev[0], s));
-  MITHRA_TRY(launch_interior(d, !tfsf));
-  MITHRA_TRY(hipEventRecord(d->ev[1], s));
-  if (tfsf) MITHRA_TRY(launch_tfsf(d, time, seed));
-  MITHRA_TRY(hipEventRecord(d->ev[2], s));
-  MITHRA_TRY(launch_boundaries(d));
-  MITHRA_TRY(hipEventRecord(d->ev[3], s));
-  MITHRA_TRY(launch_eval_eb_step(d));
-  MITHRA_TRY(hipEventRecord(d->ev[4], s));
-  MITHRA_TRY(launch_push(d, p, d->und_dev, n_und, time_bunch, bunch_time_step, n_sub, 1));
-  MITHRA_TRY(hipEventRecord(d->ev[5], s));
-  if (power_row >= 0) {
-    if (!d->prows || (unsigned)power_row >= d->prows) return MITHRA_E_STATE;
-    MITHRA_TRY(launch_power(d, n_time, (unsigned)power_row));
-  }
-  MITHRA_TRY(hipEventRecord(d->ev[6], s));
-  mithra_dev_field_shift(d);
-  rc = mithra_dev_current_reset(d);
-  if (rc) return rc;
-  MITHRA_TRY(hipEventRecord(d->ev[7], s));
-  rc = mithra_dev_current_update(d);
-  if (rc) return rc;
-  MITHRA_TRY(hipEventRecord(d->ev[8], s));
-  MITHRA_TRY(hipEventSynchronize(d->ev[8]));
-  for (int i = 0; i < MITHRA_STEP_PHASES; ++i) MITHRA_TRY(hipEventElapsedTime(&ms[i], d->ev[i], d->ev[i + 1]));
+  if (!d || !ms) return MITHRA_E_ARG;
+  return run_step(d, time, seed, p, und, n_und, time_bunch, bunch_time_step, n_sub, n_time, power_row, 0, ms);
+}
+
+int mithra_dev_set_fused_tile(mithra_dev* d, int variant) {
+  if (!d || variant < 0 || variant >= kFusedVariants) return MITHRA_E_ARG;
+  d->tile_fused = variant;
   return MITHRA_OK;
+}
+
+int mithra_fused_tile_shape(int variant, int* rows, int* planes) {
+  if (variant < 0 || variant >= kFusedVariants) return MITHRA_E_ARG;
+  if (rows) *rows = kFusedShapes[variant][0];
+  if (planes) *planes = kFusedShapes[variant][1];
+  return MITHRA_OK;
+}
+
+int mithra_dev_autotune_fused(mithra_dev* d, int reps, int* chosen, float* ms) {
+  if (!d || reps < 1) return MITHRA_E_ARG;
+  if (d->ncomp != 3) return MITHRA_E_UNSUPPORTED;
+  if (int rc = settle(d)) return rc;
+  hipStream_t s = d->stream;
+  // the kernel writes its X into the A^{n-1} level and E/B into the field array: time it there
+  // and restore both after
+  const size_t level_bytes = (size_t)d->ncomp * ((d->nodes * d->real_size + 255) / 256 * 256);
+  const size_t eb_bytes = sizeof(float) * 2 * kEBPairs * (size_t)d->nodes;
+  char* saved = nullptr;
+  if (hipMalloc(&saved, level_bytes + eb_bytes) != hipSuccess) return MITHRA_E_NOMEM;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_status(hipEventCreate(&e0));
+  if (!rc) rc = hip_status(hipEventCreate(&e1));
+  void* nm1 = d->lvl[d->i_nm1].c[0];  // the components of one level are contiguous
+  if (!rc) rc = hip_status(hipMemcpyAsync(saved, nm1, level_bytes, hipMemcpyDeviceToDevice, s));
+  if (!rc) rc = hip_status(hipMemcpyAsync(saved + level_bytes, d->eb, eb_bytes, hipMemcpyDeviceToDevice, s));
+  const int keep = d->tile_fused;
+  int best = keep;
+  float best_ms = 0.0f;
+  for (int v = 0; v < kFusedVariants && !rc; ++v) {
+    d->tile_fused = v;
+    float total = 0.0f;
+    for (int r = 0; r <= reps && !rc; ++r) {  // r = 0 warms up
+      rc = hip_status(hipEventRecord(e0, s));
+      if (!rc) rc = hip_status(launch_stencil_eb(d));
+      if (!rc) rc = hip_status(hipEventRecord(e1, s));
+      if (!rc) rc = hip_status(hipEventSynchronize(e1));
+      float t = 0.0f;
+      if (!rc) rc = hip_status(hipEventElapsedTime(&t, e0, e1));
+      if (r > 0) total += t;
+    }
+    if (!rc && (v == 0 || total < best_ms)) {
+      best_ms = total;
+      best = v;
+    }
+    if (ms) ms[v] = total / reps;
+  }
+  hipError_t e = hipMemcpyAsync(nm1, saved, level_bytes, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d->eb, saved + level_bytes, eb_bytes, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (!rc && e != hipSuccess) rc = hip_status(e);
+  (void)hipFree(saved);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  d->tile_fused = rc ? keep : best;
+  if (!rc && chosen) *chosen = best;
+  return rc;
 }
 
 }  // extern "C"
@@ -725,3 +817,5 @@ int mithra_copy_bandwidth_elem(unsigned long bytes, int reps, int elem_bytes, do
 }
 
 }  // extern "C"
+
+static_assert(MITHRA_FUSED_TILE_VARIANTS == mithra::kFusedVariants, "header and library disagree on the fused shapes");

@@ -160,6 +160,10 @@ struct Domain {
   unsigned* pid_alt;
   int tile_interior;        // tile shape of the interior update (index into kTileShapes)
   int tile_eval;            // tile shape of the E/B evaluation
+  int tile_fused;           // tile shape of the pipelined stencil + E/B kernel (kFusedShapes)
+  // pipelined step: the A^{n+1} buffer holds the marched A^{n+1} (interior and source) awaiting
+  // its seed injection and boundaries, instead of J; J itself was never stored
+  int pipe;
   // lazy E/B evaluation: only the planes the bunch (z cells zrange[0..1], device) and the power
   // sampler (planes pk_lo .. pk_hi + 1) read
   int lazy_fields;
@@ -266,6 +270,11 @@ hipError_t launch_boundaries(Domain* d);
 hipError_t launch_eval_eb(Domain* d);
 hipError_t launch_eval_eb_range(Domain* d, int klo, int khi);  // planes klo..khi (clipped to 1..N2-2)
 hipError_t launch_eval_eb_step(Domain* d);  // all planes, or the lazy subset when enabled and known
+// Pipelined step: E/B of step n and X, the next interior update without its source term, into
+// the A^{n-1} buffer (no space charge).  Lazy E/B as launch_eval_eb_step.
+hipError_t launch_stencil_eb(Domain* d);
+constexpr int kFusedVariants = 4;
+extern const int kFusedShapes[kFusedVariants][2];
 hipError_t launch_zrange(Domain* d);
 // Tile shapes {rows, planes} of the marched field kernels, selected by Domain::tile_*.
 constexpr int kTileVariants = 10;  // 7..9: LDS-staged interior update
@@ -275,7 +284,9 @@ hipError_t launch_bunch_begin(Domain* d);
 hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const UndDev* und, int n_und,
                        double time_bunch, double bunch_time_step, int n_sub, int save_rnm);
 // Current deposition into the A^{n+1} buffer: accumulate = 0 replaces its contents with the
-// deposit, 1 adds the deposit to them.
+// deposit, 1 adds the deposit to them, kDepositSource (row-tiled path only) replaces each
+// interior node's X with X + a4 J (the pipelined step) and leaves the boundary nodes alone.
+constexpr int kDepositSource = 2;
 hipError_t launch_deposit(Domain* d, int accumulate);
 bool deposit_tiled(const Domain* d);
 hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks);

@@ -784,6 +784,202 @@ __global__ __launch_bounds__(kTileJ* TI) void k_eval_lds(FieldSet An, FieldSet A
                               xcd_logical_block(blockIdx.x, gridDim.x));
 }
 
+// ---- pipelined step: E/B of step n fused with the J-free part of the next interior update ----
+//
+// Once A^{n+1} is complete (interior, seed injection and Mur boundaries), the two field passes
+// that remain before the next deposit read the same two levels: the E/B evaluation of step n
+// reads A^n and A^{n+1}, and the interior update of step n+1 reads A^{n+1} (stencil) and A^n
+// (its A^{n-1}).  Only the update's last term, a4 J^{n+1}, waits for the deposit.  The update
+// (src/database.cpp:44-134) sums its terms left to right with the source last, so
+//   A^{n+2} = X + a4 J,  X = a0 A^{n+1} - A^n + (transverse terms) + a3 (z terms)
+// splits exactly: this kernel writes X (into the A^{n-1} buffer, free once the boundaries of
+// A^{n+1} are done) and the E/B of step n; the row-tiled deposit then writes X + a4 J in place
+// of J.  In FP64 the result is the unsplit update's bit for bit; in FP32 X is rounded to float
+// once before the source is added.
+//
+// A workgroup owns TI x-rows x 64 y-nodes of all three components and marches K planes: A^{n+1}
+// with a one-node halo ring in a three-plane LDS ring per component (the stencil's planes k-1,
+// k, k+1), A^n of plane k with its ring in one LDS plane per component (the curl's transverse
+// neighbours), and the z neighbours of A^n_x, A^n_y in a register queue.  Same expressions, in
+// the same order, as k_interior_lds (without the source) and k_eval_lds (without phi).
+struct StencilConst {
+  double a0, a1, a2, a3, alpha, beta;
+};
+#ifndef MITHRA_STENCIL_WAVES
+#define MITHRA_STENCIL_WAVES 4
+#endif
+constexpr int kStencilWaves = MITHRA_STENCIL_WAVES;  // waves per SIMD the register budget allows
+
+template <typename R, bool NSFD, int TI, int K>
+__global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldSet An1, FieldSet An, FieldSet X,
+                                                           float* __restrict__ eb, int N0, int N1, int N2, int tilesJ,
+                                                           int tilesI, StencilConst s, EvalConst q,
+                                                           const int* __restrict__ zr, int pb0, int pb1) {
+  constexpr int W = kTileJ + 2, P = (TI + 2) * W;
+  __shared__ double L1[9 * P];  // A^{n+1}: [component][slot][P]
+  __shared__ double L0[3 * P];  // A^n, plane k: [component][P]
+  int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tj = lb % tilesJ;
+  lb /= tilesJ;
+  const int ti = lb % tilesI;
+  const int kc = lb / tilesI;
+  const int kb = 1 + kc * K;
+  const int ke = min(kb + K, N2 - 1);
+  if (kb >= ke) return;  // uniform over the workgroup
+  const bool ebneed = !zr || lazy_needed(kb, ke, zr, pb0, pb1);
+  const gptr<f2> ebg = G(reinterpret_cast<f2*>(eb));
+  const int lane = threadIdx.x % kTileJ, w = threadIdx.x / kTileJ;
+  const int j0 = tj * kTileJ, i0 = 1 + ti * TI;
+  const int j = j0 + lane, i = i0 + w;
+  const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
+  const long N1N0 = (long)N1 * N0;
+  const gptr<const R> p[3] = {G((const R*)An1.c[0]), G((const R*)An1.c[1]), G((const R*)An1.c[2])};
+  const gptr<const R> n[3] = {G((const R*)An.c[0]), G((const R*)An.c[1]), G((const R*)An.c[2])};
+  const gptr<R> x[3] = {G((R*)X.c[0]), G((R*)X.c[1]), G((R*)X.c[2])};
+  // staged per plane by this thread: its own node, and (as in k_interior_lds) a halo row node
+  // (waves 0 and 1) or a halo column node (lanes < 2 TI of wave 0)
+  const bool own_ok = i <= N0 - 1 && j <= N1 - 1;
+  const long own_off = (long)N1 * i + j;
+  const int own_slot = (w + 1) * W + lane + 1;
+  int hrow_i = -1, hrow_slot = 0;
+  if (w == 0) { hrow_i = i0 - 1; hrow_slot = lane + 1; }
+  if (w == 1) { hrow_i = i0 + TI; hrow_slot = (TI + 1) * W + lane + 1; }
+  const bool hrow_ok = hrow_i >= 0 && hrow_i <= N0 - 1 && j <= N1 - 1;
+  const long hrow_off = (long)N1 * hrow_i + j;
+  int hcol_j = -1, hcol_i = 0, hcol_slot = 0;
+  if (w == 0 && lane < TI) { hcol_i = i0 + lane; hcol_j = j0 - 1; hcol_slot = (lane + 1) * W; }
+  if (w == 0 && lane >= TI && lane < 2 * TI) {
+    hcol_i = i0 + lane - TI; hcol_j = j0 + kTileJ; hcol_slot = (lane - TI + 1) * W + kTileJ + 1;
+  }
+  const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
+  const long hcol_off = (long)N1 * hcol_i + hcol_j;
+  // A^{n+1} of the plane to stage next
+  double p_own[3] = {}, p_row[3] = {}, p_col[3] = {};
+  auto fetch_p = [&](int k) {
+    const long b = N1N0 * k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (own_ok) p_own[c] = (double)p[c][b + own_off];
+      if (hrow_ok) p_row[c] = (double)p[c][b + hrow_off];
+      if (hcol_ok) p_col[c] = (double)p[c][b + hcol_off];
+    }
+  };
+  auto stage_p = [&](int slot) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double* d = L1 + (c * 3 + slot) * P;
+      if (own_ok) d[own_slot] = p_own[c];
+      if (hrow_ok) d[hrow_slot] = p_row[c];
+      if (hcol_ok) d[hcol_slot] = p_col[c];
+    }
+  };
+  // A^n: own A_x, A_y at planes k-1 .. k+2 (qx, qy), own A_z at k and k+1, halo of the next plane
+  double qx[4] = {}, qy[4] = {}, qz = 0.0, qz_next = 0.0, n_row[3] = {}, n_col[3] = {};
+  auto fetch_n_halo = [&](int k) {
+    const long b = N1N0 * k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (hrow_ok) n_row[c] = (double)n[c][b + hrow_off];
+      if (hcol_ok) n_col[c] = (double)n[c][b + hcol_off];
+    }
+  };
+  fetch_p(kb - 1);
+  stage_p(0);
+  fetch_p(kb);
+  stage_p(1);
+  fetch_p(kb + 1);
+  if (own_ok) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      qx[t] = (double)n[0][N1N0 * (kb - 1 + t) + own_off];
+      qy[t] = (double)n[1][N1N0 * (kb - 1 + t) + own_off];
+    }
+    qz = (double)n[2][N1N0 * kb + own_off];
+  }
+  fetch_n_halo(kb);
+  const double mdt = q.mdt, dx2 = q.dx2, dy2 = q.dy2, dz2 = q.dz2;
+  auto dt_ = [&](double v) { return div_rn(v, mdt, q.rmdt); };
+  auto dx_ = [&](double v) { return div_rn(v, dx2, q.rdx2); };
+  auto dy_ = [&](double v) { return div_rn(v, dy2, q.rdy2); };
+  auto dz_ = [&](double v) { return div_rn(v, dz2, q.rdz2); };
+  long m = N1N0 * kb + own_off;
+  for (int k = kb; k < ke; ++k, m += N1N0) {
+    const int s0 = (k - kb + 1) % 3, sp = (k - kb + 2) % 3, sm = (k - kb) % 3;
+    stage_p(sp);
+    {
+      const double own[3] = {qx[1], qy[1], qz};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double* d = L0 + c * P;
+        if (own_ok) d[own_slot] = own[c];
+        if (hrow_ok) d[hrow_slot] = n_row[c];
+        if (hcol_ok) d[hcol_slot] = n_col[c];
+      }
+    }
+    __syncthreads();
+    if (k + 2 <= ke) fetch_p(k + 2);
+    if (k + 1 < ke) {
+      fetch_n_halo(k + 1);
+      if (own_ok) {
+        qx[3] = (double)n[0][m + 2 * N1N0];
+        qy[3] = (double)n[1][m + 2 * N1N0];
+        qz_next = (double)n[2][m + N1N0];
+      }
+    }
+    if (active) {
+      const double v1[3] = {qx[1], qy[1], qz};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double* c0 = L1 + (c * 3 + s0) * P + own_slot;
+        const double* cp = L1 + (c * 3 + sp) * P + own_slot;
+        const double* cm = L1 + (c * 3 + sm) * P + own_slot;
+        double r;
+        if (NSFD) {
+          double tx = c0[W] + c0[-W] + s.beta * (cp[W] + cm[W] + cp[-W] + cm[-W]);
+          double ty = c0[1] + c0[-1] + s.beta * (cp[1] + cm[1] + cp[-1] + cm[-1]);
+          r = s.a0 * c0[0] - v1[c] + s.alpha * (s.a1 * tx + s.a2 * ty) + s.a3 * (cp[0] + cm[0]);
+        } else {
+          r = s.a0 * c0[0] - v1[c] + s.a1 * (c0[W] + c0[-W]) + s.a2 * (c0[1] + c0[-1]) + s.a3 * (cp[0] + cm[0]);
+        }
+        x[c][m] = (R)r;
+      }
+      if (ebneed) {
+        const double* n0l = L0 + own_slot;
+        const double* n1l = L0 + P + own_slot;
+        const double* n2l = L0 + 2 * P + own_slot;
+        const double* p0l = L1 + (0 * 3 + s0) * P + own_slot;
+        const double* p1l = L1 + (1 * 3 + s0) * P + own_slot;
+        const double* p2l = L1 + (2 * 3 + s0) * P + own_slot;
+        const double p0p = L1[(0 * 3 + sp) * P + own_slot], p0m = L1[(0 * 3 + sm) * P + own_slot];
+        const double p1p = L1[(1 * 3 + sp) * P + own_slot], p1m = L1[(1 * 3 + sm) * P + own_slot];
+        const double ex = dt_(p0l[0]) - dt_(qx[1]);
+        const double ey = dt_(p1l[0]) - dt_(qy[1]);
+        float ez = (float)dt_(p2l[0]);
+        ez = (float)((double)ez - dt_(n2l[0]));
+        const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_(qy[2] - qy[0]) +
+                                        dy_(p2l[1] - p2l[-1]) - dz_(p1p - p1m)));
+        const float by = (float)(0.5 * (dz_(qx[2] - qx[0]) - dx_(n2l[W] - n2l[-W]) +
+                                        dz_(p0p - p0m) - dx_(p2l[W] - p2l[-W])));
+        const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
+                                        dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
+        const f2 v0 = {(float)ex, (float)ey}, vv1 = {ez, bx}, v2 = {by, bz};
+        auto put = [&](long node) {
+          ebg[kEBPairs * node] = v0;
+          ebg[kEBPairs * node + 1] = vv1;
+          ebg[kEBPairs * node + 2] = v2;
+        };
+        put(m);
+        if (k == 1) put(m - N1N0);
+        if (k == N2 - 2) put(m + N1N0);
+      }
+    }
+    __syncthreads();
+    qx[0] = qx[1]; qx[1] = qx[2]; qx[2] = qx[3];
+    qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3];
+    qz = qz_next;
+  }
+}
+
 // ---- batched forms: persistent workgroups walking a table of (instance, local tile) ---------
 // Workgroup b takes entries r * G + l(b), r = 0, 1, ..., where l gives every XCD a contiguous
 // run of entries, so neighbouring tiles of one instance meet in one L2.
@@ -953,6 +1149,49 @@ hipError_t field_variant(Domain* d, int kernel, int v, int klo, int khi, const i
   return hipErrorInvalidValue;
 }
 
+template <typename R, bool NSFD, int TI, int K>
+hipError_t stencil_eb_launch(Domain* d, const int* zr, int pb0, int pb1) {
+  const mithra_grid& g = d->g;
+  const int tilesJ = (g.N1 + kTileJ - 1) / kTileJ;
+  const int tilesI = (g.N0 - 2 + TI - 1) / TI;
+  const int chunks = (g.N2 - 2 + K - 1) / K;
+  const long nblk = (long)tilesJ * tilesI * chunks;
+  const double* a = d->co.a;
+  const StencilConst s = {a[0], a[1], a[2], a[3], d->co.alpha, d->co.beta};
+  EvalConst q;
+  q.mdt = -g.dt; q.dx2 = 2.0 * g.dx; q.dy2 = 2.0 * g.dy; q.dz2 = 2.0 * g.dz;
+  q.rmdt = 1.0 / q.mdt; q.rdx2 = 1.0 / q.dx2; q.rdy2 = 1.0 / q.dy2; q.rdz2 = 1.0 / q.dz2;
+  hipLaunchKernelGGL((k_stencil_eb<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
+                     d->lvl[d->i_np1], d->lvl[d->i_n], d->lvl[d->i_nm1], d->eb, g.N0, g.N1, g.N2, tilesJ, tilesI, s,
+                     q, zr, pb0, pb1);
+  return hipGetLastError();
+}
+
+template <typename R, bool NSFD>
+hipError_t stencil_eb_variant(Domain* d, const int* zr, int pb0, int pb1) {
+  static_assert(kFusedVariants == 4, "variant table and switch disagree");
+  switch (d->tile_fused) {
+    case 0: return stencil_eb_launch<R, NSFD, 4, 64>(d, zr, pb0, pb1);
+    case 1: return stencil_eb_launch<R, NSFD, 8, 64>(d, zr, pb0, pb1);
+    case 2: return stencil_eb_launch<R, NSFD, 16, 64>(d, zr, pb0, pb1);
+    case 3: return stencil_eb_launch<R, NSFD, 8, 32>(d, zr, pb0, pb1);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t stencil_eb_impl(Domain* d) {
+  const int* zr = nullptr;
+  int pb0 = 0, pb1 = -1;
+  if (d->lazy_fields && d->zrange_valid) {
+    zr = d->zrange;
+    pb0 = d->pk_lo;
+    pb1 = d->pk_lo <= d->pk_hi ? d->pk_hi + 1 : -1;
+  }
+  return d->g.solver == MITHRA_SOLVER_NSFD ? stencil_eb_variant<R, true>(d, zr, pb0, pb1)
+                                           : stencil_eb_variant<R, false>(d, zr, pb0, pb1);
+}
+
 template <typename R>
 hipError_t interior_impl(Domain* d, bool fuse_yfaces) {
   const int ymask = fuse_yfaces ? yface_mask(d->tile_interior, d->g.N1) : 0;
@@ -1021,6 +1260,13 @@ extern const int kTileShapes[kTileVariants][2] = {{4, 64}, {2, 64}, {4, 32}, {4,
 hipError_t launch_interior(Domain* d, bool fuse_yfaces) {
   return d->precision == MITHRA_PRECISION_FP64 ? interior_impl<double>(d, fuse_yfaces)
                                                : interior_impl<float>(d, fuse_yfaces);
+}
+
+extern const int kFusedShapes[kFusedVariants][2] = {{4, 64}, {8, 64}, {16, 64}, {8, 32}};
+
+hipError_t launch_stencil_eb(Domain* d) {
+  if (d->ncomp != 3 || d->tile_fused < 0 || d->tile_fused >= kFusedVariants) return hipErrorInvalidValue;
+  return d->precision == MITHRA_PRECISION_FP64 ? stencil_eb_impl<double>(d) : stencil_eb_impl<float>(d);
 }
 
 hipError_t launch_tfsf(Domain* d, double time, const mithra_seed* seed) {

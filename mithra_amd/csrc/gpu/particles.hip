@@ -520,11 +520,12 @@ __device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, d
 constexpr int kDepositThreads = 512;
 constexpr int kDepositChunk = 16;  // node planes per workgroup (at most)
 
-template <typename R, bool SC>
+template <typename R, bool SC, bool SRC = false>
 __device__ inline void deposit_rows_body(const double* __restrict__ part, long cap, FieldSet J, const DepositConst& c,
                                          const unsigned* __restrict__ p_row, const unsigned* __restrict__ p_idx,
                                          const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-                                         int TI, int tilesI, int KC, int accumulate, int block) {
+                                         int TI, int tilesI, int KC, int accumulate, double src_coef,
+                                         int block) {
   extern __shared__ double tile[];  // [2][ncomp][TI][N1]
   constexpr int ncomp = SC ? 4 : 3;
   const int chunk = block / tilesI;
@@ -535,13 +536,14 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   const long N1N0 = (long)N1 * c.N0;
   const int tsz = TI * N1, psz = ncomp * tsz, own = ti * N1;
   const int lo = max(i0 - 1, 0), hi = min(i0 + ti - 1, c.N0 - 2);
-  // slot k & 1 := node plane k (the buffer's values with `accumulate`, else zero)
+  // slot k & 1 := node plane k (the buffer's values with `accumulate` = 1, else zero)
+  const bool add = accumulate == 1;
   auto start_plane = [&](int k) {
     double* p = tile + (k & 1) * psz;
     const long base = N1N0 * k + (long)N1 * i0;
     for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
       const int comp = t / own, o = t - comp * own;
-      p[comp * tsz + o] = accumulate ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
+      p[comp * tsz + o] = add ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
     }
   };
   start_plane(kb);
@@ -579,22 +581,32 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
       double* p = tile + (kc & 1) * psz;
       const long base = N1N0 * kc + (long)N1 * i0;
       const bool next = kc + 2 < ke;
+      const bool inner_plane = kc >= 1 && kc <= c.N2 - 2;
       for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
         const int comp = t / own, o = t - comp * own;
-        G((R*)component(J, comp))[base + o] = (R)p[comp * tsz + o];
-        if (next) p[comp * tsz + o] = accumulate ? (double)G((const R*)component(J, comp))[base + 2 * N1N0 + o] : 0.0;
+        const gptr<R> out = G((R*)component(J, comp));
+        if (SRC) {
+          // the pipelined step: X + a4 J at interior nodes, J rounded as the buffer would hold it
+          const int r = o / N1, jj = o - r * N1;
+          if (inner_plane && i0 + r >= 1 && i0 + r <= c.N0 - 2 && jj >= 1 && jj <= N1 - 2)
+            out[base + o] = (R)((double)out[base + o] + src_coef * (double)(R)p[comp * tsz + o]);
+        } else {
+          out[base + o] = (R)p[comp * tsz + o];
+        }
+        if (next) p[comp * tsz + o] = add ? (double)G((const R*)component(J, comp))[base + 2 * N1N0 + o] : 0.0;
       }
       __syncthreads();
     }
   }
 }
 
-template <typename R, bool SC>
+template <typename R, bool SC, bool SRC = false>
 __global__ __launch_bounds__(kDepositThreads, SC ? 4 : 8) void k_deposit_rows(
     const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
     const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-    int TI, int tilesI, int KC, int accumulate) {
-  deposit_rows_body<R, SC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, KC, accumulate, blockIdx.x);
+    int TI, int tilesI, int KC, int accumulate, double src_coef) {
+  deposit_rows_body<R, SC, SRC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, KC, accumulate, src_coef,
+                                blockIdx.x);
 }
 
 // Bunching factor partial sums: per workgroup sum of q cos(k z), q sin(k z), q over its particles.
@@ -814,7 +826,7 @@ __global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* _
       const CellOrder mo = v.ord[1 - v.parity];
       const DepositConst c = v.dep;  // a local copy: the LDS atomics cannot alias it
       deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
-                                  v.dtilesI, kDepositChunk, 0, e.y);
+                                  v.dtilesI, kDepositChunk, 0, 0.0, e.y);
     }
     __syncthreads();  // the LDS tile is reused by the next entry
   }
@@ -1020,6 +1032,8 @@ bool deposit_tiled(const Domain* d) {
 
 hipError_t launch_deposit(Domain* d, int accumulate) {
   const mithra_grid& g = d->g;
+  if (accumulate == kDepositSource && (!deposit_tiled(d) || d->np == 0 || d->ncomp != 3)) return hipErrorInvalidValue;
+  const double src_coef = d->co.a[4];
   if (!deposit_tiled(d)) {
     if (!accumulate) {
       for (int c = 0; c < d->ncomp; ++c) {
@@ -1055,18 +1069,24 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   const dim3 grid((unsigned)deposit_blocks(g.N2, tilesI, KC));
   const unsigned *pr = d->ord[0].row, *pi = d->ord[0].idx, *mr = d->ord[1].row, *mi = d->ord[1].idx;
   const bool f64 = d->precision == MITHRA_PRECISION_FP64;
-  if (f64 && c.sc)
+  if (accumulate == kDepositSource && f64)
+    hipLaunchKernelGGL((k_deposit_rows<double, false, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part,
+                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef);
+  else if (accumulate == kDepositSource)
+    hipLaunchKernelGGL((k_deposit_rows<float, false, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part,
+                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef);
+  else if (f64 && c.sc)
     hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
   else if (f64)
     hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
   else if (c.sc)
     hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
   else
     hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // Periodic reorder of the particle arrays into the rnp row order.
   if (d->sort_interval > 0 && d->sort_count % d->sort_interval == 0)
