@@ -941,7 +941,7 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
         } else {
           r = s.a0 * c0[0] - v1[c] + s.a1 * (c0[W] + c0[-W]) + s.a2 * (c0[1] + c0[-1]) + s.a3 * (cp[0] + cm[0]);
         }
-        x[c][m] = (R)r;
+        st_stream(x[c], m, (R)r);  // X and E/B are not read again here: keep L2 for the halos
       }
       if (ebneed) {
         const double* n0l = L0 + own_slot;
@@ -964,9 +964,9 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
                                         dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
         const f2 v0 = {(float)ex, (float)ey}, vv1 = {ez, bx}, v2 = {by, bz};
         auto put = [&](long node) {
-          ebg[kEBPairs * node] = v0;
-          ebg[kEBPairs * node + 1] = vv1;
-          ebg[kEBPairs * node + 2] = v2;
+          st_stream(ebg, kEBPairs * node, v0);
+          st_stream(ebg, kEBPairs * node + 1, vv1);
+          st_stream(ebg, kEBPairs * node + 2, v2);
         };
         put(m);
         if (k == 1) put(m - N1N0);

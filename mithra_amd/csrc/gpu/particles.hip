@@ -588,8 +588,10 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
         if (SRC) {
           // the pipelined step: X + a4 J at interior nodes, J rounded as the buffer would hold it
           const int r = o / N1, jj = o - r * N1;
-          if (inner_plane && i0 + r >= 1 && i0 + r <= c.N0 - 2 && jj >= 1 && jj <= N1 - 2)
-            out[base + o] = (R)((double)out[base + o] + src_coef * (double)(R)p[comp * tsz + o]);
+          if (inner_plane && i0 + r >= 1 && i0 + r <= c.N0 - 2 && jj >= 1 && jj <= N1 - 2) {
+            const double xo = (double)ld_stream((gptr<const R>)out, base + o);  // read once, written once
+            st_stream(out, base + o, (R)(xo + src_coef * (double)(R)p[comp * tsz + o]));
+          }
         } else {
           out[base + o] = (R)p[comp * tsz + o];
         }

@@ -96,7 +96,8 @@ def test_pipelined_bit_exact_sparse(case, steps):
 
 def test_pipelined_full_bunch(case):
     """The reference's whole bunch over its fixture's step count: the pipelined run agrees with the
-    plain steps to the rounding of the deposit's summation order."""
+    plain steps to the rounding of the deposit's summation order, held to the 100-step bar of the
+    parity tests (1e-10 relative L2; two plain runs differ by the same order)."""
     plain, pipe = case.open("gpu"), case.open("gpu")
     n = case.n_steps
     run(case, plain, n, False)
@@ -105,7 +106,7 @@ def test_pipelined_full_bunch(case):
     for (k, x), (_, y) in zip(flatten(a), flatten(b)):
         d = rel_l2(y, x)
         print(f"{case.name} {k}: {d:.3e}", flush=True)
-        assert d < 1e-11, k
+        assert d < 1e-10, k
     for key in ("level0", "level1", "level2"):   # and the oracle-checked plain path's own bar
         assert rel_l2(b[key][0], case.arr(("preN_an", "preN_anm1", "preN_jn")[int(key[-1])])) < 1e-10
 
