@@ -618,10 +618,10 @@ __device__ inline void eval_body(FieldSet An, FieldSet Ap, float* __restrict__ e
     const float bz = (float)(0.5 * (dx_(rd(n1, m + N1) - rd(n1, m - N1)) - dy_(rd(n0, m + 1) - rd(n0, m - 1)) +
                                     dx_(rd(p1, m + N1) - rd(p1, m - N1)) - dy_(rd(p0, m + 1) - rd(p0, m - 1))));
     const f2 v0 = {(float)ex, (float)ey}, v1 = {ez, bx}, v2 = {by, bz};
-    auto put = [&](long node) {
-      ebg[kEBPairs * node] = v0;
-      ebg[kEBPairs * node + 1] = v1;
-      ebg[kEBPairs * node + 2] = v2;
+    auto put = [&](long node) {  // E/B is read by later kernels only: stream past L2
+      st_stream(ebg, kEBPairs * node, v0);
+      st_stream(ebg, kEBPairs * node + 1, v1);
+      st_stream(ebg, kEBPairs * node + 2, v2);
     };
     put(m);
     if (k == 1) put(m - N1N0);
@@ -757,10 +757,10 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
       const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
                                       dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
       const f2 v0 = {(float)ex, (float)ey}, v1 = {ez, bx}, v2 = {by, bz};
-      auto put = [&](long node) {
-        ebg[kEBPairs * node] = v0;
-        ebg[kEBPairs * node + 1] = v1;
-        ebg[kEBPairs * node + 2] = v2;
+      auto put = [&](long node) {  // E/B is read by later kernels only: stream past L2
+        st_stream(ebg, kEBPairs * node, v0);
+        st_stream(ebg, kEBPairs * node + 1, v1);
+        st_stream(ebg, kEBPairs * node + 2, v2);
       };
       put(m);
       if (k == 1) put(m - N1N0);
