@@ -11,10 +11,15 @@ quiet start (four beamlets lambda_r / 4 apart, shot-noise modulation per lambda_
 Lorentz boost, coefficients) exactly as `mithra job.job` does.
 
 One step = one iteration of Solver::solve()'s main loop on the GPU: field update (interior,
-Mur, E/B), bunch push, P(z) sample, shift, current reset, deposition.
+Mur, E/B), bunch push, P(z) sample, shift, current reset, deposition.  By default the steps run
+pipelined (mithra_dev_step_pipelined): E/B of a step is evaluated with the next interior update's
+J-free part in one kernel, and the deposit adds a4 J to it; the timed steps form one complete run
+(its first step makes the interior update, its last the separate E/B and J deposit), with the
+same FP64 results as the plain step.  --plain times mithra_dev_step instead.
 
 Prints ONE JSON line.  value = cell-updates/s (nodes x steps / s); particle-pushes/s and the
-roofline of the dominant kernel (the interior stencil) are reported beside it, with the CPU
+roofline of the dominant kernel (the fused stencil + E/B kernel; the interior stencil with
+--plain) are reported beside it, with the CPU
 oracle timed on a bounded sample of the same workload on this host.  scan_points_per_hour is
 the batched parameter scan's rate (64 seeded instances of uneven size, each a job taken to its
 P(z) and bunching at 128 positions, host set-up included), measured in the same run beside the
