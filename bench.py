@@ -334,7 +334,8 @@ def main():
         n_part = job.particles().shape[0]
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
-        pipelined = not args.plain
+        # the library pipelines FP64 without space charge only; FP32-bulk takes the plain step
+        pipelined = not args.plain and precision == abi.PRECISION_FP64
         elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision, args.reorder,
                                                       pipelined)
         log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
