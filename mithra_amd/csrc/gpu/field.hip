@@ -1174,7 +1174,7 @@ hipError_t stencil_eb_variant(Domain* d, const int* zr, int pb0, int pb1) {
     case 0: return stencil_eb_launch<R, NSFD, 4, 64>(d, zr, pb0, pb1);
     case 1: return stencil_eb_launch<R, NSFD, 8, 64>(d, zr, pb0, pb1);
     case 2: return stencil_eb_launch<R, NSFD, 16, 64>(d, zr, pb0, pb1);
-    case 3: return stencil_eb_launch<R, NSFD, 8, 32>(d, zr, pb0, pb1);
+    case 3: return stencil_eb_launch<R, NSFD, 8, 128>(d, zr, pb0, pb1);
   }
   return hipErrorInvalidValue;
 }
@@ -1262,7 +1262,7 @@ hipError_t launch_interior(Domain* d, bool fuse_yfaces) {
                                                : interior_impl<float>(d, fuse_yfaces);
 }
 
-extern const int kFusedShapes[kFusedVariants][2] = {{4, 64}, {8, 64}, {16, 64}, {8, 32}};
+extern const int kFusedShapes[kFusedVariants][2] = {{4, 64}, {8, 64}, {16, 64}, {8, 128}};
 
 hipError_t launch_stencil_eb(Domain* d) {
   if (d->ncomp != 3 || d->tile_fused < 0 || d->tile_fused >= kFusedVariants) return hipErrorInvalidValue;
