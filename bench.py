@@ -307,7 +307,7 @@ def traffic_from_profiles(kernel, precision):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0 skips it)")
