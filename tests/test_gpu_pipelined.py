@@ -188,3 +188,23 @@ def test_timed_phases(case):
     assert all(x >= 0 for m in ms for x in m)
     if not case.grid.space_charge:
         assert ms[1][0] < ms[0][0]
+
+
+def test_no_particles_runs_plain(case):
+    """A domain without particles cannot hand its J to X (there is no deposit to add it), so a
+    more = 1 request runs the plain step: nothing is left pending and the march equals the plain
+    one bit for bit."""
+    plain, pipe = case.open("gpu"), case.open("gpu")
+    for dom in (plain, pipe):
+        dom.set_particles(np.zeros((0, 11)))
+    time, tb = case.time0, case.time_bunch0
+    for step in range(3):
+        pipe.step_pipelined(time, case.seed, case.bunch, case.undulators, tb, case.dtb, case.n_sub, step, step,
+                            more=True)
+        assert not pipe.pipeline_pending()
+        plain.step(time, case.seed, case.bunch, case.undulators, tb, case.dtb, case.n_sub, step, step)
+        for _ in range(case.n_sub):
+            tb += case.dtb
+        time += case.dt
+    for (k, x), (_, y) in zip(flatten(state(case, plain, 3)), flatten(state(case, pipe, 3))):
+        assert np.array_equal(x, y), k
