@@ -95,6 +95,112 @@ FEL-OUTPUT
 """
 
 
+# Space-charge and seeded workload: 128 x 128 x 1024 nodes, 4,194,304 particles, NSFD, Mur order 2,
+# space charge (the phi update and grad phi in E) and a Gaussian-beam seed injected by TF/SF.  The
+# cells are those of the reference's CPU-runnable case (380 um x 380 um x 1.6 um lab at gamma
+# 196.7, the smallest transverse cell NSFD accepts there), the bunch a Gaussian of 100 pC spanning
+# about an eighth of the mesh along each axis: a throughput case for the space-charge and TF/SF
+# path, not a physics one (the mesh does not resolve the radiation).
+C3_NODES = (128, 128, 1024)
+C3_PARTICLES = 4_194_304
+C3_GAMMA = 196.7
+C3_DX, C3_DZ = 380.0, 1.6
+C3_JOB = """MESH
+{{
+  length-scale            = MICROMETER
+  time-scale              = PICOSECOND
+  mesh-lengths            = ( {lx:.10g}, {lx:.10g}, {lz:.10g} )
+  mesh-resolution         = ( {dx:.10g}, {dx:.10g}, {dz:.10g} )
+  mesh-center             = ( 0.0, 0.0, 0.0 )
+  total-time              = 1.0e6
+  mesh-truncation-order   = 2
+  space-charge            = true
+  solver                  = NSFD
+}}
+BUNCH
+{{
+  bunch-initialization
+  {{
+    type                  = other
+    charge                = {charge:.10g}
+    number-of-particles   = {n}
+    gamma                 = {gamma:.10g}
+    direction             = ( 0.0, 0.0, 1.0 )
+  }}
+}}
+FIELD
+{{
+  field-initialization
+  {{
+    type                  = gaussian-beam
+    position              = ( 0.0, 0.0, {zseed:.10g} )
+    direction             = ( 0.0, 0.0, 1.0 )
+    polarization          = ( 0.0, 1.0, 0.0 )
+    radius-parallel       = {rseed:.10g}
+    radius-perpendicular  = {rseed:.10g}
+    strength-parameter    = 2.0e-5
+    signal-type           = gaussian
+    offset                = 0.0
+    pulse-length          = 1.0e12
+    wavelength            = {wseed:.10g}
+    CEP                   = 0.0
+  }}
+}}
+UNDULATOR
+{{
+  static-undulator
+  {{
+    undulator-parameter   = 1.0
+    period                = 1.5e4
+    length                = 100
+    polarization-angle    = 0.0
+  }}
+}}
+FEL-OUTPUT
+{{
+  radiation-power
+  {{
+    sample                = true
+    type                  = at-point
+    directory             = {dir}/
+    base-name             = power-sampling/power
+    plane-position        = {zpower:.10g}
+    normalized-frequency  = 1.00
+  }}
+}}
+"""
+
+
+def c3_bunch(n, seed=3):
+    """Lab-frame records q x y z gbx gby gbz (um) of the Gaussian bunch of the C3 workload."""
+    rng = np.random.default_rng(seed)
+    lx = (C3_NODES[0] - 1.5) * C3_DX
+    lz = (C3_NODES[2] - 1.5) * C3_DZ
+    electrons = 100e-12 / 1.602e-19
+    rec = np.empty((n, 7))
+    rec[:, 0] = electrons / n
+    rec[:, 1] = rng.normal(0.0, lx / 16.0, n)
+    rec[:, 2] = rng.normal(0.0, lx / 16.0, n)
+    rec[:, 3] = rng.normal(0.0, lz / 16.0, n)
+    rec[:, 4] = rng.normal(0.0, 1e-3, n)
+    rec[:, 5] = rng.normal(0.0, 1e-3, n)
+    rec[:, 6] = np.sqrt(C3_GAMMA ** 2 - 1.0) * (1.0 + 1e-3 * rng.normal(0.0, 1.0, n))
+    return rec, electrons
+
+
+def set_up_c3(workdir):
+    log("generating the C3 bunch (space charge, Gaussian seed) and running the host set-up")
+    bunch, electrons = c3_bunch(C3_PARTICLES)
+    lx, lz = (C3_NODES[0] - 1.5) * C3_DX, (C3_NODES[2] - 1.5) * C3_DZ   # (int)(L / d) + 2 nodes
+    path = os.path.join(workdir, "c3.job")
+    with open(path, "w") as f:
+        f.write(C3_JOB.format(lx=lx, lz=lz, dx=C3_DX, dz=C3_DZ, charge=electrons, n=C3_PARTICLES, gamma=C3_GAMMA,
+                              zseed=lz / 4.0, rseed=lx / 4.0, wseed=20.0 * C3_DZ, zpower=lz / 8.0, dir=workdir))
+    job = Job(path, external_bunch=bunch)
+    del bunch
+    return job
+
+
 def quiet_start_bunch(n, seed=2):
     """Lab-frame records q x y z gbx gby gbz (lengths in um) of the flat-top slice."""
     rng = np.random.default_rng(seed)
@@ -314,6 +420,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--reorder", type=int, default=16,
                     help="deposits between reorders of the particle arrays into cell-row order (0: never)")
+    ap.add_argument("--workload", choices=("production", "c3"), default="production",
+                    help="production (256x256x2048, no space charge) or c3 (128x128x1024, space charge, "
+                         "Gaussian seed; the plain step, as the pipelined form covers runs without space charge)")
     ap.add_argument("--plain", action="store_true",
                     help="time the plain step (mithra_dev_step) instead of the pipelined one")
     ap.add_argument("--scan", type=int, default=64,
@@ -326,7 +435,7 @@ def main():
     precision = abi.PRECISION_FP64 if args.precision == "fp64" else abi.PRECISION_FP32_BULK
     with tempfile.TemporaryDirectory(prefix="mithra_bench_") as work:
         t_setup = time.perf_counter()
-        job = set_up(work)
+        job = set_up(work) if args.workload == "production" else set_up_c3(work)
         t_setup = time.perf_counter() - t_setup
         g = job.grid()
         nodes = g.N0 * g.N1 * g.N2
@@ -335,7 +444,7 @@ def main():
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
         # the library pipelines FP64 without space charge only; FP32-bulk takes the plain step
-        pipelined = not args.plain and precision == abi.PRECISION_FP64
+        pipelined = not args.plain and precision == abi.PRECISION_FP64 and args.workload == "production"
         elapsed, phases, counters, P, tiles = run_gpu(job, args.steps, args.warmup, precision, args.reorder,
                                                       pipelined)
         log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
@@ -344,15 +453,16 @@ def main():
         push_rate = n_part * n_sub * args.steps / elapsed
         # the dominant field kernel and its algorithmic bytes per interior node
         real = 8 if precision == abi.PRECISION_FP64 else 4
+        ncomp = 4 if args.workload == "c3" else 3
         if pipelined:   # reads A^{n+1}, A^n; writes X and E/B (6 floats)
             kernel, phase, bytes_per_node = "stencil_eb", "eval_eb", 9 * real + 24
             kernel_name = "k_stencil_eb (E/B fused with the next NSFD update's stencil)"
-        else:           # reads A^n, A^{n-1}, J; writes A^{n+1}
-            kernel, phase, bytes_per_node = "interior", "interior", 12 * real
-            kernel_name = "k_interior (NSFD stencil)"
+        else:           # reads A^n, A^{n-1}, J (and phi's three levels with space charge); writes A^{n+1}
+            kernel, phase, bytes_per_node = "interior", "interior", 4 * ncomp * real
+            kernel_name = "k_interior (NSFD stencil" + (", A and phi)" if ncomp == 4 else ")")
         t_k = phases[abi.PHASES.index(phase)] * 1e-3
         achieved = bytes_per_node * interior / t_k / 1e9
-        scan_out = run_scan(args.scan, work) if args.scan > 0 else None
+        scan_out = run_scan(args.scan, work) if args.scan > 0 and args.workload == "production" else None
         copy_gbs = abi.copy_bandwidth()
         stream_gbs = abi.stream_bandwidth()
         cpu = None
@@ -361,9 +471,9 @@ def main():
             cpu_t, used = run_cpu_baseline(job, args.cpu_steps, args.cpu_threads)
             cpu = {"value": nodes * args.cpu_steps / cpu_t, "unit": "cell-updates/s", "cores": used,
                    "kind": "port",
-                   "sample": f"{args.cpu_steps} full steps of the same 256x256x2048 mesh and 16.8M-particle "
-                             f"bunch with the CPU oracle (C++ -O3, OpenMP over nodes and particles, "
-                             f"sequential deposition), {cpu_t:.1f} s",
+                   "sample": f"{args.cpu_steps} full steps of the same {g.N0}x{g.N1}x{g.N2} mesh and "
+                             f"{n_part}-particle bunch with the CPU oracle (C++ -O3, OpenMP over nodes and "
+                             f"particles, sequential deposition), {cpu_t:.1f} s",
                    "particle_pushes_per_s": n_part * n_sub * args.cpu_steps / cpu_t}
     out = {
         "metric": METRIC,
@@ -379,7 +489,10 @@ def main():
         "dtype": "f64" if precision == abi.PRECISION_FP64 else "f32 fields, f64 particles",
         "data": "synthetic: seeded quiet-start bunch, random-free job set-up through the drop-in host",
         "config": {"workload": "production single instance (256x256x2048 nodes, 16.8M macro-particles, "
-                               "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)",
+                               "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)"
+                               if args.workload == "production" else
+                               "C3: 128x128x1024 nodes, 4.2M macro-particles, 100 MeV, K 1, lambda_u 1.5 cm, NSFD, "
+                               "Mur order 2, space charge, Gaussian-beam seed (TF/SF)",
                    "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
                    "precision": args.precision, "parallelism": "single GPU, whole domain resident",
                    "reorder_interval": args.reorder,
@@ -392,7 +505,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "frac_of_measured_stream": achieved / stream_gbs, "measured_stream_3r1w_GBps": stream_gbs,
                      "measured_copy_GBps": copy_gbs,
-                     "traffic": traffic_from_profiles(kernel, args.precision),
+                     "traffic": traffic_from_profiles(kernel, args.precision) if args.workload == "production" else None,
                      "algorithmic_bytes_per_launch": bytes_per_node * interior},
         "cpu_baseline": cpu,
         "setup_s": t_setup,
