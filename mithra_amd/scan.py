@@ -308,8 +308,10 @@ def run_batch(prepared, variant=None, use_graph=True, autotune=True):
 
 
 def main():
-    """`python -m mithra_amd.scan [--instances N] [--mode batch|single] [--seed S]`: sets up the
-    seeded scan through the drop-in host and marches it, printing one JSON line of timings."""
+    """`python -m mithra_amd.scan [--instances N] [--mode batch|single] [--seed S] [--eager]`: sets up
+    the seeded scan through the drop-in host and marches it, printing one JSON line of timings.
+    --eager launches the batch's kernels one by one instead of replaying the captured HIP graph
+    (same kernels, same results; for kernel traces)."""
     import argparse
     import json
     import sys
@@ -319,6 +321,7 @@ def main():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--mode", choices=("batch", "single"), default="batch")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp32")
+    ap.add_argument("--eager", action="store_true", help="no captured graph (batch mode)")
     args = ap.parse_args()
     precision = abi.PRECISION_FP64 if args.precision == "fp64" else abi.PRECISION_FP32_BULK
     insts = generate(args.instances, seed=args.seed)
@@ -327,14 +330,14 @@ def main():
         prep = prepare(insts, work, precision, log=lambda s: print(f"[scan] {s}", file=sys.stderr, flush=True))
         t_setup = time.perf_counter() - t0
         if args.mode == "batch":
-            _, t_march, tuned = run_batch(prep)
+            _, t_march, tuned = run_batch(prep, use_graph=not args.eager)
         else:
             _, t_march = run_single(prep)
             tuned = None
         for p in prep:
             p.dom.close()
             p.job.close()
-    print(json.dumps({"instances": args.instances, "seed": args.seed, "mode": args.mode,
+    print(json.dumps({"instances": args.instances, "seed": args.seed, "mode": args.mode, "graph": not args.eager,
                       "precision": args.precision, "setup_s": t_setup, "march_s": t_march,
                       "points_per_hour": 3600.0 * args.instances / (t_setup + t_march),
                       "tile_variant": tuned[0] if tuned else None}), flush=True)
