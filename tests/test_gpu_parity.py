@@ -1,7 +1,7 @@
 """Parity of the HIP path (libmithra_gpu.so, through its C-ABI) against the CPU oracle and the
 reference's own fixtures, FP64 drop-in mode.
 
-Bar (north-star tolerance): fields, currents and particles after 100 steps within 1e-10
+Bar: fields, currents and particles after 100 steps within 1e-10
 relative L2 of the oracle, P(z) within 1e-6 relative of the reference's power file.  The field
 update itself is checked bit-exact after one step (same operation order, no contraction); the
 push uses the device's libm and the deposition FP64 atomics, whose summation order differs from
