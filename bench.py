@@ -416,7 +416,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU-oracle sample (0 skips it)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU-oracle sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--reorder", type=int, default=16,
                     help="deposits between reorders of the particle arrays into cell-row order (0: never)")
