@@ -873,14 +873,16 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
       if (hcol_ok) d[hcol_slot] = p_col[c];
     }
   };
-  // A^n: own A_x, A_y at planes k-1 .. k+2 (qx, qy), own A_z at k and k+1, halo of the next plane
+  // A^n: own A_x, A_y at planes k-1 .. k+2 (qx, qy), own A_z at k and k+1, halo of the next plane.
+  // The curl reads A^n_x only at y +- 1 and A^n_y only at x +- 1, so A^n_x needs no halo rows and
+  // A^n_y no halo columns.
   double qx[4] = {}, qy[4] = {}, qz = 0.0, qz_next = 0.0, n_row[3] = {}, n_col[3] = {};
   auto fetch_n_halo = [&](int k) {
     const long b = N1N0 * k;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      if (hrow_ok) n_row[c] = (double)n[c][b + hrow_off];
-      if (hcol_ok) n_col[c] = (double)n[c][b + hcol_off];
+      if (c != 0 && hrow_ok) n_row[c] = (double)n[c][b + hrow_off];
+      if (c != 1 && hcol_ok) n_col[c] = (double)n[c][b + hcol_off];
     }
   };
   fetch_p(kb - 1);
@@ -912,8 +914,8 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
       for (int c = 0; c < 3; ++c) {
         double* d = L0 + c * P;
         if (own_ok) d[own_slot] = own[c];
-        if (hrow_ok) d[hrow_slot] = n_row[c];
-        if (hcol_ok) d[hcol_slot] = n_col[c];
+        if (c != 0 && hrow_ok) d[hrow_slot] = n_row[c];
+        if (c != 1 && hcol_ok) d[hcol_slot] = n_col[c];
       }
     }
     __syncthreads();
