@@ -692,6 +692,10 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
 #pragma unroll
     for (int a = 0; a < NZ; ++a) qz[a][s] = own_ok ? (double)A[zarr[a]][N1N0 * k + own_off] : 0.0;
   };
+  // The curl reads A_x (arrays 0, 3) only at y +- 1 and A_y (arrays 1, 4) only at x +- 1: they
+  // need no halo rows and no halo columns respectively.
+  auto need_row = [](int a) { return a != 0 && a != 3; };
+  auto need_col = [](int a) { return a != 1 && a != 4; };
   auto fetch_plane = [&](int k) {  // what plane k stages besides the queued own values
     const long b = N1N0 * k;
     if (own_ok) {
@@ -700,8 +704,8 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      if (hrow_ok) pre_row[a] = (double)A[a][b + hrow_off];
-      if (hcol_ok) pre_col[a] = (double)A[a][b + hcol_off];
+      if (need_row(a) && hrow_ok) pre_row[a] = (double)A[a][b + hrow_off];
+      if (need_col(a) && hcol_ok) pre_col[a] = (double)A[a][b + hcol_off];
     }
   };
   fetch_z(kb - 1, 0);
@@ -727,8 +731,8 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      if (hrow_ok) L[a * P + hrow_slot] = pre_row[a];
-      if (hcol_ok) L[a * P + hcol_slot] = pre_col[a];
+      if (need_row(a) && hrow_ok) L[a * P + hrow_slot] = pre_row[a];
+      if (need_col(a) && hcol_ok) L[a * P + hcol_slot] = pre_col[a];
     }
     __syncthreads();
     if (k + 1 < ke) fetch_plane(k + 1);
