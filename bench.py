@@ -15,7 +15,10 @@ Mur, E/B), bunch push, P(z) sample, shift, current reset, deposition.  By defaul
 pipelined (mithra_dev_step_pipelined): E/B of a step is evaluated with the next interior update's
 J-free part in one kernel, and the deposit adds a4 J to it; the timed steps form one complete run
 (its first step makes the interior update, its last the separate E/B and J deposit), with the
-same FP64 results as the plain step.  --plain times mithra_dev_step instead.
+same FP64 results as the plain step.  --plain times mithra_dev_step instead.  By default the timed
+run is the production configuration's full 2000 steps: the push gathers grid fields only for
+particles whose emission flag is set (once their lab-frame z passes -dist, src/solver.cpp:1459),
+a fraction that grows from 0 to 1 by step ~1500, so a short run would time a cheaper push.
 
 Prints ONE JSON line.  value = cell-updates/s (nodes x steps / s); particle-pushes/s and the
 roofline of the dominant kernel (the fused stencil + E/B kernel; the interior stencil with
@@ -413,7 +416,8 @@ def traffic_from_profiles(kernel, precision):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=2000,
+                    help="timed steps (default: the production run's full 2000 steps, about 15 s)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU-oracle sample (0 skips it)")
