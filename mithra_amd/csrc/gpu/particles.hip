@@ -541,9 +541,10 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   auto start_plane = [&](int k) {
     double* p = tile + (k & 1) * psz;
     const long base = N1N0 * k + (long)N1 * i0;
-    for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
-      const int comp = t / own, o = t - comp * own;
-      p[comp * tsz + o] = add ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
+    for (int o = threadIdx.x; o < own; o += blockDim.x) {
+#pragma unroll
+      for (int comp = 0; comp < ncomp; ++comp)
+        p[comp * tsz + o] = add ? (double)G((const R*)component(J, comp))[base + o] : 0.0;
     }
   };
   start_plane(kb);
@@ -582,20 +583,27 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
       const long base = N1N0 * kc + (long)N1 * i0;
       const bool next = kc + 2 < ke;
       const bool inner_plane = kc >= 1 && kc <= c.N2 - 2;
-      for (int t = threadIdx.x; t < ncomp * own; t += blockDim.x) {
-        const int comp = t / own, o = t - comp * own;
-        const gptr<R> out = G((R*)component(J, comp));
+      // one node per thread and pass, its components unrolled: no index division per element
+      for (int o = threadIdx.x; o < own; o += blockDim.x) {
+        bool inner = false;
         if (SRC) {
-          // the pipelined step: X + a4 J at interior nodes, J rounded as the buffer would hold it
           const int r = o / N1, jj = o - r * N1;
-          if (inner_plane && i0 + r >= 1 && i0 + r <= c.N0 - 2 && jj >= 1 && jj <= N1 - 2) {
-            const double xo = (double)ld_stream((gptr<const R>)out, base + o);  // read once, written once
-            st_stream(out, base + o, (R)(xo + src_coef * (double)(R)p[comp * tsz + o]));
-          }
-        } else {
-          out[base + o] = (R)p[comp * tsz + o];
+          inner = inner_plane && i0 + r >= 1 && i0 + r <= c.N0 - 2 && jj >= 1 && jj <= N1 - 2;
         }
-        if (next) p[comp * tsz + o] = add ? (double)G((const R*)component(J, comp))[base + 2 * N1N0 + o] : 0.0;
+#pragma unroll
+        for (int comp = 0; comp < ncomp; ++comp) {
+          const gptr<R> out = G((R*)component(J, comp));
+          if (SRC) {
+            // the pipelined step: X + a4 J at interior nodes, J rounded as the buffer would hold it
+            if (inner) {
+              const double xo = (double)ld_stream((gptr<const R>)out, base + o);  // read once, written once
+              st_stream(out, base + o, (R)(xo + src_coef * (double)(R)p[comp * tsz + o]));
+            }
+          } else {
+            out[base + o] = (R)p[comp * tsz + o];
+          }
+          if (next) p[comp * tsz + o] = add ? (double)G((const R*)component(J, comp))[base + 2 * N1N0 + o] : 0.0;
+        }
       }
       __syncthreads();
     }
