@@ -518,7 +518,7 @@ __device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, d
 // belongs to exactly one workgroup, so no global atomics are needed and the J buffer needs no
 // separate reset (`accumulate` = 0 writes the deposit alone, 1 adds it to what the buffer holds).
 constexpr int kDepositThreads = 512;
-constexpr int kDepositChunk = 16;  // node planes per workgroup (at most)
+constexpr int kDepositChunk = 4;  // node planes per workgroup (at most)
 
 template <typename R, bool SC, bool SRC = false>
 __device__ inline void deposit_rows_body(const double* __restrict__ part, long cap, FieldSet J, const DepositConst& c,
