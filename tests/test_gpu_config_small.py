@@ -174,7 +174,7 @@ def diffs(a, b, steps):
 
 def test_small_configuration_400_steps():
     lib = load_oracle()
-    lib.oracle_set_threads(min(16, os.cpu_count() or 1))  # the GPU box's share of host cores
+    lib.oracle_set_threads(min(16, os.cpu_count() or 1))  # at most 16 host threads, the share of host cores one GPU gets
     with tempfile.TemporaryDirectory() as work:
         path = os.path.join(work, "small.job")
         with open(path, "w") as f:
