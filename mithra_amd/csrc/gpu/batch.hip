@@ -6,8 +6,11 @@
 // parity, time, time_bunch, step), and per kernel a table of (instance, local tile) entries over
 // the unfinished instances.  One step is a fixed sequence of batched launches, each a set of
 // persistent workgroups walking its table; since every launch reads its per-step values from
-// the descriptors, one captured HIP graph replays every step, and is re-captured only when an
-// instance finishes and the tables shrink.  The deposition's particle orders come from a
+// the descriptors, one HIP graph, captured once when the scan starts, replays every step to the
+// end of the scan; a finished instance's table entries are skipped on the device by its
+// descriptor's `active` flag, so nothing is re-captured or re-instantiated mid-scan (an earlier
+// form rebuilt the tables and re-instantiated the graph each time an instance finished, some 60
+// times per 64-instance scan).  The deposition's particle orders come from a
 // counting sort by cell row (graph-safe: kernels only).  Every instance computes with the same
 // device functions as its single-domain run, so its P(z) and bunching equal that run's to the
 // rounding of the deposit's summation order.
@@ -95,7 +98,6 @@ struct mithra_batch {
   size_t dep_lds = 0;
   bool prepared = false;
   std::vector<long> step;           // host mirror of each instance's step
-  std::vector<char> built_active;   // active set the tables and graph were built for
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
 };
@@ -122,7 +124,9 @@ int alloc_owned(mithra_batch* b, size_t bytes, void** out) {
 }
 
 // Builds every table over the instances still running; the largest instances come first so
-// the persistent workgroups end together.
+// the persistent workgroups end together.  The tables are built once per scan (and per tile
+// shape while autotuning): an instance that finishes keeps its entries, which the kernels then
+// skip by its descriptor's `active` flag.
 int build_tables(mithra_batch* b) {
   std::vector<int> order;
   for (int i = 0; i < b->n; ++i)
@@ -180,8 +184,6 @@ int build_tables(mithra_batch* b) {
   BatchTiles* ts[] = {&b->t_int[0], &b->t_int[1], &b->t_eval, &b->t_faces, &b->t_edges, &b->t_corners,
                       &b->t_part, &b->t_rows, &b->t_inst, &b->t_dep, &b->t_pnodes, &b->t_pred};
   for (BatchTiles* T : ts) T->t = b->tiles + (size_t)T->t;
-  b->built_active.assign(b->n, 0);
-  for (int i : order) b->built_active[i] = 1;
   return MITHRA_OK;
 }
 
@@ -332,14 +334,15 @@ int mithra_batch_instance(mithra_batch* b, int i, double time, double time_bunch
   if (!b || i < 0 || i >= b->n || !p || n_sub < 1 || n_steps < 0 || bunching_slots < 0) return MITHRA_E_ARG;
   if (b->prepared) return MITHRA_E_STATE;
   mithra_dev* d = b->dev[i];
+  // The batched sort and deposit walk every instance's particle orders, which exist only for
+  // a domain holding particles; an instance without a bunch radiates nothing either.
+  if (d->np <= 0) return MITHRA_E_UNSUPPORTED;
   BATCH_TRY(hipStreamSynchronize(d->stream));
   if (int rc = settle(d)) return rc;
   if (int rc = upload_undulators(d, und, n_und, p->c0)) return rc;
   const mithra_grid& g = d->g;
   const unsigned rows = (unsigned)((g.N2 - 1) * (long)g.N0);
-  if (d->np > 0) {
-    if (hipError_t e = ensure_orders(d, rows); e != hipSuccess) return e == hipErrorOutOfMemory ? MITHRA_E_NOMEM : status(e);
-  }
+  if (hipError_t e = ensure_orders(d, rows); e != hipSuccess) return e == hipErrorOutOfMemory ? MITHRA_E_NOMEM : status(e);
   BATCH_TRY(hipStreamSynchronize(d->stream));
   BatchView& v = b->hv[i];
   std::memset(&v, 0, sizeof v);
@@ -371,7 +374,7 @@ int mithra_batch_instance(mithra_batch* b, int i, double time, double time_bunch
   v.ev.mdt = -g.dt; v.ev.dx2 = 2.0 * g.dx; v.ev.dy2 = 2.0 * g.dy; v.ev.dz2 = 2.0 * g.dz;
   v.ev.rmdt = 1.0 / v.ev.mdt; v.ev.rdx2 = 1.0 / v.ev.dx2; v.ev.rdy2 = 1.0 / v.ev.dy2; v.ev.rdz2 = 1.0 / v.ev.dz2;
   v.dt = g.dt;
-  if (d->np > 0 && !d->part_alt) {  // spare arrays of the periodic reorder
+  if (!d->part_alt) {  // spare arrays of the periodic reorder
     if (hipMalloc(&d->part_alt, sizeof(double) * PCOUNT * (size_t)d->cap) != hipSuccess) return MITHRA_E_NOMEM;
     if (hipMalloc(&d->pid_alt, sizeof(unsigned) * (size_t)d->cap) != hipSuccess) {
       (void)hipFree(d->part_alt);
@@ -454,18 +457,11 @@ int mithra_batch_run(mithra_batch* b, long steps, long* done) {
     if (int rc = prepare(b)) return rc;
   long k = 0;
   for (; k < steps; ++k) {
-    bool any = false, changed = false;
-    for (int i = 0; i < b->n; ++i) {
-      const bool act = b->step[i] < b->hv[i].n_steps;
-      any = any || act;
-      changed = changed || (act != (bool)b->built_active[i]);
-    }
+    bool any = false;
+    for (int i = 0; i < b->n; ++i) any = any || b->step[i] < b->hv[i].n_steps;
     if (!any) break;
-    if (changed) {  // an instance finished: shrink the tables, re-capture the step
-      BATCH_TRY(hipStreamSynchronize(b->stream));
-      if (int rc = build_tables(b)) return rc;
-      if (int rc = capture(b)) return rc;
-    }
+    // the graph captured at preparation replays every step to the end of the scan: each
+    // kernel skips the table entries of instances whose descriptor says they are finished
     if (b->exec) {
       BATCH_TRY(hipGraphLaunch(b->exec, b->stream));
     } else if (int rc = enqueue_step(b)) {

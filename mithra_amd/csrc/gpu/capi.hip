@@ -27,6 +27,20 @@ inline int hip_status(hipError_t e) { return e == hipSuccess ? MITHRA_OK : MITHR
     if (_e != hipSuccess) return hip_status(_e); \
   } while (0)
 
+// Bunching factor: k_bunching runs on a fixed grid, each workgroup leaving (sum q cos kz,
+// sum q sin kz, sum q) over its particles; the host adds the partials in workgroup order.
+constexpr int kBunchingBlocks = 512;
+
+double bunching_from_partials(const double* h, int blocks) {
+  double c = 0.0, s = 0.0, q = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    c += h[3 * b];
+    s += h[3 * b + 1];
+    q += h[3 * b + 2];
+  }
+  return q != 0.0 ? std::sqrt(c * c + s * s) / std::fabs(q) : 0.0;
+}
+
 bool valid_grid(const mithra_grid* g) {
   return g && g->N0 >= 3 && g->N1 >= 3 && g->N2 >= 3 && (g->solver == MITHRA_SOLVER_FD || g->solver == MITHRA_SOLVER_NSFD) &&
          (g->truncation_order == 1 || g->truncation_order == 2) && g->dx > 0 && g->dy > 0 && g->dz > 0 && g->dt > 0 &&
@@ -194,7 +208,9 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
   d->sort_interval = kDefaultSortInterval;
   d->pk_lo = 1;
   d->pk_hi = 0;
-  hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&d->device);
+  if (e != hipSuccess) { delete d; return hip_status(e); }
+  e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete d; return hip_status(e); }
   const size_t per = (size_t)d->nodes * d->real_size;
   // 256-byte aligned component arrays inside one allocation
@@ -320,6 +336,12 @@ int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
     d->part = nullptr;
     d->pid = nullptr;
     d->cap = 0;
+    // until the new arrays exist the domain holds no particles: a failed allocation below
+    // must leave no launch reading the freed ones
+    d->np = 0;
+    d->rm_order_valid = 0;
+    d->order1_is_rnp = 0;
+    d->zrange_valid = 0;
     const long cap = (n + 63) / 64 * 64;
     if (hipMalloc(&d->part, sizeof(double) * PCOUNT * (size_t)cap) != hipSuccess ||
         hipMalloc(&d->pid, sizeof(unsigned) * (size_t)cap) != hipSuccess) {
@@ -630,23 +652,17 @@ int mithra_dev_autotune(mithra_dev* d, int reps, int* chosen, float* ms) {
 
 int mithra_dev_bunching(mithra_dev* d, double k, double* b_out) {
   if (!d || !b_out) return MITHRA_E_ARG;
-  const int blocks = 512;
+  const int blocks = kBunchingBlocks;
   double* partial = nullptr;
   if (hipMalloc(&partial, sizeof(double) * 3 * blocks) != hipSuccess) return MITHRA_E_NOMEM;
-  MITHRA_TRY(launch_fill_zero(d, partial, sizeof(double) * 3 * blocks));
-  hipError_t e = launch_bunching(d, k, partial, blocks);
+  hipError_t e = launch_fill_zero(d, partial, sizeof(double) * 3 * blocks);
+  if (e == hipSuccess) e = launch_bunching(d, k, partial, blocks);
   std::vector<double> h(3 * blocks);
   if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
   if (e == hipSuccess) e = hipMemcpy(h.data(), partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
   hipFree(partial);
   if (e != hipSuccess) return hip_status(e);
-  double c = 0.0, s = 0.0, q = 0.0;
-  for (int b = 0; b < blocks; ++b) {
-    c += h[3 * b];
-    s += h[3 * b + 1];
-    q += h[3 * b + 2];
-  }
-  *b_out = q != 0.0 ? std::sqrt(c * c + s * s) / std::fabs(q) : 0.0;
+  *b_out = bunching_from_partials(h.data(), blocks);
   return MITHRA_OK;
 }
 

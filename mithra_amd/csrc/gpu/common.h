@@ -134,6 +134,7 @@ struct Domain {
   int ncomp;                // 3, or 4 with space charge
   size_t real_size;         // bytes per potential value
   long N1N0, nodes;
+  int device;               // the GPU the domain's memory and stream belong to
   hipStream_t stream;
   void* pot_block;          // one allocation: 3 levels x ncomp x nodes
   FieldSet lvl[3];          // physical buffers

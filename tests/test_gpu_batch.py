@@ -72,3 +72,19 @@ def test_batch_continues_in_single_domain_calls():
             print(f"instance {i}: dP {rel_l2(P, P_ref):.2e}", flush=True)
             assert rel_l2(P, P_ref) <= 1e-12
             assert rel_l2(p.dom.get_particles(), ref[i].dom.get_particles()) <= 1e-12
+
+
+def test_batch_rejects_instance_without_particles():
+    """An instance whose domain holds no particles is refused with MITHRA_E_UNSUPPORTED (the
+    batched sort and deposit walk particle orders that only a bunch has); only the returned
+    status is checked, no batch runs."""
+    from golden_state import Case
+    case = Case("nsfd_o2")
+    dom = case.open("gpu")
+    dom.set_particles(np.zeros((0, 11)))
+    b = abi.Batch([dom])
+    with pytest.raises(abi.MithraError) as e:
+        b.instance(0, case.time0, case.time_bunch0, case.dtb, case.n_sub, 3, case.bunch, case.undulators)
+    print("empty instance:", e.value, flush=True)
+    assert "status -2" in str(e.value)
+    b.close()
