@@ -19,11 +19,14 @@
  *
  * Parity status of each part (what pins it):
  *   every routine above -- bit-exact against the reference's own dumps after 1 and 100
- *   steps for five jobs (NSFD/FD, Mur order 1/2, space charge on/off, Gaussian-beam and
+ *   steps for the golden jobs (NSFD/FD, Mur order 1/2, space charge on/off, Gaussian-beam and
  *   plane-wave seeds, undulator array, ellipsoid and file bunches): tests/test_oracle_golden.py
  *   plus closed-form checks in tests/test_oracle_invariants.py.
- *   Seed types truncated-plane-wave and super-gaussian-beam and the neumann, flat-top and
- *   inverse-gaussian signals: parity unpinned (no fixture exercises them).
+ *   oracle_signal and oracle_seed_fields, every signal type (neumann, gaussian, secant,
+ *   flat-top, inverse-gaussian) and every seed type (plane-wave, truncated-plane-wave,
+ *   Gaussian-beam, super-gaussian-beam with its accumulation into the caller's vector) --
+ *   bit-exact against known-answer tables the reference itself computes
+ *   (oracle/kat_harness.cpp -> tests/golden/kat_seed_signal.npz, tests/test_oracle_kat.py).
  */
 #include "cpu_ref.h"
 
