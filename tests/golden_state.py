@@ -13,7 +13,9 @@ import numpy as np
 from mithra_amd import abi
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+# the march fixtures: one per job file (other .npz files hold known-answer tables)
+CASES = sorted(f[:-4] for f in os.listdir(GOLDEN)
+               if f.endswith(".npz") and os.path.exists(os.path.join(GOLDEN, "jobs", f[:-4] + ".job")))
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(REPO, "oracle", "_build", "libcpu_ref.so")
 
