@@ -317,6 +317,15 @@ int mithra_mlp_gemm(const unsigned short* A, long sam, long sak, long a_len, con
  * lab-frame transform.  No counterpart in the reference (its radiation diagnostics only). */
 int mithra_dev_bunching(mithra_dev* dev, double k, double* b_out);
 
+/* mithra_dev_bunching without any device allocation: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Its per-workgroup partial sums (3 x 512 doubles, the same
+ * fixed grid as mithra_dev_bunching, so every particle count the library accepts fits) live in a
+ * static device array that each GPU holds until the process ends; b equals mithra_dev_bunching's
+ * bit for bit.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (several host threads, or handles on several streams of one GPU) race. */
+int mithra_dev_bunching_noalloc(mithra_dev* dev, double k, double* b_out);
+
 /* mithra_dev_step with HIP events recorded on the handle's stream between its phases; ms[i]
  * receives the duration of phase i: 0 interior update, 1 TF/SF, 2 Mur boundaries, 3 E/B
  * evaluation, 4 bunch sub-steps, 5 power sample, 6 current reset, 7 current deposition.

@@ -174,6 +174,7 @@ def _declare(lib, prefix):
         sig["pipeline_pending"] = (C.c_int, [vp])
         sig["set_fused_tile"] = (C.c_int, [vp, C.c_int])
         sig["autotune_fused"] = (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)])
+        sig["bunching_noalloc"] = (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)])
     fns = {}
     for name, (res, args) in sig.items():
         f = getattr(lib, prefix + name)
@@ -413,6 +414,13 @@ class Domain:
         """|sum q exp(i k z)| / |sum q| over the particles."""
         b = C.c_double()
         self._check(self.f["bunching"](self.h, k, C.byref(b)))
+        return b.value
+
+    def bunching_noalloc(self, k):
+        """bunching() through the entry point that allocates no device memory (device library
+        only)."""
+        b = C.c_double()
+        self._check(self.f["bunching_noalloc"](self.h, k, C.byref(b)))
         return b.value
 
 

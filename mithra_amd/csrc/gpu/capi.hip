@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -39,6 +40,36 @@ double bunching_from_partials(const double* h, int blocks) {
     q += h[3 * b + 2];
   }
   return q != 0.0 ? std::sqrt(c * c + s * s) / std::fabs(q) : 0.0;
+}
+
+// Device memory of mithra_dev_bunching_noalloc, which may not allocate: a file-scope array of
+// kBunchingScratch doubles that every GPU the library runs on holds until the process ends.
+// Its name is a host-side stand-in; the array's address on each GPU comes from
+// hipGetSymbolAddress with that GPU current, once, and only that address is handed to kernels.
+// Every call on one GPU shares that GPU's array, so calls that overlap on one GPU race.
+constexpr long kBunchingScratch = 3L * kBunchingBlocks;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kBunchingScratch];
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+
+int bunching_scratch(int device, double** out) {
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur >= 0 && cur != device) {
+      const hipError_t r = hipSetDevice(cur);
+      if (e == hipSuccess) e = r;
+    }
+    if (e != hipSuccess) return MITHRA_E_HIP - (int)e;
+    scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = scratch_ptr[device];
+  return MITHRA_OK;
 }
 
 bool valid_grid(const mithra_grid* g) {
@@ -663,6 +694,24 @@ int mithra_dev_bunching(mithra_dev* d, double k, double* b_out) {
   hipFree(partial);
   if (e != hipSuccess) return hip_status(e);
   *b_out = bunching_from_partials(h.data(), blocks);
+  return MITHRA_OK;
+}
+
+int mithra_dev_bunching_noalloc(mithra_dev* d, double k, double* b_out) {
+  if (!d || !b_out) return MITHRA_E_ARG;
+  // the partial sums of k_bunching: one per workgroup of the fixed grid, whatever the particle
+  // count, so every domain the library accepts fits
+  constexpr long need = 3L * kBunchingBlocks;
+  static_assert(need <= kBunchingScratch, "bunching partials exceed the static scratch");
+  if (d->device < 0 || d->device >= kMaxDevices) return MITHRA_E_ARG;
+  double* partial = nullptr;
+  if (int rc = bunching_scratch(d->device, &partial)) return rc;
+  MITHRA_TRY(launch_fill_zero(d, partial, sizeof(double) * need));
+  MITHRA_TRY(launch_bunching(d, k, partial, kBunchingBlocks));
+  double h[need];
+  MITHRA_TRY(hipMemcpyAsync(h, partial, sizeof h, hipMemcpyDeviceToHost, d->stream));
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  *b_out = bunching_from_partials(h, kBunchingBlocks);
   return MITHRA_OK;
 }
 
