@@ -151,6 +151,16 @@ int mithra_dev_set_particles(mithra_dev* dev, const double* charge11, long n);
 int mithra_dev_get_particles(mithra_dev* dev, double* charge11, long n);
 long mithra_dev_particle_count(const mithra_dev* dev);
 
+/* The analytic seed on the device at n listed points, as the TF/SF injection evaluates it.
+ * signal_evaluate: out[q] = Signal::self(t[q], phase[q]) (src/classes.cpp:534-575) for the
+ * seed's signal.  seed_evaluate: a[3q..3q+2] = Seed::fields(r[3q..3q+2], time[q])
+ * (src/classes.cpp:740-855), the boosted-frame vector potential at boosted-frame point r; each
+ * point on its own, so the super-gaussian type, whose reference form adds into the caller's
+ * previous output, returns MITHRA_E_UNSUPPORTED (as the field update does).  Host arrays in
+ * and out; the calls wait for the device. */
+int mithra_signal_evaluate(const mithra_seed* seed, const double* t, const double* phase, long n, double* out);
+int mithra_seed_evaluate(const mithra_seed* seed, const double* r, const double* time, long n, double* a);
+
 /* One field step: NSFD/FD interior update of A (and phi), TF/SF seed injection, Mur faces,
  * edges and corners, then E/B on every node a particle or sampler may read.
  * Replaces FdTd::fieldUpdate (src/fdtd.cpp:231-800) / FdTdSC::fieldUpdate

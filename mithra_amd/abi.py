@@ -128,6 +128,34 @@ def stream_bandwidth(nbytes=2 << 30, reps=10):
     return out.value
 
 
+def signal_evaluate(seed, t, phase):
+    """Signal::self of the seed's signal at times t with phases `phase`, on the device."""
+    lib = load_gpu_library()
+    f = lib.mithra_signal_evaluate
+    f.restype, f.argtypes = C.c_int, [C.POINTER(Seed), _dp, _dp, C.c_long, _dp]
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    phase = np.ascontiguousarray(phase, dtype=np.float64)
+    out = np.empty(t.size)
+    rc = f(C.byref(seed), _ptr(t), _ptr(phase), t.size, _ptr(out))
+    if rc != 0:
+        raise MithraError(f"mithra_signal_evaluate: status {rc}")
+    return out
+
+
+def seed_evaluate(seed, r, time):
+    """Seed::fields at boosted-frame points r (n x 3) and times (n), on the device; (n x 3)."""
+    lib = load_gpu_library()
+    f = lib.mithra_seed_evaluate
+    f.restype, f.argtypes = C.c_int, [C.POINTER(Seed), _dp, _dp, C.c_long, _dp]
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    out = np.empty((time.size, 3))
+    rc = f(C.byref(seed), _ptr(r), _ptr(time), time.size, _ptr(out))
+    if rc != 0:
+        raise MithraError(f"mithra_seed_evaluate: status {rc}")
+    return out
+
+
 def _declare(lib, prefix):
     vp = C.c_void_p
     sig = {

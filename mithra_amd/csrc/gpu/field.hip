@@ -6,6 +6,9 @@
 // potentials bit for bit.
 #include <algorithm>
 #include <cmath>
+#include <initializer_list>
+#include <utility>
+#include <vector>
 
 #include "common.h"
 
@@ -393,6 +396,24 @@ __global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid
     tfsf_apply<R>(A, base + N1N0 * (N2 - 2), x, y, g.zmin + (N2 - 3) * g.dz, time, s, coef, true);
     tfsf_apply<R>(A, base + N1N0 * (N2 - 3), x, y, g.zmin + (N2 - 2) * g.dz, time, s, coef, false);
   }
+}
+
+// The analytic seed on its own, at listed points: one thread per point.
+__global__ void k_signal_eval(mithra_seed s, const double* __restrict__ t, const double* __restrict__ phase, long n,
+                              double* __restrict__ out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = signal(s, t[q], phase[q]);
+}
+
+__global__ void k_seed_eval(mithra_seed s, const double* __restrict__ r, const double* __restrict__ time, long n,
+                            double* __restrict__ a) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double v[3];
+  seed_fields(s, r[3 * q], r[3 * q + 1], r[3 * q + 2], time[q], v);
+  a[3 * q] = v[0];
+  a[3 * q + 1] = v[1];
+  a[3 * q + 2] = v[2];
 }
 
 // ---- Mur absorbing boundaries (src/database.cpp:137-288, loops src/fdtd.cpp:377-725) ------
@@ -1371,3 +1392,68 @@ hipError_t launch_batch_mur(BatchView* V, BatchTiles faces, BatchTiles edges, Ba
 }
 
 }  // namespace mithra
+
+namespace {
+
+// Runs one of the analytic-seed kernels over n host points: inputs in (n_in arrays of per[i]
+// doubles per point), one output of per_out doubles per point.
+template <typename Launch>
+int eval_points(long n, std::initializer_list<std::pair<const double*, int>> in, int per_out, double* out,
+                Launch launch) {
+  if (n < 0 || (n > 0 && !out)) return MITHRA_E_ARG;
+  for (const auto& a : in)
+    if (n > 0 && !a.first) return MITHRA_E_ARG;
+  if (n == 0) return MITHRA_OK;
+  std::vector<double*> dev;
+  hipError_t e = hipSuccess;
+  auto cleanup = [&]() {
+    for (double* p : dev) (void)hipFree(p);
+  };
+  for (const auto& a : in) {
+    double* p = nullptr;
+    if ((e = hipMalloc(&p, sizeof(double) * a.second * n)) != hipSuccess) break;
+    dev.push_back(p);
+    if ((e = hipMemcpy(p, a.first, sizeof(double) * a.second * n, hipMemcpyHostToDevice)) != hipSuccess) break;
+  }
+  double* o = nullptr;
+  if (e == hipSuccess && (e = hipMalloc(&o, sizeof(double) * per_out * n)) == hipSuccess) dev.push_back(o);
+  if (e == hipSuccess) {
+    launch(dev.data(), o, (unsigned)((n + 255) / 256));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, o, sizeof(double) * per_out * n, hipMemcpyDeviceToHost);
+  cleanup();
+  if (e == hipErrorOutOfMemory) return MITHRA_E_NOMEM;
+  return e == hipSuccess ? MITHRA_OK : MITHRA_E_HIP - (int)e;
+}
+
+bool seed_supported(const mithra_seed* s) {
+  return s->seed_type == MITHRA_SEED_PLANEWAVE || s->seed_type == MITHRA_SEED_PLANEWAVE_TRUNCATED ||
+         s->seed_type == MITHRA_SEED_GAUSSIANBEAM;
+}
+
+bool signal_valid(const mithra_seed* s) { return s->signal_type >= MITHRA_SIGNAL_NEUMANN && s->signal_type <= MITHRA_SIGNAL_INVGAUSSIAN; }
+
+}  // namespace
+
+extern "C" {
+
+int mithra_signal_evaluate(const mithra_seed* seed, const double* t, const double* phase, long n, double* out) {
+  if (!seed || !signal_valid(seed)) return MITHRA_E_ARG;
+  const mithra_seed s = *seed;
+  return eval_points(n, {{t, 1}, {phase, 1}}, 1, out, [&](double** in, double* o, unsigned blocks) {
+    hipLaunchKernelGGL(mithra::k_signal_eval, dim3(blocks), dim3(256), 0, 0, s, in[0], in[1], n, o);
+  });
+}
+
+int mithra_seed_evaluate(const mithra_seed* seed, const double* r, const double* time, long n, double* a) {
+  if (!seed || !signal_valid(seed)) return MITHRA_E_ARG;
+  if (!seed_supported(seed)) return MITHRA_E_UNSUPPORTED;
+  const mithra_seed s = *seed;
+  return eval_points(n, {{r, 3}, {time, 1}}, 3, a, [&](double** in, double* o, unsigned blocks) {
+    hipLaunchKernelGGL(mithra::k_seed_eval, dim3(blocks), dim3(256), 0, 0, s, in[0], in[1], n, o);
+  });
+}
+
+}  // extern "C"

@@ -165,6 +165,10 @@ class Harness : public Base {
     o.d("seed_t0", s.seed_.signal_.t0_); o.d("seed_s", s.seed_.signal_.s_);
     o.d("seed_f0", s.seed_.signal_.f0_); o.d("seed_cep", s.seed_.signal_.cep_);
     o.i("seed_nR", (long long)s.seed_.signal_.nR_);
+    o.dv("seed_sigma_invg", s.seed_.signal_.sigmaInvG_.data(), s.seed_.signal_.sigmaInvG_.size());
+    std::vector<double> order(s.seed_.order_.begin(), s.seed_.order_.end());
+    order.resize(2, 0.0);
+    o.dv("seed_order", order.data(), 2);
     o.d("seed_gamma", s.seed_.gamma_); o.d("seed_beta", s.seed_.beta_);
     o.d("seed_dt", s.seed_.dt_); o.d("seed_c0", s.seed_.c0_);
     /* undulators after sorting/shifting */

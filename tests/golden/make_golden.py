@@ -4,7 +4,8 @@ Test infrastructure, run in the development container only (the reference source
 on the GPU machines).  It builds oracle/_ref/ref_harness (oracle/Makefile), runs it on every
 job in tests/golden/jobs/ and packs what it recorded into small compressed fixtures:
 
-  <case>.npz        state before step 0, 1 and N (A^n, A^{n-1}, J, phi, rho, particles, all
+  <case>.npz        state before step 0, 1 and N (N = 100, or the job's `# golden-steps: N`)
+                    (A^n, A^{n-1}, J, phi, rho, particles, all
                     update coefficients) and after the field update of steps 0 and 1
                     (A^{n+1}, E, B) -- see oracle/ref_harness.cpp for the exact hook points
   <case>_power.txt  the reference's own P(z) output file (radiation.cpp:222-229, precision 15)
@@ -31,8 +32,18 @@ PRE_KEYS = ("an", "anm1", "jn", "fn", "fnm1", "rn", "particles")
 POST_KEYS = ("anp1", "fnp1", "en", "bn")
 
 
-def run_case(job, steps=N_STEPS):
+def job_steps(job):
+    """Steps a fixture records: N_STEPS, or a `# golden-steps: N` line of the job file (the
+    reference's reader drops everything after '#', src/readdata.cpp:44)."""
+    for line in open(job):
+        if line.startswith("# golden-steps:"):
+            return int(line.split(":")[1])
+    return N_STEPS
+
+
+def run_case(job, steps=None):
     name = os.path.splitext(os.path.basename(job))[0]
+    steps = job_steps(job) if steps is None else steps
     work = tempfile.mkdtemp(prefix=f"golden_{name}_")
     try:
         shutil.copy(job, work)

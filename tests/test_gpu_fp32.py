@@ -5,13 +5,13 @@ must stay far inside the 2 % budget of the scan path."""
 import numpy as np
 import pytest
 
-from golden_state import CASES, Case, march, rel_l2
+from golden_state import GPU_CASES, Case, march, rel_l2
 from mithra_amd import abi
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", GPU_CASES)
 def test_fp32_bulk_error(name):
     print(f"[fp32] {name}", flush=True)
     case = Case(name)

@@ -8,7 +8,7 @@ import sys
 import numpy as np
 import pytest
 
-from golden_state import CASES, GOLDEN, Case
+from golden_state import GPU_CASES, GOLDEN, Case
 from mithra_amd import abi
 from mithra_amd.job import Job
 
@@ -33,7 +33,7 @@ def _compare(case, path, rows):
     assert np.max(np.abs(ours[:, 1] - ref[:, 1])) <= 1e-6 * scale
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", GPU_CASES)
 def test_job_run_matches_reference_power(name, tmp_path):
     print(f"[gpu job] {name}", flush=True)
     _stage(name, tmp_path)

@@ -10,13 +10,13 @@ the reference's, so those are held to the tolerance rather than to bit equality.
 import numpy as np
 import pytest
 
-from golden_state import CASES, Case, march, rel_l2
+from golden_state import DEVICE_REFUSED, GPU_CASES, Case, march, rel_l2
 from mithra_amd import abi
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=CASES)
+@pytest.fixture(scope="module", params=GPU_CASES)
 def case(request):
     print(f"[gpu parity] case {request.param}", flush=True)
     return Case(request.param)
@@ -176,3 +176,21 @@ def test_lazy_fields_change_no_result(case):
     p_l = lazy.power_fetch(0, 8, case.power_N, case.power_Nl)
     p_f = full.power_fetch(0, 8, case.power_N, case.power_Nl)
     assert rel_l2(p_l, p_f) < 1e-12
+
+
+@pytest.mark.parametrize("name", DEVICE_REFUSED)
+def test_device_refuses_super_gaussian_seed(name):
+    """The super-gaussian seed is refused with MITHRA_E_UNSUPPORTED by the field update and the
+    whole-step call, and leaves the domain usable; its oracle parity is in test_oracle_golden."""
+    case = Case(name)
+    assert case.seed.seed_type == 3 and abs(case.seed.amplitude) > 1e-50
+    gpu = case.open("gpu")
+    for call in (lambda: gpu.field_update(case.time0, case.seed),
+                 lambda: gpu.step(case.time0, case.seed, case.bunch, case.undulators, case.time_bunch0,
+                                  case.dtb, case.n_sub, 0, 0)):
+        with pytest.raises(abi.MithraError) as e:
+            call()
+        print("refused:", e.value, flush=True)
+        assert "status -2" in str(e.value)
+    a0, _ = gpu.get_potentials(0)
+    assert np.array_equal(a0, case.arr("pre0_an"))

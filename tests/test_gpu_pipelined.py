@@ -15,13 +15,13 @@ do.  Cases with space charge, and FP32-bulk domains, run the plain step whatever
 import numpy as np
 import pytest
 
-from golden_state import CASES, Case, rel_l2
+from golden_state import GPU_CASES, Case, rel_l2
 from mithra_amd import abi
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=CASES)
+@pytest.fixture(scope="module", params=GPU_CASES)
 def case(request):
     print(f"[gpu pipelined] case {request.param}", flush=True)
     return Case(request.param)
