@@ -98,16 +98,24 @@ FEL-OUTPUT
 """
 
 
-# Space-charge and seeded workload: 128 x 128 x 1024 nodes, 4,194,304 particles, NSFD, Mur order 2,
-# space charge (the phi update and grad phi in E) and a Gaussian-beam seed injected by TF/SF.  The
-# cells are those of the reference's CPU-runnable case (380 um x 380 um x 1.6 um lab at gamma
-# 196.7, the smallest transverse cell NSFD accepts there), the bunch a Gaussian of 100 pC spanning
-# about an eighth of the mesh along each axis: a throughput case for the space-charge and TF/SF
-# path, not a physics one (the mesh does not resolve the radiation).
+# Space-charge and seeded workload (C3): 128 x 128 x 1024 nodes, 4,194,304 particles of a 30 MeV
+# beam (gamma 59.7) with a 500 A peak current, Gaussian sigma_r 200 um, sigma_z 100 um, 0.2 % energy
+# spread, 1 mm mrad normalised emittance, in a planar undulator (lambda_u 2 cm, K 0.8), NSFD with
+# second-order Mur boundaries, space charge, and a 1 MW Gaussian-beam seed of 2 mm waist at the
+# resonant wavelength injected by TF/SF.  The mesh resolves the radiation (dz = lambda_r / 16 lab)
+# over 1022.5 cells, so it holds the middle 237 um of the bunch: the particles are loaded within
+# +-0.35 of the window (+-83 um) with the charge the 500 A Gaussian carries there.
 C3_NODES = (128, 128, 1024)
 C3_PARTICLES = 4_194_304
-C3_GAMMA = 196.7
-C3_DX, C3_DZ = 380.0, 1.6
+C3_GAMMA = 59.7
+C3_K, C3_LU = 0.8, 2.0e4
+C3_LAMBDA_R = C3_LU / (2.0 * C3_GAMMA ** 2) * (1.0 + C3_K ** 2 / 2.0)    # 3.70 um
+C3_DZ = C3_LAMBDA_R / 16.0
+C3_DX = 20.0             # above NSFD's 1.02 sqrt(2) gamma_frame dz = 17.4 um: the mesh stays as written
+C3_SIGMA_R, C3_SIGMA_Z = 200.0, 100.0
+C3_PEAK_A = 500.0
+C3_SEED_W0 = 2000.0
+C3_SEED_A0 = 1.263e-5    # e E0 lambda_r / (2 pi m c^2), E0 = sqrt(4 Z0 P / (pi w0^2)) for P = 1 MW
 C3_JOB = """MESH
 {{
   length-scale            = MICROMETER
@@ -141,7 +149,7 @@ FIELD
     polarization          = ( 0.0, 1.0, 0.0 )
     radius-parallel       = {rseed:.10g}
     radius-perpendicular  = {rseed:.10g}
-    strength-parameter    = 2.0e-5
+    strength-parameter    = {a0:.10g}
     signal-type           = gaussian
     offset                = 0.0
     pulse-length          = 1.0e12
@@ -153,8 +161,8 @@ UNDULATOR
 {{
   static-undulator
   {{
-    undulator-parameter   = 1.0
-    period                = 1.5e4
+    undulator-parameter   = {K:.10g}
+    period                = {lu:.10g}
     length                = 100
     polarization-angle    = 0.0
   }}
@@ -175,30 +183,45 @@ FEL-OUTPUT
 
 
 def c3_bunch(n, seed=3):
-    """Lab-frame records q x y z gbx gby gbz (um) of the Gaussian bunch of the C3 workload."""
+    """Lab-frame records q x y z gbx gby gbz (um) of the C3 bunch: the part of the 500 A Gaussian
+    inside +-0.35 of the mesh's z window, with its charge."""
+    import math
     rng = np.random.default_rng(seed)
-    lx = (C3_NODES[0] - 1.5) * C3_DX
     lz = (C3_NODES[2] - 1.5) * C3_DZ
-    electrons = 100e-12 / 1.602e-19
+    half = 0.35 * lz
+    z = rng.normal(0.0, C3_SIGMA_Z, 2 * n)
+    z = z[np.abs(z) <= half][:n]
+    while z.size < n:  # the truncated Gaussian keeps about half of the draws
+        extra = rng.normal(0.0, C3_SIGMA_Z, 2 * n)
+        z = np.concatenate([z, extra[np.abs(extra) <= half]])[:n]
+    inside = math.erf(half / (math.sqrt(2.0) * C3_SIGMA_Z))
+    coulomb = C3_PEAK_A / 2.99792458e8 * math.sqrt(2.0 * math.pi) * C3_SIGMA_Z * 1e-6 * inside
+    electrons = coulomb / 1.602e-19
+    sgb = 1.0e-6 / (C3_SIGMA_R * 1e-6)                     # emittance / sigma_r
     rec = np.empty((n, 7))
     rec[:, 0] = electrons / n
-    rec[:, 1] = rng.normal(0.0, lx / 16.0, n)
-    rec[:, 2] = rng.normal(0.0, lx / 16.0, n)
-    rec[:, 3] = rng.normal(0.0, lz / 16.0, n)
-    rec[:, 4] = rng.normal(0.0, 1e-3, n)
-    rec[:, 5] = rng.normal(0.0, 1e-3, n)
-    rec[:, 6] = np.sqrt(C3_GAMMA ** 2 - 1.0) * (1.0 + 1e-3 * rng.normal(0.0, 1.0, n))
+    rec[:, 1] = rng.normal(0.0, C3_SIGMA_R, n)
+    rec[:, 2] = rng.normal(0.0, C3_SIGMA_R, n)
+    rec[:, 3] = z
+    rec[:, 4] = rng.normal(0.0, sgb, n)
+    rec[:, 5] = rng.normal(0.0, sgb, n)
+    rec[:, 6] = np.sqrt(C3_GAMMA ** 2 - 1.0) * (1.0 + 2e-3 * rng.normal(0.0, 1.0, n))
     return rec, electrons
 
 
-def set_up_c3(workdir):
-    log("generating the C3 bunch (space charge, Gaussian seed) and running the host set-up")
-    bunch, electrons = c3_bunch(C3_PARTICLES)
+def c3_job_text(workdir, electrons, n):
     lx, lz = (C3_NODES[0] - 1.5) * C3_DX, (C3_NODES[2] - 1.5) * C3_DZ   # (int)(L / d) + 2 nodes
+    return C3_JOB.format(lx=lx, lz=lz, dx=C3_DX, dz=C3_DZ, charge=electrons, n=n, gamma=C3_GAMMA,
+                         zseed=lz / 4.0, rseed=C3_SEED_W0, a0=C3_SEED_A0, wseed=C3_LAMBDA_R, K=C3_K, lu=C3_LU,
+                         zpower=lz / 8.0, dir=workdir)
+
+
+def set_up_c3(workdir, n=C3_PARTICLES):
+    log("generating the C3 bunch (space charge, Gaussian seed) and running the host set-up")
+    bunch, electrons = c3_bunch(n)
     path = os.path.join(workdir, "c3.job")
     with open(path, "w") as f:
-        f.write(C3_JOB.format(lx=lx, lz=lz, dx=C3_DX, dz=C3_DZ, charge=electrons, n=C3_PARTICLES, gamma=C3_GAMMA,
-                              zseed=lz / 4.0, rseed=lx / 4.0, wseed=20.0 * C3_DZ, zpower=lz / 8.0, dir=workdir))
+        f.write(c3_job_text(workdir, electrons, n))
     job = Job(path, external_bunch=bunch)
     del bunch
     return job
@@ -495,8 +518,8 @@ def main():
         "config": {"workload": "production single instance (256x256x2048 nodes, 16.8M macro-particles, "
                                "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)"
                                if args.workload == "production" else
-                               "C3: 128x128x1024 nodes, 4.2M macro-particles, 100 MeV, K 1, lambda_u 1.5 cm, NSFD, "
-                               "Mur order 2, space charge, Gaussian-beam seed (TF/SF)",
+                               "C3: 128x128x1024 nodes, 4.2M macro-particles, 30 MeV, 500 A peak, K 0.8, lambda_u "
+                               "2 cm, NSFD, Mur order 2, space charge, 1 MW Gaussian-beam seed at lambda_r (TF/SF)",
                    "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
                    "precision": args.precision, "parallelism": "single GPU, whole domain resident",
                    "reorder_interval": args.reorder,

@@ -88,3 +88,95 @@ def test_batch_rejects_instance_without_particles():
     print("empty instance:", e.value, flush=True)
     assert "status -2" in str(e.value)
     b.close()
+
+
+def oracle_scan_point(p, lib):
+    """One prepared scan instance marched by the oracle (the reference's loop body, FP64): P(z) and
+    the bunching factor at the instance's sample steps, and the final particle records."""
+    ora = abi.Domain(p.job.grid(), p.job.coeffs(), lib=lib, prefix="oracle_")
+    for lv in (0, 1):
+        a, phi = p.job.potentials(lv)
+        ora.set_potentials(lv, a, phi)
+    ora.set_particles(p.job.particles())
+    N, Nl, Nf, pc, z, wv = p.job.power()
+    ora.power_setup(abi.make_power_params(z, wv, Nf, pc, p.bp.gamma, p.bp.beta, p.bp.c0), p.inst.n_steps)
+    t = dict(p.t)
+    time_, tb = t["time"], t["time_bunch"]
+    bunch = []
+    for s in range(p.inst.n_steps):
+        ora.field_update(time_, p.seed)
+        ora.bunch_begin()
+        for _ in range(t["n_sub"]):
+            ora.bunch_update(p.bp, p.und, tb)
+            tb += t["bunch_time_step"]
+        if s % p.inst.every == 0 and len(bunch) < scan.SAMPLES:
+            bunch.append(ora.bunching(p.k))
+        ora.power_sample(s, s)
+        ora.field_shift()
+        ora.current_reset()
+        ora.current_update()
+        time_ += t["dt"]
+    idx = np.arange(scan.SAMPLES) * p.inst.every
+    idx = idx[idx < p.inst.n_steps]
+    out = ora.power_fetch(0, p.inst.n_steps, 1, 1)[:, 0, 0][idx], np.array(bunch)[:len(idx)], ora.get_particles()
+    ora.close()
+    return out
+
+
+def test_batch_against_oracle():
+    """Each instance of a small FP64 scan, marched as one batch, against the oracle's call-by-call
+    march of the reference's loop body on the same set-up: P(z) and the bunching factor at every
+    sample step, and the final particles, within 1e-10 (relative L2; bunching to 1e-10 of its
+    peak)."""
+    from golden_state import load_oracle
+    insts = small_scan(4, seed=13)
+    lib = load_oracle()
+    with tempfile.TemporaryDirectory() as w:
+        prep = scan.prepare(insts, w, abi.PRECISION_FP64)
+        res_b, _, _ = scan.run_batch(prep, autotune=False)
+        for i, p in enumerate(prep):
+            P_o, b_o, q_o = oracle_scan_point(p, lib)
+            P_b, b_b = res_b[i]
+            q_b = p.dom.get_particles()
+            print(f"instance {i} ({p.inst.n_perp}x{p.inst.n_perp}x{p.inst.n_z}, {p.inst.n_particles} particles, "
+                  f"{p.inst.n_steps} steps): dP {rel_l2(P_b, P_o):.2e} db "
+                  f"{np.max(np.abs(b_b - b_o)) / np.max(np.abs(b_o)):.2e} dparticles "
+                  f"{rel_l2(q_b[:, 1:10], q_o[:, 1:10]):.2e}", flush=True)
+            assert np.max(np.abs(P_o)) > 0.0
+            assert rel_l2(P_b, P_o) <= 1e-10
+            assert np.max(np.abs(b_b - b_o)) <= 1e-10 * np.max(np.abs(b_o))
+            assert rel_l2(q_b[:, 1:10], q_o[:, 1:10]) <= 1e-10
+        for p in prep:
+            p.dom.close()
+            p.job.close()
+
+
+def test_c4_scan_at_size_against_oracle():
+    """The bench's 64-instance scan (seed 4: transverse 32-128, longitudinal 128-1024 cells, 1e4-1e6
+    particles) as one FP32-bulk batch; every instance yields finite P(z) and bunching at its sample
+    steps, and the three smallest instances, re-solved by the FP64 oracle, are within the scan's
+    2 % P(z) budget (relative L1, the reference's metric)."""
+    from golden_state import load_oracle
+    insts = scan.generate(64, seed=4)
+    lib = load_oracle()
+    lib.oracle_set_threads(8)
+    with tempfile.TemporaryDirectory() as w:
+        prep = scan.prepare(insts, w, abi.PRECISION_FP32_BULK)
+        res_b, seconds, tuned = scan.run_batch(prep)
+        print(f"C4 batch: 64 instances marched in {seconds:.2f} s (tile variant {tuned[0]})", flush=True)
+        for P, b in res_b:
+            assert np.all(np.isfinite(P)) and np.all(np.isfinite(b)) and len(P) == len(b) > 0
+        cost = [i.n_perp ** 2 * i.n_z * i.n_steps + 50 * i.n_particles * i.n_steps for i in insts]
+        for i in np.argsort(cost)[:3]:
+            p = prep[i]
+            P_o, b_o, _ = oracle_scan_point(p, lib)
+            P_b, b_b = res_b[i]
+            l1 = float(np.sum(np.abs(P_b - P_o)) / np.sum(np.abs(P_o)))
+            db = float(np.max(np.abs(b_b - b_o)) / np.max(np.abs(b_o)))
+            print(f"C4 instance {i} ({p.inst.n_perp}x{p.inst.n_perp}x{p.inst.n_z}, {p.inst.n_particles} particles): "
+                  f"FP32-bulk batch vs FP64 oracle P(z) L1 {l1:.2e}, bunching {db:.2e} of its peak", flush=True)
+            assert np.sum(np.abs(P_o)) > 0.0
+            assert l1 <= 0.02
+        for p in prep:
+            p.dom.close()
+            p.job.close()

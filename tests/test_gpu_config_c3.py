@@ -1,0 +1,120 @@
+"""The space-charge and seeded configuration at its size: 128 x 128 x 1024 nodes, 4,194,304 particles of
+a 30 MeV, 500 A Gaussian beam (sigma_r 200 um, sigma_z 100 um, 0.2 % spread), K 0.8, lambda_u 2 cm,
+a 1 MW Gaussian-beam seed of 2 mm waist at the resonant wavelength (bench.py's C3 workload, through
+the drop-in host set-up).
+
+1. FP64 against the oracle: the plain GPU step (space charge takes the plain step) and the oracle's
+   call-by-call march of the reference's loop body for 3 steps from the same set-up.  Bar: every
+   level of A and phi, J, rho and the particles within 1e-12 relative L2, P(z) within 1e-10 of its
+   peak.  The seed's analytic field uses the device's libm and the deposit sums in another order,
+   both far inside the bar.
+2. FP32-bulk against FP64 on the GPU over 200 steps: P(z) relative L1 error (the reference's metric,
+   doc/MITHRA_EXAMPLES/MITHRA_EXAMPLES.tex:182-185) and the bunching factor's error, printed and held
+   to the 2 % P(z) budget of the scan path with a wide margin."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from golden_state import load_oracle, rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_STEPS = 3
+FP32_STEPS = 200
+
+
+@pytest.fixture(scope="module")
+def c3(tmp_path_factory):
+    import bench
+    work = str(tmp_path_factory.mktemp("c3"))
+    job = bench.set_up_c3(work)
+    g = job.grid()
+    assert (g.N0, g.N1, g.N2) == (128, 128, 1024) and g.space_charge == 1
+    assert job.particles().shape[0] == 4_194_304
+    yield job
+    job.close()
+
+
+def plain_march(dom, m, n, k=None):
+    """n plain steps (the reference's loop body in one call each); the bunching factor after each
+    step's push when k is given."""
+    b = []
+    for _ in range(n):
+        dom.step(m.time, m.seed, m.bp, m.und, m.tb, m.t["bunch_time_step"], m.t["n_sub"], m.n, m.n)
+        if k is not None:
+            b.append(dom.bunching(k))
+        m.advance()
+    return np.array(b)
+
+
+def test_c3_fp64_against_oracle(c3):
+    import bench
+    from mithra_amd import abi
+
+    lib = load_oracle()
+    lib.oracle_set_threads(min(16, os.cpu_count() or 1))
+    gpu = bench.open_domain(c3, abi.PRECISION_FP64, rows=ORACLE_STEPS)
+    ora = bench.open_domain(c3, abi.PRECISION_FP64, lib=lib, prefix="oracle_", rows=ORACLE_STEPS)
+    mg, mo = bench.March(c3), bench.March(c3)
+    assert abs(mg.seed.amplitude) > 1e-50 and mg.seed.seed_type == 2
+    plain_march(gpu, mg, ORACLE_STEPS)
+    for s in range(ORACLE_STEPS):
+        ora.field_update(mo.time, mo.seed)
+        ora.bunch_begin()
+        tb = mo.tb
+        for _ in range(mo.t["n_sub"]):
+            ora.bunch_update(mo.bp, mo.und, tb)
+            tb += mo.t["bunch_time_step"]
+        ora.power_sample(mo.n, mo.n)
+        ora.field_shift()
+        ora.current_reset()
+        ora.current_update()
+        mo.advance()
+        print(f"oracle step {s} done", flush=True)
+    out = {}
+    for lv, name in ((0, "n"), (1, "n-1"), (2, "J/rho")):
+        (ag, fg), (ao, fo) = gpu.get_potentials(lv), ora.get_potentials(lv)
+        out[f"A {name}"] = rel_l2(ag, ao)
+        out[f"phi {name}"] = rel_l2(fg, fo)
+    pg, po = gpu.get_particles(), ora.get_particles()
+    out["particles"] = rel_l2(pg[:, 1:10], po[:, 1:10])
+    P_g = gpu.power_fetch(0, ORACLE_STEPS, 1, 1)[:, 0, 0]
+    P_o = ora.power_fetch(0, ORACLE_STEPS, 1, 1)[:, 0, 0]
+    out["P"] = float(np.max(np.abs(P_g - P_o)) / max(np.max(np.abs(P_o)), 1e-300))
+    print(f"C3 FP64, GPU vs oracle after {ORACLE_STEPS} steps: {out}; P {P_o}", flush=True)
+    assert np.max(np.abs(P_o)) > 0.0
+    assert np.array_equal(pg[:, 10], po[:, 10])
+    for key, v in out.items():
+        assert v <= (1e-10 if key == "P" else 1e-12), key
+    gpu.close()
+    ora.close()
+
+
+def test_c3_fp32_bulk_error(c3):
+    import bench
+    from mithra_amd import abi, scan
+
+    k = scan.bunching_wavenumber(c3)
+    runs = {}
+    for name, prec in (("fp64", abi.PRECISION_FP64), ("fp32", abi.PRECISION_FP32_BULK)):
+        dom = bench.open_domain(c3, prec, rows=FP32_STEPS)
+        dom.set_lazy_fields(1)
+        b = plain_march(dom, bench.March(c3), FP32_STEPS, k)
+        P = dom.power_fetch(0, FP32_STEPS, 1, 1)[:, 0, 0]
+        runs[name] = (P, b)
+        dom.close()
+        print(f"C3 {name}: {FP32_STEPS} steps, P[-1] {P[-1]:.6e}, b[-1] {b[-1]:.6e}", flush=True)
+    (P64, b64), (P32, b32) = runs["fp64"], runs["fp32"]
+    p_l1 = float(np.sum(np.abs(P32 - P64)) / np.sum(np.abs(P64)))
+    b_err = float(np.max(np.abs(b32 - b64)))
+    b_rel = float(b_err / np.max(np.abs(b64)))
+    print(f"C3 FP32-bulk vs FP64 over {FP32_STEPS} steps: P(z) relative L1 {p_l1:.3e}, bunching max abs "
+          f"error {b_err:.3e} ({b_rel:.3e} of its peak)", flush=True)
+    assert np.sum(np.abs(P64)) > 0.0
+    assert p_l1 < 1e-3
+    assert b_rel < 1e-3
