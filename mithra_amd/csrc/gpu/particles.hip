@@ -525,7 +525,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
                                          const unsigned* __restrict__ p_row, const unsigned* __restrict__ p_idx,
                                          const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
                                          int TI, int tilesI, int KC, int accumulate, double src_coef,
-                                         int block) {
+                                         int skip_empty, int block) {
   extern __shared__ double tile[];  // [2][ncomp][TI][N1]
   constexpr int ncomp = SC ? 4 : 3;
   const int chunk = block / tilesI;
@@ -550,6 +550,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
   start_plane(kb);
   if (kb + 1 < ke) start_plane(kb + 1);
   __syncthreads();
+  unsigned prev_total = 0;  // halves listed under the previous cell plane (this tile's rows)
   for (int kc = kb - 1; kc < ke; ++kc) {
     // the two ranges of cell plane kc: halves 0 (rnp order) and 1 (rnm order), rows lo..hi
     unsigned s0 = 0, n0 = 0, s1 = 0, n1 = 0;
@@ -578,11 +579,16 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
       if (b.ok) deposit_item<SC>(b, c, tile, psz, TI, kb, ke, i0, ti, lo, hi);
     }
     __syncthreads();
+    // Node plane kc takes halves of cell planes kc-1 and kc only.  When neither lists any, its J
+    // is +0, and in the pipelined form X + a4 J = X bit for bit when a4 carries a minus sign
+    // (a4 (+0) = -0, and x + -0 = x for every x, -0 included), so that plane is left untouched.
+    const bool untouched = SRC && skip_empty && prev_total + total == 0;
+    prev_total = total;
     if (kc >= kb) {  // node plane kc is complete: write it, and its slot becomes plane kc+2
       double* p = tile + (kc & 1) * psz;
       const long base = N1N0 * kc + (long)N1 * i0;
       const bool next = kc + 2 < ke;
-      const bool inner_plane = kc >= 1 && kc <= c.N2 - 2;
+      const bool inner_plane = kc >= 1 && kc <= c.N2 - 2 && !untouched;
       // one node per thread and pass, its components unrolled: no index division per element
       for (int o = threadIdx.x; o < own; o += blockDim.x) {
         bool inner = false;
@@ -614,9 +620,9 @@ template <typename R, bool SC, bool SRC = false>
 __global__ __launch_bounds__(kDepositThreads, SC ? 4 : 8) void k_deposit_rows(
     const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
     const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-    int TI, int tilesI, int KC, int accumulate, double src_coef) {
+    int TI, int tilesI, int KC, int accumulate, double src_coef, int skip_empty) {
   deposit_rows_body<R, SC, SRC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, KC, accumulate, src_coef,
-                                blockIdx.x);
+                                skip_empty, blockIdx.x);
 }
 
 // Bunching factor partial sums: per workgroup sum of q cos(k z), q sin(k z), q over its particles.
@@ -836,7 +842,7 @@ __global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* _
       const CellOrder mo = v.ord[1 - v.parity];
       const DepositConst c = v.dep;  // a local copy: the LDS atomics cannot alias it
       deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
-                                  v.dtilesI, kDepositChunk, 0, 0.0, e.y);
+                                  v.dtilesI, kDepositChunk, 0, 0.0, 0, e.y);
     }
     __syncthreads();  // the LDS tile is reused by the next entry
   }
@@ -1079,24 +1085,25 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   const dim3 grid((unsigned)deposit_blocks(g.N2, tilesI, KC));
   const unsigned *pr = d->ord[0].row, *pi = d->ord[0].idx, *mr = d->ord[1].row, *mi = d->ord[1].idx;
   const bool f64 = d->precision == MITHRA_PRECISION_FP64;
+  const int skip = std::signbit(src_coef) ? 1 : 0;  // see deposit_rows_body: exact only then
   if (accumulate == kDepositSource && f64)
     hipLaunchKernelGGL((k_deposit_rows<double, false, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part,
-                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef);
+                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   else if (accumulate == kDepositSource)
     hipLaunchKernelGGL((k_deposit_rows<float, false, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part,
-                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef);
+                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   else if (f64 && c.sc)
     hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   else if (f64)
     hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   else if (c.sc)
     hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   else
     hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef);
+                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // Periodic reorder of the particle arrays into the rnp row order.
   if (d->sort_interval > 0 && d->sort_count % d->sort_interval == 0)

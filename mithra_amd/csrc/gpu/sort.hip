@@ -4,14 +4,15 @@
 // few x-rows of the mesh, and needs every particle half whose cell lies in the adjacent cell
 // rows.  An "order" is the list of particle indices sorted by the row (k, i) of one half's cell,
 // plus the start of each row in that list; a workgroup then reads two contiguous ranges per
-// half.  Sorting is a radix sort of 32-bit (row, index) pairs.
+// half.  Sorting is rocPRIM's LSD radix sort of 32-bit (row, index) pairs over the key's
+// significant bits only; it is stable, so the halves of one row keep their particle-array order.
 //
 // The particle arrays themselves are permuted into row order now and then, so the sorted
 // indices walk memory almost sequentially; the original index travels with each particle in
 // `pid`, and the host always sees the records in load order.
 #include "common.h"
 
-#include <hipcub/device/device_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace mithra {
 namespace {
@@ -97,9 +98,9 @@ hipError_t ensure_orders(Domain* d, unsigned rows) {
 hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows) {
   int bits = 1;
   while (bits < 32 && (rows >> bits) != 0) ++bits;  // the sentinel `rows` must fit
-  hipcub::DoubleBuffer<unsigned> k(o.key, d->key_alt), v(o.idx, d->idx_alt);
+  rocprim::double_buffer<unsigned> k(o.key, d->key_alt), v(o.idx, d->idx_alt);
   size_t bytes = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)n, 0, bits, d->stream);
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, k, v, (unsigned)n, 0, bits, d->stream);
   if (e != hipSuccess) return e;
   if (bytes > d->sort_tmp_bytes || !d->sort_tmp) {
     (void)hipFree(d->sort_tmp);
@@ -109,15 +110,15 @@ hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows) {
     d->sort_tmp_bytes = bytes;
   }
   bytes = d->sort_tmp_bytes;
-  if ((e = hipcub::DeviceRadixSort::SortPairs(d->sort_tmp, bytes, k, v, (int)n, 0, bits, d->stream)) != hipSuccess)
+  if ((e = rocprim::radix_sort_pairs(d->sort_tmp, bytes, k, v, (unsigned)n, 0, bits, d->stream)) != hipSuccess)
     return e;
-  if (k.Current() != o.key) {  // keep the sorted pairs in the order's own arrays
+  if (k.current() != o.key) {  // keep the sorted pairs in the order's own arrays
     d->key_alt = o.key;
-    o.key = k.Current();
+    o.key = k.current();
   }
-  if (v.Current() != o.idx) {
+  if (v.current() != o.idx) {
     d->idx_alt = o.idx;
-    o.idx = v.Current();
+    o.idx = v.current();
   }
   hipLaunchKernelGGL(k_row_start, dim3((unsigned)(((long)rows + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, n,
                      rows, o.row);

@@ -111,3 +111,18 @@ def test_no_particles_gives_zero_current():
     dom.current_update()
     j, _ = dom.get_potentials(2)
     assert not np.any(j)
+
+
+@pytest.mark.parametrize("accumulate", [False, True], ids=["replace", "accumulate"])
+def test_deposit_full_chunks_space_charge(accumulate):
+    """A mesh large enough that the deposit keeps its full 4-plane chunks (no shrinking to fill the
+    GPU; 2-row tiles for 300-node rows with rho): J and rho against the oracle, written once and
+    added a second time without a reset (the ring slots of later planes reload the buffer)."""
+    g = grid(64, 300, 256, sc=1)
+    rec = particles(g, 100_000, seed=23)
+    print(f"deposit full chunks sc, accumulate={accumulate}", flush=True)
+    jg, rg = deposit(g, rec, "gpu", twice=accumulate)
+    jo, ro = deposit(g, rec, "oracle", twice=accumulate)
+    assert np.count_nonzero(jo) > 0 and np.count_nonzero(ro) > 0
+    assert rel_l2(jg, jo) < 1e-13
+    assert rel_l2(rg, ro) < 1e-13
