@@ -424,6 +424,61 @@ def run_cpu_baseline(job, steps, threads):
     return elapsed, used
 
 
+def run_reference_baseline(workload, ranks, steps, work):
+    """Seconds per step of the reference itself: oracle/_ref/ref_harness, the MITHRA reference
+    built from its sources with a step clock around its own loop, on `ranks` MPI ranks (its z-slab
+    decomposition, src/solver.cpp:619-641) over a job of oracle/calibrate_cpu.py's workloads
+    (its own Gaussian-ellipsoid bunch generator).  None when the build or MPICH is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import calibrate_cpu
+    if not (os.path.exists(calibrate_cpu.HARNESS) and os.path.exists(calibrate_cpu.MPIEXEC)):
+        return None
+    w = calibrate_cpu.WORKLOADS[workload]
+    d = tempfile.mkdtemp(prefix="ref_", dir=work)
+    t0 = time.perf_counter()
+    s = calibrate_cpu.time_reference(d, calibrate_cpu.job_text(**w), ranks, steps, timeout=600)
+    log(f"reference {workload} on {ranks} rank(s): {s:.3f} s/step ({time.perf_counter() - t0:.1f} s with set-up)")
+    return s, w["nodes"][0] ** 2 * w["nodes"][1], w["n"]
+
+
+def cpu_baseline_leg(job, args, nodes, n_part, n_sub, work):
+    """The CPU baseline beside the GPU figure, on this host's cores (rank 0, N = 1): the reference
+    itself on the production mesh over args.cpu_threads MPI ranks, its one-core rate on an eighth
+    of that mesh, and the CPU oracle (this build's C++ restatement) on the same threads."""
+    g = job.grid()
+    out = None
+    if args.workload == "production":
+        log(f"CPU baseline: the reference on {args.cpu_threads} MPI ranks, {args.cpu_steps} steps")
+        ref = run_reference_baseline("c2", args.cpu_threads, args.cpu_steps, work)
+        if ref:
+            s, rn, rp = ref
+            out = {"value": rn / s, "unit": "cell-updates/s", "cores": args.cpu_threads, "ranks": args.cpu_threads,
+                   "kind": "reference",
+                   "sample": f"the MITHRA reference built from source (g++ -O3, MPICH), {args.cpu_steps} steps after "
+                             f"one untimed, {args.cpu_threads} MPI ranks (z slabs) on the production mesh "
+                             f"{g.N0}x{g.N1}x{g.N2} with a {rp}-particle bunch from its own Gaussian-ellipsoid "
+                             f"generator (in place of the flat-top slice), {s:.3f} s/step",
+                   "particle_pushes_per_s": rp * n_sub / s}
+            one = run_reference_baseline("c2_slice", 1, 2, work)
+            if one:
+                s1, n1, p1 = one
+                out["one_core"] = {"value": n1 / s1, "unit": "cell-updates/s", "cores": 1, "ranks": 1,
+                                   "sample": f"the reference on one core, 2 steps of the production cells over "
+                                             f"256 of the 2048 planes ({n1} nodes, {p1} particles), "
+                                             f"{s1:.3f} s/step"}
+    log(f"CPU oracle sample: {args.cpu_steps} steps on {args.cpu_threads} threads")
+    cpu_t, used = run_cpu_baseline(job, args.cpu_steps, args.cpu_threads)
+    port = {"value": nodes * args.cpu_steps / cpu_t, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": f"{args.cpu_steps} full steps of the same {g.N0}x{g.N1}x{g.N2} mesh and {n_part}-particle "
+                      f"bunch with the CPU oracle (C++ -O3, OpenMP over nodes and particles, sequential "
+                      f"deposition), {cpu_t:.1f} s",
+            "particle_pushes_per_s": n_part * n_sub * args.cpu_steps / cpu_t}
+    if out is None:
+        return port
+    out["port"] = port
+    return out
+
+
 def traffic_from_profiles(kernel, precision):
     """HBM bytes per launch of `kernel` (interior, stencil_eb) from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on the stream probe),
@@ -499,14 +554,7 @@ def main():
         stream_gbs = abi.stream_bandwidth()
         cpu = None
         if args.cpu_steps > 0:
-            log(f"CPU oracle sample: {args.cpu_steps} steps on {args.cpu_threads} threads")
-            cpu_t, used = run_cpu_baseline(job, args.cpu_steps, args.cpu_threads)
-            cpu = {"value": nodes * args.cpu_steps / cpu_t, "unit": "cell-updates/s", "cores": used,
-                   "kind": "port",
-                   "sample": f"{args.cpu_steps} full steps of the same {g.N0}x{g.N1}x{g.N2} mesh and "
-                             f"{n_part}-particle bunch with the CPU oracle (C++ -O3, OpenMP over nodes and "
-                             f"particles, sequential deposition), {cpu_t:.1f} s",
-                   "particle_pushes_per_s": n_part * n_sub * args.cpu_steps / cpu_t}
+            cpu = cpu_baseline_leg(job, args, nodes, n_part, n_sub, work)
     out = {
         "metric": METRIC,
         "value": cell_rate,

@@ -22,8 +22,11 @@
  * Environment:
  *   MITHRA_DUMP_DIR    output directory (default ".")
  *   MITHRA_DUMP_STEPS  comma separated step numbers to dump (pre and post)
- *   MITHRA_MAX_STEPS   stop (exit 0) when nTime_ reaches this value, after
- *                      writing the pre dump of that step.
+ *   MITHRA_MAX_STEPS   stop (exit 0) when nTime_ reaches this value (after the pre
+ *                      dump of that step when it is listed in MITHRA_DUMP_STEPS).
+ *   MITHRA_STEP_CLOCK  file to which rank 0 writes "nTime seconds" at the start of every
+ *                      step (MPI_Wtime after a barrier of all ranks), for timing the
+ *                      reference's own steps without its set-up; no dumps are needed.
  *
  * Usage: ref_harness <job-file>   (single MPI rank, the one-domain case)
  */
@@ -93,6 +96,14 @@ long max_steps() {
   return e ? std::atol(e) : -1;
 }
 
+FILE* step_clock() {
+  const char* e = std::getenv("MITHRA_STEP_CLOCK");
+  if (!e) return nullptr;
+  int rank = 0;
+  MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+  return rank == 0 ? std::fopen(e, "w") : nullptr;
+}
+
 }  // namespace
 
 /* On one rank Solver::bunchUpdate() sends the migrating-particle buffers to itself with a
@@ -123,6 +134,8 @@ class Harness : public Base {
   std::set<unsigned> steps_ = parse_steps();
   std::string dir_ = dump_dir();
   long maxSteps_ = max_steps();
+  bool clock_ = std::getenv("MITHRA_STEP_CLOCK") != nullptr;
+  FILE* clockFile_ = step_clock();
 
   void dumpCommon(Dump& o) {
     Solver& s = *this;
@@ -241,9 +254,14 @@ class Harness : public Base {
 
   void fieldUpdate() override {
     Solver& s = *this;
+    if (clock_) {
+      MPI_Barrier(MPI_COMM_WORLD);
+      if (clockFile_) std::fprintf(clockFile_, "%u %.9f\n", s.nTime_, MPI_Wtime());
+    }
     bool dump = steps_.count(s.nTime_) > 0;
-    if (dump || (maxSteps_ >= 0 && (long)s.nTime_ >= maxSteps_)) dumpPre();
+    if (dump) dumpPre();
     if (maxSteps_ >= 0 && (long)s.nTime_ >= maxSteps_) {
+      if (clockFile_) std::fclose(clockFile_);
       std::fflush(nullptr);
       MPI_Finalize();
       std::exit(0);
