@@ -8,7 +8,8 @@ boundaries, no space charge.  As in the reference's own LCLS-scale example
 lambda_r / 23) and holds a slice of the flat-top bunch: 0.28 um of it, loaded by a seeded
 quiet start (four beamlets lambda_r / 4 apart, shot-noise modulation per lambda_r bucket),
 3 kA.  Particles are synthetic; the set-up runs through the drop-in host (job-file parser,
-Lorentz boost, coefficients) exactly as `mithra job.job` does.
+Lorentz boost, coefficients) exactly as `mithra job.job` does, with the bunch generated and boosted
+on the device.
 
 One step = one iteration of Solver::solve()'s main loop on the GPU: field update (interior,
 Mur, E/B), bunch push, P(z) sample, shift, current reset, deposition.  By default the steps run
@@ -227,35 +228,22 @@ def set_up_c3(workdir, n=C3_PARTICLES):
     return job
 
 
-def quiet_start_bunch(n, seed=2):
-    """Lab-frame records q x y z gbx gby gbz (lengths in um) of the flat-top slice."""
-    rng = np.random.default_rng(seed)
+def production_bunch(seed=2):
+    """The flat-top slice of the production workload, generated on the device (lab frame, lengths in
+    um): 16.8M records of a 3 kA beam over SLICE_UM, four beamlets lambda_r / 4 apart per base
+    particle, one shot-noise modulation per lambda_r bucket, sigma_r 50 um, gb_perp spread 0.02,
+    0.01 % energy spread, centred where the frame shift puts it mid-mesh."""
     k = 1.4
     lam = 1.8e4 / (2.0 * GAMMA ** 2) * (1.0 + k * k / 2.0)     # resonant wavelength, um
-    beta_rel = 0.33                                           # bunch speed in the frame before the undulator
-    z0 = beta_rel * SLICE_UM / 2.0                            # centres the slice after the frame shift
-    nb = n // 4
-    zb = z0 + rng.uniform(-SLICE_UM / 2.0, SLICE_UM / 2.0, nb)
-    x = rng.normal(0.0, 50.0, nb)
-    y = rng.normal(0.0, 50.0, nb)
-    gbx = rng.normal(0.0, 0.02, nb)
-    gby = rng.normal(0.0, 0.02, nb)
-    gbz = np.sqrt(GAMMA ** 2 - 1.0) * (1.0 + 1.0e-4 * rng.normal(0.0, 1.0, nb))
     electrons = CURRENT_A * SLICE_UM * 1e-6 / 2.99792458e8 / 1.602e-19
-    n_bucket = int(np.ceil(SLICE_UM / lam)) + 2
-    ne = electrons * lam / SLICE_UM
-    amp = np.sqrt(-2.0 * np.log(rng.uniform(1e-12, 1.0, n_bucket))) / np.sqrt(ne)
-    phase = 2.0 * np.pi * rng.uniform(0.0, 1.0, n_bucket)
-    bucket = ((zb - zb.min()) / lam).astype(np.int64)
-    out = np.empty((4 * nb, 7))
-    for ii in range(4):                                       # four beamlets lambda / 4 apart
-        z = zb - lam / 4.0 * ii
-        z = z - lam / np.pi * amp[bucket] * np.sin(2.0 * np.pi / lam * z + phase[bucket])
-        s = slice(ii, 4 * nb, 4)
-        out[s, 0] = electrons / (4 * nb)
-        out[s, 1], out[s, 2], out[s, 3] = x, y, z
-        out[s, 4], out[s, 5], out[s, 6] = gbx, gby, gbz
-    return out, electrons
+    f = abi.Flattop()
+    f.n, f.charge, f.gamma, f.lambda_, f.length = N_PARTICLES, electrons, GAMMA, lam, SLICE_UM
+    f.z_center = 0.33 * SLICE_UM / 2.0        # bunch speed in the frame before the undulator x half slice
+    f.sigma_r, f.sigma_gb_perp, f.energy_spread = 50.0, 0.02, 1.0e-4
+    f.electrons_per_bucket = electrons * lam / SLICE_UM
+    bunch = abi.DeviceBunch()
+    bunch.flattop(f, seed)
+    return bunch, electrons
 
 
 def log(msg):
@@ -263,27 +251,28 @@ def log(msg):
 
 
 def set_up(workdir):
-    log("generating the quiet-start bunch and running the host set-up")
-    bunch, electrons = quiet_start_bunch(N_PARTICLES)
+    log("generating the quiet-start bunch on the device and running the host set-up")
+    bunch, electrons = production_bunch()
     path = os.path.join(workdir, "production.job")
     with open(path, "w") as f:
         f.write(JOB.format(charge=electrons, n=N_PARTICLES, gamma=GAMMA, dir=workdir))
-    job = Job(path, external_bunch=bunch)
-    del bunch
-    return job
+    return Job(path, external_bunch=bunch)   # the job keeps the bunch, boosted in place
 
 
 def open_domain(job, precision, lib=None, prefix="mithra_dev_", rows=64):
+    """A domain holding the job's pre-step state: the GPU's through the host (potentials only when a
+    seed set them, particles device to device when generated there), the oracle's from host copies."""
     g, c = job.grid(), job.coeffs()
     if lib is None:
         dom = abi.Domain(g, c, precision=precision)
+        job.fill(dom)
     else:
         dom = abi.Domain(g, c, lib=lib, prefix=prefix)
-    an, fn = job.potentials(0)
-    am, fm = job.potentials(1)
-    dom.set_potentials(0, an, fn)
-    dom.set_potentials(1, am, fm)
-    dom.set_particles(job.particles())
+        an, fn = job.potentials(0)
+        am, fm = job.potentials(1)
+        dom.set_potentials(0, an, fn)
+        dom.set_potentials(1, am, fm)
+        dom.set_particles(job.particles())
     N, Nl, Nf, pc, z, w = job.power()
     t = job.times()
     bp = job.bunch_params()
@@ -567,7 +556,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if precision == abi.PRECISION_FP64 else "f32 fields, f64 particles",
-        "data": "synthetic: seeded quiet-start bunch, random-free job set-up through the drop-in host",
+        "data": "synthetic: seeded quiet-start bunch generated on the device, job set-up through the drop-in host",
         "config": {"workload": "production single instance (256x256x2048 nodes, 16.8M macro-particles, "
                                "1 GeV, K 1.4, lambda_u 1.8 cm, NSFD, Mur order 2, no space charge)"
                                if args.workload == "production" else

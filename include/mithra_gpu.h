@@ -259,6 +259,67 @@ int mithra_dev_set_tiles(mithra_dev* dev, int interior, int eval);
  * counterpart in the reference (a CPU code with no launch shapes). */
 int mithra_dev_autotune(mithra_dev* dev, int reps, int* chosen, float* ms);
 
+/* ---- bunches generated and boosted on the device ------------------------------------------
+ * A mithra_bunch holds Charge records on the device.  The generators append lab-frame records;
+ * the boost calls are the particle loops of Solver::lorentzBoostBunch (src/solver.cpp:263-423)
+ * and Solver::shiftBackInTime (src/solver.cpp:1185-1206), the host keeping the scalar formulas
+ * between them; mithra_dev_take_bunch then makes the records a domain's particles (device to
+ * device).  The drop-in host uses them when a job asks for device generation. */
+typedef struct mithra_bunch mithra_bunch;
+enum { MITHRA_DISTRIBUTION_UNIFORM = 0, MITHRA_DISTRIBUTION_GAUSSIAN = 1 };
+
+/* One `ellipsoid` bunch-initialization at one position (BunchInitialize, src/database.h, after
+ * Solver::setSimulationParameters scaled it): Bunch::initializeEllipsoid (src/classes.cpp:104-298)
+ * with the halton generator.  first_index = the records this initialization already produced at
+ * earlier positions (the reference's Np0); lambda = BunchInitialize::lambda_ (0 without an
+ * undulator: one record per draw instead of four beamlets). */
+typedef struct mithra_ellipsoid {
+  long n;                         /* number-of-particles (rounded up to a multiple of four)   */
+  unsigned first_index;
+  int distribution;               /* MITHRA_DISTRIBUTION_*                                    */
+  int shot_noise;
+  double charge;                  /* cloudCharge_ (electrons)                                 */
+  double gamma, beta_vector[3];   /* initialGamma_, betaVector_                               */
+  double position[3], sigma_position[3], sigma_gb[3];
+  double tran_trunc, long_trunc;  /* tranTrun_, longTrun_                                     */
+  double lambda, bunching, bunching_phase;  /* lambda_, bF_, bFP_ (degrees)                   */
+} mithra_ellipsoid;
+
+/* A quiet-start flat-top slice for `type = other` bunches (no counterpart in the reference,
+ * which leaves such bunches to an external program, src/solver.cpp:1164-1165): n records (a
+ * multiple of four) of charge/n electrons; n/4 base particles uniform in z over [z_center -
+ * length/2, z_center + length/2], Gaussian in x, y (sigma_r) and gb_x, gb_y (sigma_gb_perp),
+ * gb_z = sqrt(gamma^2 - 1)(1 + energy_spread N(0,1)); four beamlets lambda/4 apart per base
+ * particle, and per lambda bucket one shot-noise modulation of mean-square amplitude
+ * 2 / electrons_per_bucket with a uniform phase.  Every record is a function of (seed, index). */
+typedef struct mithra_flattop {
+  long n;
+  double charge, gamma, lambda, length, z_center, sigma_r, sigma_gb_perp, energy_spread;
+  double electrons_per_bucket;
+} mithra_flattop;
+
+int mithra_bunch_create(mithra_bunch** out);
+int mithra_bunch_destroy(mithra_bunch* bunch);
+long mithra_bunch_count(const mithra_bunch* bunch);
+/* the records as n Charge records (load order) on the host */
+int mithra_bunch_get(mithra_bunch* bunch, double* charge11, long n);
+/* appends; *added (optional) = records appended */
+int mithra_bunch_ellipsoid(mithra_bunch* bunch, const mithra_ellipsoid* spec, long* added);
+int mithra_bunch_flattop(mithra_bunch* bunch, const mithra_flattop* spec, unsigned long long seed);
+/* first loop of lorentzBoostBunch: z *= gamma, gb_z = gamma g (beta_z - beta); *zmax = max z */
+int mithra_bunch_boost(mithra_bunch* bunch, double gamma, double beta, double* zmax);
+/* second loop: r += gb / g (z - zu) beta */
+int mithra_bunch_drift(mithra_bunch* bunch, double beta, double zu);
+/* sums of z and beta_z, and min z (per-workgroup partials added in a fixed order) */
+int mithra_bunch_moments(mithra_bunch* bunch, double* z_sum, double* bz_sum, double* z_min);
+/* z -= shift (the optimised bunch position, src/solver.cpp:369-377) */
+int mithra_bunch_shift(mithra_bunch* bunch, double shift);
+/* r -= c0 time_shift / g gb (Solver::shiftBackInTime) */
+int mithra_bunch_time_shift(mithra_bunch* bunch, double c0, double time_shift);
+/* the bunch's records become the domain's particles (mithra_dev_set_particles from device
+ * memory); the bunch keeps its copy */
+int mithra_dev_take_bunch(mithra_dev* dev, const mithra_bunch* bunch);
+
 /* ---- batched parameter scans ------------------------------------------------------------
  * A batch marches many independent domains of uneven size together: each step is one fixed
  * sequence of launches whose persistent workgroups walk tables of (instance, tile) entries, and

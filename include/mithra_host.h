@@ -28,6 +28,17 @@ const char* mithra_job_error(const mithra_job* job);
  * (src/solver.cpp:1164-1165): n lab-frame records q x y z gbx gby gbz. */
 int mithra_job_set_external_bunch(mithra_job* job, const double* charge7, long n);
 
+/* Device generation (before initialize): on = 1 generates every `ellipsoid` bunch-initialization
+ * on the GPU (mithra_bunch_ellipsoid) and runs the particle loops of Solver::lorentzBoostBunch
+ * and Solver::shiftBackInTime there (src/solver.cpp:1130-1176, :263-423, :1185-1206); the records
+ * then stay on the device and the domain takes them over without a host copy.  Other bunch types
+ * than ellipsoid and `other` are refused in this mode.  No counterpart in the reference. */
+int mithra_job_device_bunch(mithra_job* job, int on);
+/* A `type = other` bunch handed over on the device (lab-frame records, e.g. from
+ * mithra_bunch_flattop): switches device generation on; the job boosts the records in place and
+ * does not own the bunch, which must outlive the job's set-up and upload. */
+int mithra_job_set_external_device_bunch(mithra_job* job, mithra_bunch* bunch);
+
 /* Solver::initialize (src/solver.cpp:547-595). */
 int mithra_job_initialize(mithra_job* job);
 
@@ -47,6 +58,11 @@ long mithra_job_particle_count(mithra_job* job);
 int mithra_job_particles(mithra_job* job, double* charge11, long n);
 /* level 0 = A^n, 1 = A^{n-1}; phi may be NULL. */
 int mithra_job_potentials(mithra_job* job, int level, double* a, double* phi);
+
+/* The initialised pre-step state in a device domain made from mithra_job_grid/coeffs: A^n and
+ * A^{n-1} (phi) when the seed set them (a new domain holds zeros, src/solver.cpp:828-839), and the
+ * particles (handed over on the device when they were generated there). */
+int mithra_job_fill_domain(mithra_job* job, mithra_dev* dev);
 
 /* Solver::solve on the GPU for at most max_steps steps (all when negative), writing the
  * reference's output files.  Returns the number of steps run, or a negative status. */

@@ -452,6 +452,72 @@ class Domain:
         return b.value
 
 
+class Ellipsoid(C.Structure):
+    _fields_ = [("n", C.c_long), ("first_index", C.c_uint), ("distribution", C.c_int), ("shot_noise", C.c_int),
+                ("charge", C.c_double), ("gamma", C.c_double), ("beta_vector", C.c_double * 3),
+                ("position", C.c_double * 3), ("sigma_position", C.c_double * 3), ("sigma_gb", C.c_double * 3),
+                ("tran_trunc", C.c_double), ("long_trunc", C.c_double), ("lambda_", C.c_double),
+                ("bunching", C.c_double), ("bunching_phase", C.c_double)]
+
+
+class Flattop(C.Structure):
+    _fields_ = [("n", C.c_long), ("charge", C.c_double), ("gamma", C.c_double), ("lambda_", C.c_double),
+                ("length", C.c_double), ("z_center", C.c_double), ("sigma_r", C.c_double),
+                ("sigma_gb_perp", C.c_double), ("energy_spread", C.c_double), ("electrons_per_bucket", C.c_double)]
+
+
+class DeviceBunch:
+    """Charge records generated on the device (mithra_bunch_*, include/mithra_gpu.h)."""
+
+    def __init__(self):
+        self.lib = load_gpu_library()
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        for name, res, args in (
+            ("mithra_bunch_create", C.c_int, [C.POINTER(vp)]),
+            ("mithra_bunch_destroy", C.c_int, [vp]),
+            ("mithra_bunch_count", C.c_long, [vp]),
+            ("mithra_bunch_get", C.c_int, [vp, dp, C.c_long]),
+            ("mithra_bunch_ellipsoid", C.c_int, [vp, C.POINTER(Ellipsoid), C.POINTER(C.c_long)]),
+            ("mithra_bunch_flattop", C.c_int, [vp, C.POINTER(Flattop), C.c_ulonglong]),
+        ):
+            f = getattr(self.lib, name)
+            f.restype, f.argtypes = res, args
+        self.h = vp()
+        self._check(self.lib.mithra_bunch_create(C.byref(self.h)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MithraError(f"status {rc}: {self.lib.mithra_strerror(rc).decode()}")
+
+    def ellipsoid(self, spec):
+        added = C.c_long()
+        self._check(self.lib.mithra_bunch_ellipsoid(self.h, C.byref(spec), C.byref(added)))
+        return added.value
+
+    def flattop(self, spec, seed):
+        self._check(self.lib.mithra_bunch_flattop(self.h, C.byref(spec), seed))
+
+    def count(self):
+        return self.lib.mithra_bunch_count(self.h)
+
+    def records(self):
+        n = self.count()
+        q = np.empty((n, 11))
+        self._check(self.lib.mithra_bunch_get(self.h, _ptr(q), n))
+        return q
+
+    def close(self):
+        if self.h:
+            self.lib.mithra_bunch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def make_power_params(z, w, Nf, pc, gamma, beta, c0):
     """PowerParams with its two arrays kept alive on the returned object."""
     z = np.ascontiguousarray(z, dtype=np.float64)

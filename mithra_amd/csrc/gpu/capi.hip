@@ -356,9 +356,13 @@ int mithra_dev_get_fields(mithra_dev* d, float* e, float* b) {
   return MITHRA_OK;
 }
 
-int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
-  if (!d || n < 0 || (n > 0 && !q)) return MITHRA_E_ARG;
-  MITHRA_TRY(hipStreamSynchronize(d->stream));
+}  // extern "C"
+
+namespace {
+
+// Room for n particles (keeping nothing: the caller fills every array), and the state of a
+// domain whose particles were just replaced.
+int prepare_particles(mithra_dev* d, long n) {
   if (n >= (1L << 31)) return MITHRA_E_ARG;
   if (n > d->cap) {
     hipFree(d->part);
@@ -388,8 +392,18 @@ int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
   d->rm_order_valid = 0;  // no cell order describes the new particles yet
   d->order1_is_rnp = 0;
   d->zrange_valid = 0;    // nor a bunch extent
+  return n ? hip_status(launch_particle_ids(d)) : MITHRA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
+  if (!d || n < 0 || (n > 0 && !q)) return MITHRA_E_ARG;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  if (int rc = prepare_particles(d, n)) return rc;
   if (n == 0) return MITHRA_OK;
-  MITHRA_TRY(launch_particle_ids(d));
   // Charge record q, rnp[3], rnm[3], gb[3], e  ->  x y z xm ym zm gx gy gz q e
   static const int from[PCOUNT] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 0, 10};
   std::vector<double> col(n);
@@ -398,6 +412,18 @@ int mithra_dev_set_particles(mithra_dev* d, const double* q, long n) {
     MITHRA_TRY(hipMemcpy(particle_array(d, a), col.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   }
   return MITHRA_OK;
+}
+
+int mithra_dev_take_bunch(mithra_dev* d, const mithra_bunch* b) {
+  const double* src = nullptr;
+  long cap = 0, n = 0;
+  if (!d || bunch_records(b, &src, &cap, &n)) return MITHRA_E_ARG;
+  MITHRA_TRY(hipDeviceSynchronize());  // the bunch's kernels run on the null stream
+  if (int rc = prepare_particles(d, n)) return rc;
+  for (int a = 0; a < PCOUNT && n > 0; ++a)
+    MITHRA_TRY(hipMemcpyAsync(particle_array(d, a), src + (size_t)a * cap, sizeof(double) * n,
+                              hipMemcpyDeviceToDevice, d->stream));
+  return hip_status(hipStreamSynchronize(d->stream));
 }
 
 int mithra_dev_get_particles(mithra_dev* d, double* q, long n) {

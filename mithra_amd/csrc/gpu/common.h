@@ -337,5 +337,6 @@ struct mithra_dev : mithra::Domain {
 
 namespace mithra {
 int upload_undulators(mithra_dev* d, const mithra_undulator* und, int n, double c0);  // capi.hip
+int bunch_records(const mithra_bunch* b, const double** part, long* cap, long* n);   // bunch.hip
 int settle(mithra_dev* d);  // writes the zeros current_reset left pending
 }  // namespace mithra

@@ -151,25 +151,43 @@ int mithra_job_power(mithra_job* job, double* p, int max) {
 
 long mithra_job_particle_count(mithra_job* job) {
   if (!job || !job->solver) return MITHRA_E_ARG;
-  return (long)job->solver->chargeVectorn_.size();
+  return job->solver->particleCount();
 }
 
 int mithra_job_particles(mithra_job* job, double* q, long n) {
   return guarded(job, [&]() {
     int rc = need_init(job);
     if (rc) return rc;
-    if (n != (long)job->solver->chargeVectorn_.size()) return MITHRA_E_ARG;
-    long p = 0;
-    for (const Charge& c : job->solver->chargeVectorn_) {
-      double* r = q + 11 * p++;
-      r[0] = c.q;
-      for (int k = 0; k < 3; ++k) {
-        r[1 + k] = c.rnp[k];
-        r[4 + k] = c.rnm[k];
-        r[7 + k] = c.gb[k];
-      }
-      r[10] = c.e;
-    }
+    if (n != job->solver->particleCount()) return MITHRA_E_ARG;
+    job->solver->particles(q, n);
+    return MITHRA_OK;
+  });
+}
+
+int mithra_job_fill_domain(mithra_job* job, mithra_dev* dev) {
+  return guarded(job, [&]() {
+    int rc = need_init(job);
+    if (rc) return rc;
+    if (!dev) return MITHRA_E_ARG;
+    job->solver->fill(dev);
+    return MITHRA_OK;
+  });
+}
+
+int mithra_job_device_bunch(mithra_job* job, int on) {
+  return guarded(job, [&]() {
+    if ((on != 0 && on != 1) || job->solver->initialized_) return MITHRA_E_ARG;
+    job->solver->deviceBunch_ = on != 0;
+    return MITHRA_OK;
+  });
+}
+
+int mithra_job_set_external_device_bunch(mithra_job* job, mithra_bunch* bunch) {
+  return guarded(job, [&]() {
+    if (!bunch || job->solver->initialized_ || job->solver->dbunch_) return MITHRA_E_ARG;
+    job->solver->deviceBunch_ = true;
+    job->solver->dbunch_ = bunch;
+    job->solver->ownsDbunch_ = false;
     return MITHRA_OK;
   });
 }

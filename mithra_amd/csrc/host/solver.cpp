@@ -48,6 +48,7 @@ Solver::Solver(Mesh& mesh, Bunch& bunch, Seed& seed, std::vector<Undulator>& und
 
 Solver::~Solver() {
   if (dev_) mithra_dev_destroy(dev_);
+  if (dbunch_ && ownsDbunch_) mithra_bunch_destroy(dbunch_);
 }
 
 void Solver::check(int status, const char* what) const {
@@ -138,7 +139,79 @@ void Solver::computeFileGamma(BunchInitialize& b) {
   for (int c = 0; c < 3; ++c) b.initialDirection_[c] /= n;
 }
 
+void Solver::deviceBunch() {
+  // Solver::initializeBunch (src/solver.cpp:1130-1176) with the records on the device: every
+  // ellipsoid initialization through the device generator, in the reference's order; a
+  // handed-over `other` bunch is already there.
+  if (!dbunch_) {
+    check(mithra_bunch_create(&dbunch_), "device bunch");
+    ownsDbunch_ = true;
+  }
+  for (auto& b : bunch_.bunchInit_) {
+    if (b.position_.empty()) b.position_.push_back(Vec3());
+    if (b.bunchType_ == "other") continue;
+    if (b.bunchType_ != "ellipsoid")
+      throw JobError("Device bunch generation takes ellipsoid and `other` bunches only, not " + b.bunchType_ + ".");
+    if (b.generator_ == "random")
+      throw JobError("Device bunch generation takes the halton generator only (random draws are not reproducible).");
+    if (b.distribution_ != "uniform" && b.distribution_ != "gaussian")
+      throw JobError("The longitudinal type is not correctly given to the code !!!");
+    long first = 0;  // records of this initialization at earlier positions (the reference's Np0)
+    for (size_t ia = 0; ia < b.position_.size(); ++ia) {
+      mithra_ellipsoid e;
+      std::memset(&e, 0, sizeof e);
+      e.n = b.numberOfParticles_;
+      e.first_index = (unsigned)first;
+      e.distribution = b.distribution_ == "uniform" ? MITHRA_DISTRIBUTION_UNIFORM : MITHRA_DISTRIBUTION_GAUSSIAN;
+      e.shot_noise = b.shotNoise_ ? 1 : 0;
+      e.charge = b.cloudCharge_;
+      e.gamma = b.initialGamma_;
+      for (int k = 0; k < 3; ++k) {
+        e.beta_vector[k] = b.betaVector_[k];
+        e.position[k] = b.position_[ia][k];
+        e.sigma_position[k] = b.sigmaPosition_[k];
+        e.sigma_gb[k] = b.sigmaGammaBeta_[k];
+      }
+      e.tran_trunc = b.tranTrun_;
+      e.long_trunc = b.longTrun_;
+      e.lambda = b.lambda_;
+      e.bunching = b.bF_;
+      e.bunching_phase = b.bFP_;
+      long added = 0;
+      check(mithra_bunch_ellipsoid(dbunch_, &e, &added), "device ellipsoid bunch");
+      first += added;
+    }
+  }
+}
+
+long Solver::particleCount() const {
+  return deviceBunch_ ? mithra_bunch_count(dbunch_) : (long)chargeVectorn_.size();
+}
+
+void Solver::particles(double* q, long n) const {
+  if (n != particleCount()) throw JobError("particle count mismatch");
+  if (deviceBunch_) {
+    check(mithra_bunch_get(dbunch_, q, n), "device bunch read-back");
+    return;
+  }
+  long i = 0;
+  for (const Charge& c : chargeVectorn_) {
+    double* r = q + 11 * i++;
+    r[0] = c.q;
+    for (int k = 0; k < 3; ++k) {
+      r[1 + k] = c.rnp[k];
+      r[4 + k] = c.rnm[k];
+      r[7 + k] = c.gb[k];
+    }
+    r[10] = c.e;
+  }
+}
+
 void Solver::initializeBunch() {
+  if (deviceBunch_) {
+    deviceBunch();
+    return;
+  }
   std::list<Charge> qv;
   for (auto& b : bunch_.bunchInit_) {
     qv.clear();
@@ -220,6 +293,10 @@ void Solver::lorentzBoostBunch() {
   for (auto& t : bunch_.bunchProfileTime_) t /= gamma_;
   bunch_.bunchProfileRhythm_ /= gamma_;
 
+  if (deviceBunch_) {
+    deviceBoost();
+    return;
+  }
   Double zmaxG = -1.0e100;
   for (auto& q : chargeVectorn_) {
     Double g = std::sqrt(1.0 + q.gb.norm2());
@@ -273,6 +350,47 @@ void Solver::lorentzBoostBunch() {
     }
     unsigned int Nq = (unsigned int)chargeVectorn_.size();
     bz /= Nq;
+    mesh_.totalTime_ = 1 / (c0_ * (bz + beta_)) * (zEnd - beta_ * c0_ * dt_ - zMin + bz / beta_ * Lu);
+  }
+}
+
+void Solver::deviceBoost() {
+  // The rest of lorentzBoostBunch above with its particle loops on the device (the time-step
+  // scaling at its start has run); the sums of the means are fixed-order per-workgroup partials.
+  Double zmaxG = -1.0e100;
+  check(mithra_bunch_boost(dbunch_, gamma_, beta_, &zmaxG), "device boost");
+  dt_ = 0.0;
+  if (undulator_.size() > 0) {
+    Double nl = (undulator_[0].type_ == STATIC) ? 2.0 : 5.0 * undulator_[0].signal_.nR_;
+    if (undulator_[0].dist_ == 0.0) undulator_[0].dist_ = nl * undulator_[0].lu_;
+    else if (undulator_[0].dist_ < nl * undulator_[0].lu_)
+      std::cout << "Warning: the undulator is set very close to the bunch, the results may be inaccurate." << std::endl;
+    dt_ = -1.0 / (beta_ * undulator_[0].c0_) * (zmaxG + undulator_[0].dist_ / gamma_);
+  }
+  seed_.dt_ = dt_;
+  bunch_.zu_ = zmaxG;
+  bunch_.beta_ = beta_;
+  check(mithra_bunch_drift(dbunch_, beta_, bunch_.zu_), "device drift");
+  const long Nq = mithra_bunch_count(dbunch_);
+  if (mesh_.optimizePosition_ && undulator_.size() > 0 && Nq > 0) {
+    Double zG = 0.0, bzG = 0.0, zlo = 0.0;
+    check(mithra_bunch_moments(dbunch_, &zG, &bzG, &zlo), "device moments");
+    zG /= (unsigned int)Nq;
+    bzG /= (unsigned int)Nq;
+    Double shift = bzG * (zmaxG + undulator_[0].dist_ / gamma_ - zG) / (bzG + beta_) + zG;
+    zmaxG -= shift;
+    bunch_.zu_ = zmaxG;
+    dt_ = -1.0 / (beta_ * undulator_[0].c0_) * (zmaxG + undulator_[0].dist_ / gamma_);
+    seed_.dt_ = dt_;
+    check(mithra_bunch_shift(dbunch_, shift), "device shift");
+  }
+  if (mesh_.totalDist_ > 0.0 && Nq > 0) {
+    double Lu = 0.0;
+    for (auto& u : undulator_) Lu += u.lu_ * u.length_ / gamma_;
+    double zEnd = mesh_.totalDist_ / gamma_;
+    double zs = 0.0, bz = 0.0, zMin = 1e100;
+    check(mithra_bunch_moments(dbunch_, &zs, &bz, &zMin), "device moments");
+    bz /= (unsigned int)Nq;
     mesh_.totalTime_ = 1 / (c0_ * (bz + beta_)) * (zEnd - beta_ * c0_ * dt_ - zMin + bz / beta_ * Lu);
   }
 }
@@ -399,6 +517,10 @@ void Solver::shiftBackInTime() {
   time_ -= mesh_.timeShift_;
   timep1_ -= mesh_.timeShift_;
   timeBunch_ -= mesh_.timeShift_;
+  if (deviceBunch_) {
+    check(mithra_bunch_time_shift(dbunch_, c0_, mesh_.timeShift_), "device time shift");
+    return;
+  }
   for (auto& q : chargeVectorn_) {
     Double t = c0_ * mesh_.timeShift_ / std::sqrt(1.0 + q.gb.norm2());
     for (int c = 0; c < 3; ++c) q.rnp[c] -= t * q.gb[c];
@@ -497,6 +619,24 @@ std::vector<mithra_undulator> Solver::undulators() const {
   return v;
 }
 
+void Solver::fill(mithra_dev* dev) {
+  // The pre-step state of Solver::initialize in a domain: the potentials when the seed set them
+  // (src/solver.cpp:828-839; a new domain holds zeros otherwise), and the particles, handed over
+  // on the device when they were generated there.
+  if (seed_.amplitude_ > 1.0e-50) {
+    check(mithra_dev_set_potentials(dev, 0, an_.data(), mesh_.spaceCharge_ ? fn_.data() : nullptr), "A^n upload");
+    check(mithra_dev_set_potentials(dev, 1, anm1_.data(), mesh_.spaceCharge_ ? fnm1_.data() : nullptr),
+          "A^n-1 upload");
+  }
+  if (deviceBunch_) {
+    check(mithra_dev_take_bunch(dev, dbunch_), "particle hand-over");
+    return;
+  }
+  std::vector<double> q(chargeVectorn_.size() * 11);
+  particles(q.data(), (long)chargeVectorn_.size());
+  check(mithra_dev_set_particles(dev, q.data(), (long)chargeVectorn_.size()), "particle upload");
+}
+
 void Solver::upload(int precision) {
   if (!initialized_) initialize();
   if (dev_) {
@@ -508,18 +648,7 @@ void Solver::upload(int precision) {
   check(mithra_dev_create(&g, &c, precision, &dev_), "device allocation");
   // E/B only where the bunch and the power sampler read them, as the reference evaluates it
   check(mithra_dev_set_lazy_fields(dev_, 1), "lazy field evaluation");
-  check(mithra_dev_set_potentials(dev_, 0, an_.data(), mesh_.spaceCharge_ ? fn_.data() : nullptr), "A^n upload");
-  check(mithra_dev_set_potentials(dev_, 1, anm1_.data(), mesh_.spaceCharge_ ? fnm1_.data() : nullptr), "A^n-1 upload");
-  std::vector<double> q;
-  q.reserve(chargeVectorn_.size() * 11);
-  for (auto& p : chargeVectorn_) {
-    q.push_back(p.q);
-    for (int k = 0; k < 3; ++k) q.push_back(p.rnp[k]);
-    for (int k = 0; k < 3; ++k) q.push_back(p.rnm[k]);
-    for (int k = 0; k < 3; ++k) q.push_back(p.gb[k]);
-    q.push_back(p.e);
-  }
-  check(mithra_dev_set_particles(dev_, q.data(), (long)chargeVectorn_.size()), "particle upload");
+  fill(dev_);
 }
 
 void Solver::writePowerRows(const std::vector<double>& rows, const std::vector<double>& tb, unsigned count) {

@@ -80,11 +80,22 @@ class Solver {
   std::vector<std::unique_ptr<std::ofstream>> pfile_;
   // external bunch for type = other (the reference leaves it to an external program)
   std::vector<Charge> external_;
+  // Device generation: the bunch lives in a mithra_bunch on the GPU from its generation on, and
+  // the particle loops of the set-up run there (ellipsoid initializations and `other` bunches
+  // handed over on the device only).  dbunch_ is owned unless it was handed over.
+  bool deviceBunch_ = false;
+  mithra_bunch* dbunch_ = nullptr;
+  bool ownsDbunch_ = false;
+  long particleCount() const;
+  void fill(mithra_dev* dev);
+  void particles(double* charge11, long n) const;
   bool initialized_ = false;
   mithra_dev* dev_ = nullptr;
 
  protected:
   void check(int status, const char* what) const;
+  void deviceBunch();
+  void deviceBoost();
   void writePowerRows(const std::vector<double>& rows, const std::vector<double>& tb, unsigned count);
 };
 

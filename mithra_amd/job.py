@@ -39,6 +39,9 @@ def load_host_library(path=abi.HOST_LIB):
             ("mithra_job_particles", C.c_int, [vp, dp, C.c_long]),
             ("mithra_job_potentials", C.c_int, [vp, C.c_int, dp, dp]),
             ("mithra_job_run", C.c_long, [vp, C.c_long, C.c_int, C.c_int]),
+            ("mithra_job_fill_domain", C.c_int, [vp, vp]),
+            ("mithra_job_device_bunch", C.c_int, [vp, C.c_int]),
+            ("mithra_job_set_external_device_bunch", C.c_int, [vp, vp]),
         ):
             f = getattr(lib, name)
             f.restype, f.argtypes = res, args
@@ -47,12 +50,23 @@ def load_host_library(path=abi.HOST_LIB):
 
 
 class Job:
-    def __init__(self, path, external_bunch=None, initialize=True):
+    """device_bunch: generate ellipsoid bunches and run the set-up's particle loops on the GPU.
+    external_bunch: the records of a `type = other` bunch, an (n, 7) array of lab-frame q x y z
+    gbx gby gbz on the host, or an abi.DeviceBunch already on the device (which the set-up boosts in
+    place; it must outlive the set-up and fill())."""
+
+    def __init__(self, path, external_bunch=None, initialize=True, device_bunch=False):
         self.lib = load_host_library()
         self.h = C.c_void_p()
         rc = self.lib.mithra_job_open(str(path).encode(), C.byref(self.h))
         self._check(rc)
-        if external_bunch is not None:
+        self._device_bunch = None
+        if isinstance(external_bunch, abi.DeviceBunch):
+            self._device_bunch = external_bunch
+            self._check(self.lib.mithra_job_set_external_device_bunch(self.h, external_bunch.h))
+        elif device_bunch:
+            self._check(self.lib.mithra_job_device_bunch(self.h, 1))
+        if external_bunch is not None and not isinstance(external_bunch, abi.DeviceBunch):
             b = np.ascontiguousarray(external_bunch, dtype=np.float64).reshape(-1, 7)
             self._check(self.lib.mithra_job_set_external_bunch(
                 self.h, b.ctypes.data_as(C.POINTER(C.c_double)), b.shape[0]))
@@ -129,6 +143,11 @@ class Job:
         self._check(self.lib.mithra_job_potentials(self.h, level, a.ctypes.data_as(dp),
                                                    None if phi is None else phi.ctypes.data_as(dp)))
         return a, phi
+
+    def fill(self, dom):
+        """Writes the pre-step state into a device domain (abi.Domain of the GPU library): the
+        potentials when a seed set them, the particles (device to device when generated there)."""
+        self._check(self.lib.mithra_job_fill_domain(self.h, dom.h))
 
     def run(self, max_steps=-1, precision=abi.PRECISION_FP64, quiet=True):
         return self._check(self.lib.mithra_job_run(self.h, max_steps, precision, 1 if quiet else 0))

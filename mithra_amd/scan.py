@@ -11,8 +11,8 @@ ordinary job file (so any instance can be run, or checked, on its own):
 * the mesh has n_perp x n_perp x n_z cells, the transverse cell dx = 2 gamma dz (above the
   NSFD solver's dx >= 1.44 gamma dz, so the reference keeps the mesh as written);
 * the bunch is Gaussian with sigma_r and sigma_z a fraction of the mesh (one eighth by default),
-  loaded at random or by a quiet start (beamlets lambda_r / 4 apart, optionally with an imposed
-  initial bunching at lambda_r),
+  loaded by the reference's own quiet start (halton numbers, beamlets lambda_r / 4 apart) with
+  shot noise, or with an imposed initial bunching at lambda_r instead, generated on the device,
   n_particles macro-particles of a beam of the given peak current (1 kA by default), normalised
   transverse momentum spread emittance / sigma_r (2e-3 by default);
 * the march runs n_z field steps (with dt = dz/c in the frame, the light-crossing time of the
@@ -48,11 +48,20 @@ BUNCH
 {{
   bunch-initialization
   {{
-    type                  = other
+    type                  = ellipsoid
+    distribution          = gaussian
     charge                = {charge:.10g}
     number-of-particles   = {n}
     gamma                 = {gamma:.10g}
     direction             = ( 0.0, 0.0, 1.0 )
+    position              = ( 0.0, 0.0, 0.0 )
+    sigma-position        = ( {sr:.10g}, {sr:.10g}, {sz:.10g} )
+    sigma-momentum        = ( {sgb:.10g}, {sgb:.10g}, {sgbz:.10g} )
+    transverse-truncation = {tt:.10g}
+    longitudinal-truncation = {lt:.10g}
+    shot-noise            = {shot}
+    bunching-factor       = {bf:.10g}
+    bunching-factor-phase = {bfp:.10g}
   }}
 }}
 UNDULATOR
@@ -90,13 +99,11 @@ class Instance:
     K: float
     lambda_u: float
     spread: float
-    seed: int
     emittance: float = 0.0    # normalised emittance (um rad); 0: transverse gamma-beta spread 2e-3
     current: float = 1000.0   # peak current (A)
     sr_frac: float = 0.125    # sigma_r / transverse mesh length
     sz_frac: float = 0.125    # sigma_z / longitudinal mesh length
-    quiet: bool = False       # quiet start: four beamlets lambda_r / 4 apart (no shot noise at lambda_r)
-    bunching: float = 0.0     # imposed initial bunching at lambda_r (amplitude, with `phase`)
+    bunching: float = 0.0     # imposed initial bunching at lambda_r (amplitude, with `phase`); 0: shot noise
     phase: float = 0.0
 
     @property
@@ -133,71 +140,50 @@ def generate(n=64, seed=4, n_perp=(32, 128), n_z=(128, 1024), n_particles=(1e4, 
         out.append(Instance(n_perp=int(round(logu(*n_perp))), n_z=int(round(logu(*n_z))),
                             n_particles=int(round(logu(*n_particles))), gamma=rng.uniform(100.0, 2000.0),
                             K=rng.uniform(0.5, 2.0), lambda_u=rng.uniform(1.0e4, 3.0e4),
-                            spread=logu(1e-4, 1e-2), seed=int(rng.integers(1 << 31))))
+                            spread=logu(1e-4, 1e-2)))
     return out
 
 
-def bunch_records(inst):
-    """Lab-frame records q x y z gbx gby gbz of the instance's Gaussian bunch, and the number of
-    electrons."""
-    rng = np.random.default_rng(inst.seed)
-    n = inst.n_particles
+def bunch_sizes(inst):
+    """sigma_r, sigma_z (lab frame, um), the transverse momentum spread and the electrons of the
+    instance's Gaussian bunch of peak current inst.current."""
     lx, lz = (inst.n_perp - 1) * inst.dx, (inst.n_z - 1) * inst.dz
     sr, sz = lx * inst.sr_frac, lz * inst.sz_frac
     sgb = inst.emittance / sr if inst.emittance > 0.0 else 2e-3
-    out = np.empty((n, 7))
     electrons = inst.current * (math.sqrt(2.0 * math.pi) * sz * 1e-6) / 2.99792458e8 / 1.602e-19
-    out[:, 0] = electrons / n
-    if inst.quiet:  # n/4 base particles, each copied at z + j lambda_r / 4 (the reference's beamlets)
-        nb = n // 4
-        base = np.column_stack([rng.normal(0.0, sr, nb), rng.normal(0.0, sr, nb), rng.normal(0.0, sz, nb),
-                                rng.normal(0.0, sgb, nb), rng.normal(0.0, sgb, nb),
-                                math.sqrt(inst.gamma ** 2 - 1.0) * (1.0 + inst.spread * rng.normal(0.0, 1.0, nb))])
-        cols = np.repeat(base, 4, axis=0)
-        cols[:, 2] -= np.tile(np.arange(4) * inst.lambda_r / 4.0, nb)
-        out = out[:4 * nb]
-        out[:, 0] = electrons / (4 * nb)
-        out[:, 1:] = cols
-        n = 4 * nb
-    else:
-        out[:, 1] = rng.normal(0.0, sr, n)
-        out[:, 2] = rng.normal(0.0, sr, n)
-        out[:, 3] = rng.normal(0.0, sz, n)
-        out[:, 4] = rng.normal(0.0, sgb, n)
-        out[:, 5] = rng.normal(0.0, sgb, n)
-        out[:, 6] = math.sqrt(inst.gamma ** 2 - 1.0) * (1.0 + inst.spread * rng.normal(0.0, 1.0, n))
-    if inst.bunching > 0.0:  # displacement lambda_r b / pi sin(k z + phase) gives bunching b
-        k = 2.0 * math.pi / inst.lambda_r
-        out[:, 3] -= inst.lambda_r / math.pi * inst.bunching * np.sin(k * out[:, 3] + inst.phase)
-    return out, electrons
+    return sr, sz, sgb, electrons
 
 
-def job_text(inst, idx, workdir, electrons):
+def job_text(inst, idx, workdir):
+    """The instance as a job file: a Gaussian ellipsoid loaded by the reference's own quiet start
+    (halton numbers, four beamlets lambda_r / 4 apart), with shot noise, or, for an instance with
+    an imposed bunching, with that bunching factor and phase instead."""
     # the reference makes (int)(L / d) + 2 nodes of a mesh length L (src/solver.cpp:610-612)
     lx, lz = (inst.n_perp - 1.5) * inst.dx, (inst.n_z - 1.5) * inst.dz
-    n = inst.n_particles // 4 * 4 if inst.quiet else inst.n_particles
-    return JOB.format(lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=n,
-                      gamma=inst.gamma, K=inst.K, lu=inst.lambda_u, dir=workdir, idx=idx, zp=0.375 * lz)
+    sr, sz, sgb, electrons = bunch_sizes(inst)
+    bunched = inst.bunching > 0.0
+    return JOB.format(lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=inst.n_particles // 4 * 4,
+                      gamma=inst.gamma, sr=sr, sz=sz, sgb=sgb, sgbz=inst.spread * math.sqrt(inst.gamma ** 2 - 1.0),
+                      tt=4.0 * sr, lt=3.5 * sz, shot="false" if bunched else "true",
+                      bf=inst.bunching if bunched else 0.0, bfp=math.degrees(inst.phase) if bunched else 0.0,
+                      K=inst.K, lu=inst.lambda_u, dir=workdir, idx=idx, zp=0.375 * lz)
 
 
 def set_up(inst, idx, workdir):
-    """Writes the instance's job file and runs the drop-in host's set-up on it."""
-    bunch, electrons = bunch_records(inst)
+    """Writes the instance's job file and runs the drop-in host's set-up on it, the bunch generated
+    and boosted on the device."""
     path = os.path.join(workdir, f"scan-{idx}.job")
     with open(path, "w") as f:
-        f.write(job_text(inst, idx, workdir, electrons))
-    return Job(path, external_bunch=bunch)
+        f.write(job_text(inst, idx, workdir))
+    return Job(path, device_bunch=True)
 
 
 def open_domain(job, precision, rows):
-    """A device domain holding the job's state before its first step."""
+    """A device domain holding the job's state before its first step (the particles handed over on
+    the device)."""
     g, c = job.grid(), job.coeffs()
     dom = abi.Domain(g, c, precision=precision)
-    an, fn = job.potentials(0)
-    am, fm = job.potentials(1)
-    dom.set_potentials(0, an, fn)
-    dom.set_potentials(1, am, fm)
-    dom.set_particles(job.particles())
+    job.fill(dom)
     N, Nl, Nf, pc, z, w = job.power()
     bp = job.bunch_params()
     dom.power_setup(abi.make_power_params(z, w, Nf, pc, bp.gamma, bp.beta, bp.c0), rows)
@@ -227,6 +213,14 @@ class Prepared:
 
     def open(self):
         self.dom = open_domain(self.job, self.precision, self.inst.n_steps)
+
+    def profile(self, bins):
+        """Initial current profile: the charge per bin of the particles' z over the mesh's z range,
+        in the frame, before the first step."""
+        q = self.job.particles()
+        g = self.job.grid()
+        h, _ = np.histogram(q[:, 3], bins=bins, range=(g.zmin, g.zmax), weights=q[:, 0])
+        return h / max(h.max(), 1e-300)
 
     def result(self, bunching):
         """P(z) and b(z) at the SAMPLES sample steps."""

@@ -2,7 +2,7 @@
 
 A scan point is described by 8 beam/undulator scalars (gamma, K, lambda_u, energy spread,
 normalised emittance, peak current, sigma_r, sigma_z) and its initial current profile (256 bins
-over the mesh length, from the loaded particles); the surrogate predicts P(z) and the bunching
+over the mesh's z range, from the particles of the set-up); the surrogate predicts P(z) and the bunching
 factor b(z) at the SAMPLES sample steps of the march.  It is an ensemble of MEMBERS multilayer
 perceptrons (4 layers, 512 wide, SiLU) trained on the GPU by the library's BF16-MFMA kernels on
 runs that the batched solver itself produces.  At use, the ensemble's spread decides: a point
@@ -26,20 +26,17 @@ SPREAD_LIMIT = 0.02
 DIMS = [8 + PROFILE_BINS, HIDDEN, HIDDEN, HIDDEN, 2 * scan.SAMPLES]
 
 
-BASE_SAMPLE = 7          # seed of the one base sample every surrogate point scales
-
-
 def generate(n, seed, n_perp=32, n_z=128, n_particles=16384):
     """Scan points for the surrogate: every beam/undulator scalar varies on one mesh shape,
     n_perp x n_perp x n_z cells with n_z = SAMPLES, so P(z) and b(z) are sampled at every step.
     (The mesh is not among the surrogate's inputs, so it must not vary: P sampled over a
-    transverse window depends on the window's size.)  The bunch is loaded by a
+    transverse window depends on the window's size.)  The bunch is loaded by the reference's
     quiet start with an imposed initial bunching (amplitude 0.5-5 %, random phase), which the
     current profile shows: P(z) is then a function of the inputs rather than of shot noise.  The
     bunch (sigma_z at most 1/16 of the window) stays clear of the power plane 3/8 of the window
-    ahead of its centre, so the plane sees radiation, not the bunch's own field.  Like the
-    reference's Halton-sequence quiet start, the loading is deterministic: every point scales
-    the same base sample of n_particles normal deviates by its own sizes and spreads."""
+    ahead of its centre, so the plane sees radiation, not the bunch's own field.  The halton
+    quiet start makes the loading deterministic: every point scales the same n_particles
+    halton-sequence draws by its own sizes and spreads."""
     rng = np.random.default_rng(seed)
 
     def logu(lo, hi):
@@ -51,9 +48,9 @@ def generate(n, seed, n_perp=32, n_z=128, n_particles=16384):
             n_perp=n_perp, n_z=n_z,
             n_particles=n_particles, gamma=rng.uniform(100.0, 2000.0),
             K=rng.uniform(0.5, 2.0), lambda_u=rng.uniform(1.0e4, 3.0e4), spread=logu(1e-4, 1e-2),
-            seed=BASE_SAMPLE, emittance=logu(0.5, 5.0), current=rng.uniform(200.0, 3000.0),
+            emittance=logu(0.5, 5.0), current=rng.uniform(200.0, 3000.0),
             sr_frac=rng.uniform(1.0 / 12.0, 1.0 / 6.0), sz_frac=rng.uniform(1.0 / 24.0, 1.0 / 16.0),
-            quiet=True, bunching=rng.uniform(0.005, 0.05), phase=rng.uniform(0.0, 2.0 * math.pi)))
+            bunching=rng.uniform(0.005, 0.05), phase=rng.uniform(0.0, 2.0 * math.pi)))
     return out
 
 
@@ -62,16 +59,10 @@ def scalars(inst):
                      math.log(inst.emittance), inst.current / 1000.0, inst.sr_frac * 8.0, inst.sz_frac * 20.0])
 
 
-def profile(inst):
-    """Initial current profile: charge per bin of the lab-frame z over the mesh length."""
-    rec, _ = scan.bunch_records(inst)
-    lz = (inst.n_z - 1) * inst.dz
-    h, _ = np.histogram(rec[:, 3], bins=PROFILE_BINS, range=(-lz / 2.0, lz / 2.0), weights=rec[:, 0])
-    return h / max(h.max(), 1e-300)
-
-
-def features(insts):
-    return np.array([np.concatenate([scalars(i), profile(i)]) for i in insts])
+def features(insts, profiles):
+    """Inputs of the points: their scalars and initial current profiles (from the set-up, see
+    solve)."""
+    return np.array([np.concatenate([scalars(i), p]) for i, p in zip(insts, profiles)])
 
 
 def targets(results):
@@ -83,15 +74,18 @@ def power_from_targets(y):
     return 10.0 ** y[..., :scan.SAMPLES] - 1.0
 
 
-def solve(insts, workdir, precision=abi.PRECISION_FP32_BULK, chunk=256, log=None):
+def solve(insts, workdir, precision=abi.PRECISION_FP32_BULK, chunk=256, log=None, profiles=None):
     """FDTD/PIC solutions of the points through the batched solver, `chunk` at a time; returns
-    (results, seconds of set-up, seconds of marching)."""
+    (results, seconds of set-up, seconds of marching).  profiles (a list, optional) receives each
+    point's initial current profile, read from its set-up outside the timed parts."""
     res, t_setup, t_march = [], 0.0, 0.0
     for c0 in range(0, len(insts), chunk):
         part = insts[c0:c0 + chunk]
         t0 = time.perf_counter()
         prep = scan.prepare(part, workdir, precision)
         t_setup += time.perf_counter() - t0
+        if profiles is not None:
+            profiles.extend(p.profile(PROFILE_BINS) for p in prep)
         r, secs, _ = scan.run_batch(prep, autotune=False)
         t_march += secs
         res.extend(r)
@@ -162,8 +156,9 @@ def run_workflow(workdir, n_train=4096, n_hold=512, epochs=2000, train_seed=5, h
     solver, and surrogate + batched fallback.  Returns a summary dict."""
     out = {"members": MEMBERS, "dims": DIMS, "n_train": n_train, "n_hold": n_hold, "epochs": epochs}
     train, hold = generate(n_train, train_seed), generate(n_hold, hold_seed)
-    res_tr, s1, m1 = solve(train, workdir, abi.PRECISION_FP32_BULK, log=log)
-    res_h32, s2, m2 = solve(hold, workdir, abi.PRECISION_FP32_BULK)
+    prof_tr, prof_h = [], []
+    res_tr, s1, m1 = solve(train, workdir, abi.PRECISION_FP32_BULK, log=log, profiles=prof_tr)
+    res_h32, s2, m2 = solve(hold, workdir, abi.PRECISION_FP32_BULK, profiles=prof_h)
     res_h64, s3, m3 = solve(hold, workdir, abi.PRECISION_FP64)
     out["solve_s"] = {"train_setup": s1, "train_march": m1, "hold_fp32_setup": s2, "hold_fp32_march": m2,
                       "hold_fp64_setup": s3, "hold_fp64_march": m3}
@@ -171,7 +166,7 @@ def run_workflow(workdir, n_train=4096, n_hold=512, epochs=2000, train_seed=5, h
     b64 = np.array([b for _, b in res_h64])
     out["fp32_bulk_p_err_vs_fp64"] = {"max": max(p_error(P, r) for (P, _), r in zip(res_h32, P64)),
                                       "median": float(np.median([p_error(P, r) for (P, _), r in zip(res_h32, P64)]))}
-    Xtr, Ytr, Xh = features(train), targets(res_tr), features(hold)
+    Xtr, Ytr, Xh = features(train, prof_tr), targets(res_tr), features(hold, prof_h)
     if save:  # the generated data set: features, targets, FP64 held-out P(z) and b(z)
         np.savez(save, Xtr=Xtr, Ytr=Ytr, Xh=Xh, Ph64=P64, bh64=b64)
     model = Surrogate(seed=1)
