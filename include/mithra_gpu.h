@@ -206,6 +206,36 @@ int mithra_dev_power_setup(mithra_dev* dev, const mithra_power_params* p, unsign
 int mithra_dev_power_sample(mithra_dev* dev, unsigned n_time, unsigned row);
 int mithra_dev_power_fetch(mithra_dev* dev, unsigned first, unsigned count, double* out);
 
+/* ---- diagnostics: what the host's file writers print, gathered on the device ------------- */
+
+/* E/B as last evaluated, A^n and phi^n at n listed nodes (m = N1 N0 k + N1 i + j): the values
+ * FdTd::fieldSample (src/fdtd.cpp:851-950) and the in-plane visualizations
+ * (src/fdtd.cpp:1128-1545) read from en_, bn_, an_ and fn_ (FdTdSC, src/fdtdSC.cpp:1147-1262)
+ * to interpolate.  e, b, a: 3 per node; phi: 1 per node (0 without space charge); any may be
+ * NULL.  A node outside the mesh is MITHRA_E_ARG.  Waits for the device. */
+int mithra_dev_sample_nodes(mithra_dev* dev, const long* nodes, long n, float* e, float* b, double* a,
+                            double* phi);
+
+/* The charge-weighted sums of Solver::bunchSample (src/solver.cpp:1582-1610) over every
+ * particle: out[0] = sum q, out[1..3] = sum q r, out[4..6] = sum r^2 q, out[7..9] = sum q gb,
+ * out[10..12] = sum gb^2 q.  Summed on a fixed grid in a fixed order (the same particle state
+ * gives the same sums every time); the reference sums in list order, so the two agree to
+ * rounding.  Waits for the device. */
+int mithra_dev_bunch_sums(mithra_dev* dev, double* out13);
+
+/* Lab-frame screens (Solver::screenProfile, src/solver.cpp:2205-2257): the particles whose lab
+ * position passed screen s (lz[s], lengths) during the field step that just ended, with the
+ * record the reference writes for each: x, y and lab time interpolated to the screen, gb_x,
+ * gb_y and the lab-frame gb_z.  time_bunch = Solver::timeBunch_ after the step's sub-steps,
+ * time_step = the field step, dt_shift = Solver::dt_.  count receives the number of crossings;
+ * the first min(count, cap) are written (screen index, particle load index, 6 doubles each) in
+ * no particular order.  Waits for the device. */
+typedef struct mithra_screen_params {
+  double gamma, beta, c0, time_bunch, time_step, dt_shift;
+} mithra_screen_params;
+int mithra_dev_screen_crossings(mithra_dev* dev, const double* lz, int n_screens, const mithra_screen_params* p,
+                                long cap, long* count, int* screen, long* index, double* rec6);
+
 /* Rotate A^{n-1} <- A^n <- A^{n+1} (and phi): FdTd::fieldShift (src/fdtd.cpp:806-812). */
 int mithra_dev_field_shift(mithra_dev* dev);
 

@@ -69,6 +69,10 @@ class PowerParams(C.Structure):
                 ("z", _dp), ("w", _dp)]
 
 
+class ScreenParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "beta", "c0", "time_bunch", "time_step", "dt_shift")]
+
+
 class MithraError(RuntimeError):
     pass
 
@@ -203,6 +207,11 @@ def _declare(lib, prefix):
         sig["set_fused_tile"] = (C.c_int, [vp, C.c_int])
         sig["autotune_fused"] = (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)])
         sig["bunching_noalloc"] = (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)])
+        lp = C.POINTER(C.c_long)
+        sig["sample_nodes"] = (C.c_int, [vp, lp, C.c_long, _fp, _fp, _dp, _dp])
+        sig["bunch_sums"] = (C.c_int, [vp, _dp])
+        sig["screen_crossings"] = (C.c_int, [vp, _dp, C.c_int, C.POINTER(ScreenParams), C.c_long, lp,
+                                             C.POINTER(C.c_int), lp, _dp])
     fns = {}
     for name, (res, args) in sig.items():
         f = getattr(lib, prefix + name)
@@ -450,6 +459,36 @@ class Domain:
         b = C.c_double()
         self._check(self.f["bunching_noalloc"](self.h, k, C.byref(b)))
         return b.value
+
+
+    def sample_nodes(self, nodes):
+        """(E, B, A^n, phi^n) at the listed node indices (device library only)."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        n = nodes.size
+        e, b = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+        a, phi = np.empty((n, 3)), np.empty(n)
+        self._check(self.f["sample_nodes"](self.h, _ptr(nodes, C.c_long), n, _ptr(e, C.c_float),
+                                           _ptr(b, C.c_float), _ptr(a), _ptr(phi)))
+        return e, b, a, phi
+
+    def bunch_sums(self):
+        """[sum q, sum q r (3), sum r^2 q (3), sum q gb (3), sum gb^2 q (3)] (device library only)."""
+        out = np.empty(13)
+        self._check(self.f["bunch_sums"](self.h, _ptr(out)))
+        return out
+
+    def screen_crossings(self, lz, params, cap=None):
+        """(screen, particle index, records[n, 6]) of the particles that crossed the lab-frame
+        screens at lz during the last field step (device library only)."""
+        lz = np.ascontiguousarray(lz, dtype=np.float64)
+        cap = self.f["particle_count"](self.h) * len(lz) if cap is None else cap
+        scr, idx = np.empty(max(cap, 1), np.int32), np.empty(max(cap, 1), np.int64)
+        rec = np.empty((max(cap, 1), 6))
+        count = C.c_long()
+        self._check(self.f["screen_crossings"](self.h, _ptr(lz), len(lz), C.byref(params), cap, C.byref(count),
+                                               _ptr(scr, C.c_int), _ptr(idx, C.c_long), _ptr(rec)))
+        n = min(count.value, cap)
+        return scr[:n], idx[:n], rec[:n], count.value
 
 
 class Ellipsoid(C.Structure):

@@ -279,6 +279,7 @@ int mithra_dev_destroy(mithra_dev* d) {
   free_sort(d);
   hipFree(d->und_dev);
   free_power(d);
+  hipFree(d->diag);
   if (d->ev_ready)
     for (auto& e : d->ev) hipEventDestroy(e);
   if (d->stream) hipStreamDestroy(d->stream);

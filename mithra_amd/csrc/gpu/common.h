@@ -190,6 +190,9 @@ struct Domain {
   double* history;          // device: [rows][N][Nl]
   int pblocks;
   int* plane_of;            // host: for each of N planes, its index among inside planes or -1
+  // staging of the diagnostics gathers (diagnostics.hip), grown on demand
+  void* diag;
+  size_t diag_bytes;
 };
 
 // Correctly rounded x / d given r = RN(1 / d): q = RN(x r) is within one ulp of x / d, and one

@@ -470,6 +470,23 @@ struct BeamKeys {
 
 }  // namespace
 
+namespace {
+
+SamplingType samplingTypeOf(const std::string& t) {  // Seed::samplingType, src/classes.cpp:871-876
+  if (t == "at-point") return ATPOINT;
+  if (t == "over-line") return OVERLINE;
+  throw JobError(t + " is an unknown sampling type.");
+}
+
+FieldType fieldTypeOf(const std::string& t) {  // Seed::fieldType, src/classes.cpp:896-908
+  static const char* names[] = {"Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az", "F"};
+  for (int i = 0; i < 10; ++i)
+    if (t == names[i]) return FieldType(i);
+  throw JobError(t + " is an unknown sampling field.");
+}
+
+}  // namespace
+
 void ParseDarius::readField(It& it) {
   if (*next(it) != "{") throw JobError("The seed directory is empty");
   next(it);
@@ -484,24 +501,63 @@ void ParseDarius::readField(It& it) {
         next(it);
       } while (*it != "}");
       seed_.initialize(k.type, k.position, k.direction, k.polarization, k.a0, k.radius, k.order, k.signal());
-    } else if (*it == "field-sampling" || *it == "field-visualization" || *it == "field-profile") {
-      const std::string group = *it;
-      if (*next(it) != "{") throw JobError("The " + group + " directory is empty");
+    } else if (*it == "field-sampling") {  // src/datainput.cpp:330-367
+      if (*next(it) != "{") throw JobError("The seed-sampling directory is empty");
       next(it);
-      static const char* sampling[] = {"sample", "type", "field", "directory", "base-name", "rhythm", "position",
-                                       "line-begin", "line-end", "number-of-points"};
-      static const char* vtk[] = {"sample", "directory", "type", "plane", "base-name", "field", "rhythm", "position"};
-      static const char* profile[] = {"sample", "directory", "base-name", "time", "field", "rhythm"};
-      if (group == "field-visualization") seed_.vtkCount_++;
       do {
         const std::string name = parameterName(*it);
-        const char** keys = group == "field-sampling" ? sampling : group == "field-visualization" ? vtk : profile;
-        const int nk = group == "field-sampling" ? 10 : group == "field-visualization" ? 8 : 6;
-        bool known = false;
-        for (int i = 0; i < nk; ++i) known = known || name == keys[i];
-        if (!known) throw JobError(name + " is not defined in seed-" + group.substr(6) + " group.");
-        if (name == "sample" && group == "field-sampling") seed_.sampling_ = boolValue(*it);
-        if (name == "sample" && group == "field-profile") seed_.profile_ = boolValue(*it);
+        if (name == "sample") seed_.sampling_ = boolValue(*it);
+        else if (name == "type") seed_.samplingType_ = samplingTypeOf(stringValue(*it));
+        else if (name == "field") seed_.samplingField_.push_back(fieldTypeOf(stringValue(*it)));
+        else if (name == "directory") seed_.samplingDirectory_ = stringValue(*it);
+        else if (name == "base-name") seed_.samplingBasename_ = stringValue(*it);
+        else if (name == "rhythm") seed_.samplingRhythm_ = doubleValue(*it);
+        else if (name == "position") seed_.samplingPosition_.push_back(vec3(vectorDoubleValue(*it)));
+        else if (name == "line-begin") seed_.samplingLineBegin_ = vec3(vectorDoubleValue(*it));
+        else if (name == "line-end") seed_.samplingLineEnd_ = vec3(vectorDoubleValue(*it));
+        else if (name == "number-of-points") seed_.samplingRes_ = intValue(*it);
+        else throw JobError(name + " is not defined in seed-sampling group.");
+        next(it);
+      } while (*it != "}");
+    } else if (*it == "field-visualization") {  // src/datainput.cpp:370-397
+      if (*next(it) != "{") throw JobError("The seed-visualization directory is empty");
+      next(it);
+      seed_.vtk_.emplace_back();
+      Seed::Vtk& v = seed_.vtk_.back();
+      do {
+        const std::string name = parameterName(*it);
+        if (name == "sample") v.sample_ = boolValue(*it);
+        else if (name == "directory") v.directory_ = stringValue(*it);
+        else if (name == "type") {
+          const std::string t = stringValue(*it);
+          if (t == "in-plane") v.type_ = INPLANE;
+          else if (t == "all-domain") v.type_ = ALLDOMAIN;
+          else throw JobError(t + " is an unknown vtk type.");
+        } else if (name == "plane") {
+          const std::string t = stringValue(*it);
+          if (t == "yz") v.plane_ = XNORMAL;
+          else if (t == "xz") v.plane_ = YNORMAL;
+          else if (t == "xy") v.plane_ = ZNORMAL;
+          else throw JobError(t + " is an unknown vtk plane type.");
+        } else if (name == "base-name") v.basename_ = stringValue(*it);
+        else if (name == "field") v.field_.push_back(fieldTypeOf(stringValue(*it)));
+        else if (name == "rhythm") v.rhythm_ = doubleValue(*it);
+        else if (name == "position") v.position_ = vec3(vectorDoubleValue(*it));
+        else throw JobError(name + " is not defined in seed-visualization group.");
+        next(it);
+      } while (*it != "}");
+    } else if (*it == "field-profile") {  // src/datainput.cpp:400-418
+      if (*next(it) != "{") throw JobError("The seed-profile directory is empty");
+      next(it);
+      do {
+        const std::string name = parameterName(*it);
+        if (name == "sample") seed_.profile_ = boolValue(*it);
+        else if (name == "directory") seed_.profileDirectory_ = stringValue(*it);
+        else if (name == "base-name") seed_.profileBasename_ = stringValue(*it);
+        else if (name == "time") seed_.profileTime_.push_back(doubleValue(*it));
+        else if (name == "field") seed_.profileField_.push_back(fieldTypeOf(stringValue(*it)));
+        else if (name == "rhythm") seed_.profileRhythm_ = doubleValue(*it);
+        else throw JobError(name + " is not defined in seed-profile group.");
         next(it);
       } while (*it != "}");
     }
@@ -645,17 +701,36 @@ void ParseDarius::readFEL(It& it) {
         next(it);
       } while (*it != "}");
       FEL_.push_back(F);
-    } else if (*it == "power-visualization" || *it == "bunch-profile-lab-frame") {
-      const std::string group = *it;
-      if (*next(it) != "{") throw JobError("The " + group + " directory is empty");
+    } else if (*it == "power-visualization") {  // src/datainput.cpp:674-692
+      if (*next(it) != "{") throw JobError("The power-visualization directory is empty");
       next(it);
       FreeElectronLaser F;
+      auto& v = F.vtkPower_;
       do {
         const std::string name = parameterName(*it);
-        if (name == "sample") {
-          if (group == "power-visualization") F.vtkPower_ = boolValue(*it);
-          else F.screenProfile_ = boolValue(*it);
-        }
+        if (name == "sample") v.sampling_ = boolValue(*it);
+        else if (name == "directory") v.directory_ = stringValue(*it);
+        else if (name == "base-name") v.basename_ = stringValue(*it);
+        else if (name == "plane-position") v.z_ = doubleValue(*it);
+        else if (name == "rhythm") v.rhythm_ = doubleValue(*it);
+        else if (name == "normalized-frequency") v.lambda_ = doubleValue(*it);
+        else throw JobError(name + " is not defined in power-visualization group.");
+        next(it);
+      } while (*it != "}");
+      FEL_.push_back(F);
+    } else if (*it == "bunch-profile-lab-frame") {  // src/datainput.cpp:726-745
+      if (*next(it) != "{") throw JobError("The bunch-profile-lab-frame directory is empty");
+      next(it);
+      FreeElectronLaser F;
+      auto& sp = F.screenProfile_;
+      do {
+        const std::string name = parameterName(*it);
+        if (name == "sample") sp.sampling_ = boolValue(*it);
+        else if (name == "directory") sp.directory_ = stringValue(*it);
+        else if (name == "base-name") sp.basename_ = stringValue(*it);
+        else if (name == "position") sp.pos_.push_back(doubleValue(*it));
+        else if (name == "rhythm") sp.rhythm_ = doubleValue(*it);
+        else throw JobError(name + " is not defined in the bunch-profile-lab-frame group.");
         next(it);
       } while (*it != "}");
       FEL_.push_back(F);

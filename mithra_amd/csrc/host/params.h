@@ -30,6 +30,8 @@ enum SeedType {
   STANDINGPLANEWAVE, STANDINGPLANEWAVETRUNCATED, STANDINGGAUSSIANBEAM, STANDINGSUPERGAUSSIANBEAM
 };
 enum SamplingType { ATPOINT, OVERLINE, INPLANE, ALLDOMAIN };
+enum FieldType { Ex, Ey, Ez, Bx, By, Bz, Ax, Ay, Az, F };  // src/classes.cpp:896-908
+enum PlaneType { XNORMAL, YNORMAL, ZNORMAL };              // src/classes.cpp:887-893
 enum UndulatorType { STATIC, OPTICAL };
 enum SolverType { FD, NSFD };
 
@@ -125,10 +127,31 @@ struct Seed {
   Double l_ = 0.0;
   std::vector<Double> zR_ = std::vector<Double>(2, 0.0);
   Double beta_ = 0.0, gamma_ = 1.0, dt_ = 0.0;
-  // field sampling / visualisation / profile requests (kept for the grammar; their writers
-  // are host diagnostics outside the device time march)
-  bool sampling_ = false, profile_ = false;
-  int vtkCount_ = 0;
+  // field-sampling, field-visualization and field-profile groups (src/classes.cpp:623-641,
+  // :858-868; read by src/datainput.cpp:330-427)
+  bool sampling_ = false;
+  SamplingType samplingType_ = ATPOINT;
+  std::vector<FieldType> samplingField_;
+  std::string samplingDirectory_, samplingBasename_;
+  Double samplingRhythm_ = 0.0;
+  std::vector<Vec3> samplingPosition_;
+  Vec3 samplingLineBegin_, samplingLineEnd_;
+  Double samplingRes_ = 0.0;
+  struct Vtk {
+    bool sample_ = false;
+    std::vector<FieldType> field_;
+    std::string directory_, basename_;
+    Double rhythm_ = 0.0;
+    SamplingType type_ = ALLDOMAIN;
+    PlaneType plane_ = ZNORMAL;
+    Vec3 position_;
+  };
+  std::vector<Vtk> vtk_;
+  bool profile_ = false;
+  std::vector<FieldType> profileField_;
+  std::string profileDirectory_, profileBasename_;
+  std::vector<Double> profileTime_;
+  Double profileRhythm_ = 0.0;
   void initialize(const std::string& type, const std::vector<Double>& position, const std::vector<Double>& direction,
                   const std::vector<Double>& polarization, Double a0, const std::vector<Double>& radius,
                   const std::vector<int>& order, const Signal& signal);
@@ -172,7 +195,19 @@ struct FreeElectronLaser {
     unsigned int lambdaRes_ = 0;
   };
   RadiationSampling radiationPower_;
-  bool vtkPower_ = false, screenProfile_ = false;
+  struct RadiationVisualization {  // power-visualization (src/datainput.cpp:674-692)
+    bool sampling_ = false;
+    std::string directory_, basename_;
+    Double z_ = 0.0, lambda_ = 0.0, rhythm_ = 0.0;
+  };
+  RadiationVisualization vtkPower_;
+  struct ScreenProfile {  // bunch-profile-lab-frame (src/datainput.cpp:726-745, src/classes.cpp:1324-1331)
+    bool sampling_ = false;
+    std::string directory_ = "./", basename_;
+    std::vector<Double> pos_;
+    Double rhythm_ = 0.0;
+  };
+  ScreenProfile screenProfile_;
 };
 
 // Job-file reader and grammar (reference src/readdata.cpp, src/datainput.cpp).

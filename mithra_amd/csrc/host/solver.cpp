@@ -468,6 +468,7 @@ void Solver::initializeBunchUpdate() {
   dtb_ = c0_ * bunch_.timeStep_;
   r1_ = -EC / (EM * c0_) * bunch_.timeStep_ / 2.0;
   r2_ = -EC / EM * bunch_.timeStep_ / 2.0;
+  initializeBunchDiagnostics();
 }
 
 void Solver::initializePowerSample() {
@@ -532,6 +533,8 @@ void Solver::initialize() {
   for (auto& u : undulator_)
     if (u.type_ == OPTICAL) throw JobError("Optical undulators are not supported by the GPU time march.");
   if (!extField_.empty()) throw JobError("EXTERNAL-FIELD beams are not supported by the GPU time march.");
+  for (auto& f : FEL_)
+    if (f.vtkPower_.sampling_) throw JobError("Power visualization is not supported by this build.");
   setSimulationParameters();
   initializeBunch();
   timeBunch_ = time_;
@@ -539,8 +542,12 @@ void Solver::initialize() {
   initializeMesh();
   lorentzBoostBunch();
   initializeField();
+  if (seed_.sampling_) initializeSeedSampling();
+  initializeSeedVTK();
+  if (seed_.profile_) initializeSeedProfile();
   initializeBunchUpdate();
   initializePowerSample();
+  initializeScreenProfile();
   shiftBackInTime();
   initialized_ = true;
 }
@@ -690,6 +697,7 @@ long Solver::solve(long max_steps, int precision, bool quiet) {
     pending = 0;
     tbs.clear();
   };
+  const bool diag = diagnosticsActive();
   timeval start, now;
   gettimeofday(&start, nullptr);
   // Particle-only march before time zero when a back shift is set (src/solver.cpp:1232-1291).
@@ -703,6 +711,10 @@ long Solver::solve(long max_steps, int precision, bool quiet) {
     }
     check(mithra_dev_bunch_advance(dev_, &bp, und.data(), (int)und.size(), timeBunch_, bunch_.timeStep_, nsub), "push");
     for (int s = 0; s < nsub; ++s) { timeBunch_ += bunch_.timeStep_; ++nTimeBunch_; }
+    if (diag) {
+      screenProfile();
+      bunchWriters();
+    }
     timem1_ += mesh_.timeStep_;
     time_ += mesh_.timeStep_;
     timep1_ += mesh_.timeStep_;
@@ -710,9 +722,18 @@ long Solver::solve(long max_steps, int precision, bool quiet) {
   }
   long steps = 0;
   while (time_ < mesh_.totalTime_ && (max_steps < 0 || steps < max_steps)) {
+    // a step whose field writers fire evaluates E/B on every node, not only where the bunch reads
+    const bool fieldDue = diag && fieldWritersDue();
+    if (fieldDue) check(mithra_dev_set_lazy_fields(dev_, 0), "field evaluation mode");
     check(mithra_dev_field_update(dev_, time_, &sd), "field update");
+    if (fieldDue) check(mithra_dev_set_lazy_fields(dev_, 1), "field evaluation mode");
     check(mithra_dev_bunch_advance(dev_, &bp, und.data(), (int)und.size(), timeBunch_, bunch_.timeStep_, nsub), "push");
     for (int s = 0; s < nsub; ++s) { timeBunch_ += bunch_.timeStep_; ++nTimeBunch_; }
+    if (diag) {
+      if (fieldDue) fieldWriters();
+      bunchWriters();
+      screenProfile();
+    }
     if (powerFEL_ >= 0) {
       check(mithra_dev_power_sample(dev_, nTime_, pending), "power sample");
       tbs.push_back(timeBunch_);
@@ -745,6 +766,10 @@ long Solver::solve(long max_steps, int precision, bool quiet) {
                    " particle sub-steps crossed the periodic z boundary of the domain (the reference duplicates such "
                    "particles on one rank; this build does not).");
   for (auto& f : pfile_) f->flush();
+  if (sfFile_) sfFile_->flush();
+  if (sbFile_) sbFile_->flush();
+  for (auto& v : scrFiles_)
+    for (auto& f : v) f->flush();
   return steps;
 }
 

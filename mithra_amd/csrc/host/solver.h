@@ -35,6 +35,23 @@ class Solver {
   void shiftBackInTime();          // src/solver.cpp:1184-1206
   void initialize();
 
+  // Diagnostics writers (diagnostics.cpp): set-up in Solver::initialize's order, writers called
+  // from solve() where the reference's main loop calls them (src/solver.cpp:1248-1370).
+  void initializeSeedSampling();   // src/solver.cpp:848-932
+  void initializeSeedVTK();        // src/solver.cpp:938-1016
+  void initializeSeedProfile();    // src/solver.cpp:1022-1044
+  void initializeBunchDiagnostics();  // src/solver.cpp:1061-1126
+  void initializeScreenProfile();  // src/solver.cpp:2145-2199
+  void fieldSample();              // src/fdtd.cpp:851-950, src/fdtdSC.cpp:1147-1262
+  void fieldVisualizeAllDomain(unsigned ivtk);  // src/fdtd.cpp:956-1106
+  void fieldVisualizeInPlane(unsigned ivtk);    // src/fdtd.cpp:1112-1545
+  void fieldProfile();             // src/fdtd.cpp:1546-1595, src/fdtdSC.cpp:1857-1910
+  void bunchSample();              // src/solver.cpp:1582-1641
+  void bunchVisualize();           // src/solver.cpp:1647-1757
+  void bunchProfile();             // src/solver.cpp:1763-1792
+  void screenProfile();            // src/solver.cpp:2205-2257
+  Vec3 rc(long m) const;           // src/solver.cpp:2302-2315
+
   // Device hand-over of the state built by initialize().
   void upload(int precision);
   // Solver::solve main loop (src/solver.cpp:1212-1414); max_steps < 0 runs to totalTime_.
@@ -97,6 +114,23 @@ class Solver {
   void deviceBunch();
   void deviceBoost();
   void writePowerRows(const std::vector<double>& rows, const std::vector<double>& tb, unsigned count);
+  // diagnostics state
+  bool diagnosticsActive() const;
+  bool fieldWritersDue() const;
+  void fieldWriters();
+  void bunchWriters();
+  void fetchFull();
+  Double nodeValue(FieldType f, long m) const;
+  const std::vector<double>& fetchParticles();
+  std::unique_ptr<std::ofstream> sfFile_, sbFile_;
+  Double sfCe_ = 0, sfCb_ = 0, sfCa_ = 0, sfCf_ = 0;  // SampleField unit factors
+  std::vector<std::string> vfName_;                   // file part of each visualization's base name
+  std::vector<std::vector<std::unique_ptr<std::ofstream>>> scrFiles_;
+  bool fullValid_ = false;
+  std::vector<float> fullE_, fullB_;
+  std::vector<double> fullA_, fullF_, partCache_, scrRec_;
+  std::vector<int> scrScreen_;
+  std::vector<long> scrIndex_;
 };
 
 // The two plugins of the reference, selected by the space-charge flag; on the GPU both run the
