@@ -236,6 +236,25 @@ typedef struct mithra_screen_params {
 int mithra_dev_screen_crossings(mithra_dev* dev, const double* lz, int n_screens, const mithra_screen_params* p,
                                 long cap, long* count, int* screen, long* index, double* rec6);
 
+/* Power map of one plane (Solver::initializePowerVisualize / powerVisualize,
+ * src/radiation.cpp:238-450).  setup: the plane's cell k and fraction dzr (modf of its boosted
+ * position over dz), the ring length Nf, the DFT phases cos(w j dt), sin(w j dt) for j < Nf, the
+ * power constant pc and the frame; the ring starts at zero, and the lazy E/B evaluation covers
+ * the plane from then on.  sample: the lab-frame Ex, Ey, Bx, By of the plane into ring row
+ * n_time % Nf (every field step, after the bunch push).  fetch: the power of every node of the
+ * plane, out[N1 i + j] (zero on the boundary nodes), from the ring as it stands. */
+typedef struct mithra_power_map_params {
+  int k;
+  double dzr;
+  unsigned Nf;
+  const double* cos_wt;
+  const double* sin_wt;
+  double pc, gamma, beta, c0;
+} mithra_power_map_params;
+int mithra_dev_power_map_setup(mithra_dev* dev, const mithra_power_map_params* p);
+int mithra_dev_power_map_sample(mithra_dev* dev, unsigned n_time);
+int mithra_dev_power_map_fetch(mithra_dev* dev, double* out);
+
 /* Rotate A^{n-1} <- A^n <- A^{n+1} (and phi): FdTd::fieldShift (src/fdtd.cpp:806-812). */
 int mithra_dev_field_shift(mithra_dev* dev);
 

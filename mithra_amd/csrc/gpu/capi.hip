@@ -239,6 +239,7 @@ int mithra_dev_create(const mithra_grid* grid, const mithra_coeffs* coeffs, int 
   d->sort_interval = kDefaultSortInterval;
   d->pk_lo = 1;
   d->pk_hi = 0;
+  d->vk = -1;
   hipError_t e = hipGetDevice(&d->device);
   if (e != hipSuccess) { delete d; return hip_status(e); }
   e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
@@ -280,6 +281,9 @@ int mithra_dev_destroy(mithra_dev* d) {
   hipFree(d->und_dev);
   free_power(d);
   hipFree(d->diag);
+  hipFree(d->vring);
+  hipFree(d->vdft);
+  hipFree(d->vmap);
   if (d->ev_ready)
     for (auto& e : d->ev) hipEventDestroy(e);
   if (d->stream) hipStreamDestroy(d->stream);

@@ -193,7 +193,28 @@ struct Domain {
   // staging of the diagnostics gathers (diagnostics.hip), grown on demand
   void* diag;
   size_t diag_bytes;
+  // power map of one plane (power visualization, diagnostics.hip): ring [Nf][4][N1N0] of the
+  // lab-frame fields, DFT phases [Nf] (cos, sin), the map; vk < 0: none
+  double* vring;
+  double* vdft;
+  double* vmap;
+  unsigned vNf;
+  int vk;
+  double vdzr, vpc, vgamma, vbeta, vc0;
 };
+
+// Planes pb0..pb1 the lazy E/B evaluation adds to the bunch's: the power-sampling planes' cells
+// and the power map's cell, each with the plane above (pb1 < 0: none).
+inline void lazy_planes(const Domain* d, int* pb0, int* pb1) {
+  int lo = d->pk_lo, hi = d->pk_hi;
+  if (d->vk >= 0) {
+    if (lo > hi) lo = hi = d->vk;
+    lo = lo < d->vk ? lo : d->vk;
+    hi = hi > d->vk ? hi : d->vk;
+  }
+  *pb0 = lo;
+  *pb1 = lo <= hi ? hi + 1 : -1;
+}
 
 // Correctly rounded x / d given r = RN(1 / d): q = RN(x r) is within one ulp of x / d, and one
 // residual step with exact FMA residual rounds it correctly (Markstein, IBM J. Res. Dev. 34,

@@ -11,6 +11,9 @@
 //                     do not change from run to run.
 //   screen_crossings  the particles that crossed a lab-frame screen during the last field step
 //                     and the record Solver::screenProfile writes for each (src/solver.cpp:2205-2257).
+//   power_map_*       Solver::powerVisualize (src/radiation.cpp:324-450): a ring of the plane's
+//                     lab-frame fields updated every step, and the per-node DFT power at the
+//                     steps the host writes the map.
 #include <algorithm>
 #include <vector>
 
@@ -136,6 +139,77 @@ __global__ __launch_bounds__(256) void k_screens(const double* __restrict__ part
 
 __global__ void k_zero_count(unsigned long long* c) { *c = 0ULL; }
 
+struct MapConst {
+  int N0, N1, k;
+  long N1N0;
+  unsigned Nf;
+  double dzr, gamma, beta, c0, pc;
+};
+
+// Solver::powerVisualize's time samples (src/radiation.cpp:346-377): the lab-frame Ex, Ey, Bx, By
+// of the plane, interpolated between cells k and k + 1, into ring row m for every interior node.
+__global__ __launch_bounds__(256) void k_map_sample(const float* __restrict__ eb, MapConst c, unsigned m,
+                                                    double* __restrict__ ring) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni_total = (long)(c.N0 - 2) * (c.N1 - 2);
+  if (q >= ni_total) return;
+  const int i = 1 + (int)(q / (c.N1 - 2)), j = 1 + (int)(q % (c.N1 - 2));
+  const long ni = (long)i * c.N1 + j;
+  const long mi = (long)c.k * c.N1N0 + ni;
+  const float* lo = eb + 6 * mi;
+  const float* hi = eb + 6 * (mi + c.N1N0);
+  const double et0 = (1.0 - c.dzr) * lo[0] + c.dzr * hi[0];
+  const double et1 = (1.0 - c.dzr) * lo[1] + c.dzr * hi[1];
+  const double bt0 = (1.0 - c.dzr) * lo[3] + c.dzr * hi[3];
+  const double bt1 = (1.0 - c.dzr) * lo[4] + c.dzr * hi[4];
+  double* r = ring + (size_t)m * 4 * c.N1N0;
+  r[ni] = c.gamma * (et0 + c.c0 * c.beta * bt1);
+  r[c.N1N0 + ni] = c.gamma * (et1 - c.c0 * c.beta * bt0);
+  r[2 * c.N1N0 + ni] = c.gamma * (bt0 - c.beta / c.c0 * et1);
+  r[3 * c.N1N0 + ni] = c.gamma * (bt1 + c.beta / c.c0 * et0);
+}
+
+// pL[ni] = pc (Re(ew1 bw1) - Re(ew2 bw2)) with ew = sum_m f[m] ep[m], bw = sum_m f[m] em[m] in the
+// reference's order (src/radiation.cpp:379-392); ep = cos + i sin, em = cos - i sin.
+__global__ __launch_bounds__(256) void k_map_power(const double* __restrict__ ring, const double* __restrict__ dft,
+                                                   MapConst c, double* __restrict__ out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni_total = (long)(c.N0 - 2) * (c.N1 - 2);
+  if (q >= ni_total) return;
+  const int i = 1 + (int)(q / (c.N1 - 2)), j = 1 + (int)(q % (c.N1 - 2));
+  const long ni = (long)i * c.N1 + j;
+  double e1r = 0.0, e1i = 0.0, b1r = 0.0, b1i = 0.0, e2r = 0.0, e2i = 0.0, b2r = 0.0, b2i = 0.0;
+  for (unsigned m = 0; m < c.Nf; ++m) {
+    const double* r = ring + (size_t)m * 4 * c.N1N0;
+    const double cs = dft[2 * m], sn = dft[2 * m + 1];
+    const double f0 = r[ni], f1 = r[c.N1N0 + ni], f2 = r[2 * c.N1N0 + ni], f3 = r[3 * c.N1N0 + ni];
+    e1r += f0 * cs;
+    e1i += f0 * sn;
+    b1r += f3 * cs;
+    b1i += f3 * -sn;
+    e2r += f1 * cs;
+    e2i += f1 * sn;
+    b2r += f2 * cs;
+    b2i += f2 * -sn;
+  }
+  out[ni] = c.pc * ((e1r * b1r - e1i * b1i) - (e2r * b2r - e2i * b2i));
+}
+
+MapConst map_const(const mithra_dev* d) {
+  MapConst c;
+  c.N0 = d->g.N0;
+  c.N1 = d->g.N1;
+  c.k = d->vk;
+  c.N1N0 = d->N1N0;
+  c.Nf = d->vNf;
+  c.dzr = d->vdzr;
+  c.gamma = d->vgamma;
+  c.beta = d->vbeta;
+  c.c0 = d->vc0;
+  c.pc = d->vpc;
+  return c;
+}
+
 }  // namespace
 
 extern "C" {
@@ -234,6 +308,61 @@ int mithra_dev_screen_crossings(mithra_dev* d, const double* lz, int n_screens, 
     DIAG_TRY(hipStreamSynchronize(d->stream));
   }
   return MITHRA_OK;
+}
+
+int mithra_dev_power_map_setup(mithra_dev* d, const mithra_power_map_params* p) {
+  if (!d || !p || p->Nf == 0 || !p->cos_wt || !p->sin_wt) return MITHRA_E_ARG;
+  if (p->k < 0 || p->k + 1 >= d->g.N2 || d->g.N0 < 3 || d->g.N1 < 3) return MITHRA_E_ARG;
+  DIAG_TRY(hipStreamSynchronize(d->stream));
+  (void)hipFree(d->vring);
+  (void)hipFree(d->vdft);
+  (void)hipFree(d->vmap);
+  d->vring = d->vdft = d->vmap = nullptr;
+  d->vk = -1;
+  const size_t ring = sizeof(double) * p->Nf * 4 * (size_t)d->N1N0;
+  if (hipMalloc(&d->vring, ring) != hipSuccess || hipMalloc(&d->vdft, sizeof(double) * 2 * p->Nf) != hipSuccess ||
+      hipMalloc(&d->vmap, sizeof(double) * (size_t)d->N1N0) != hipSuccess) {
+    (void)hipFree(d->vring);
+    (void)hipFree(d->vdft);
+    (void)hipFree(d->vmap);
+    d->vring = d->vdft = d->vmap = nullptr;
+    return MITHRA_E_NOMEM;
+  }
+  std::vector<double> dft(2 * (size_t)p->Nf);
+  for (unsigned j = 0; j < p->Nf; ++j) {
+    dft[2 * j] = p->cos_wt[j];
+    dft[2 * j + 1] = p->sin_wt[j];
+  }
+  DIAG_TRY(hipMemcpy(d->vdft, dft.data(), sizeof(double) * dft.size(), hipMemcpyHostToDevice));
+  DIAG_TRY(launch_fill_zero(d, d->vring, ring));  // SampleRadiationPower::fdt starts at zero
+  DIAG_TRY(launch_fill_zero(d, d->vmap, sizeof(double) * (size_t)d->N1N0));
+  d->vNf = p->Nf;
+  d->vk = p->k;
+  d->vdzr = p->dzr;
+  d->vpc = p->pc;
+  d->vgamma = p->gamma;
+  d->vbeta = p->beta;
+  d->vc0 = p->c0;
+  return MITHRA_OK;
+}
+
+int mithra_dev_power_map_sample(mithra_dev* d, unsigned n_time) {
+  if (!d || d->vk < 0) return MITHRA_E_STATE;
+  const long n = (long)(d->g.N0 - 2) * (d->g.N1 - 2);
+  hipLaunchKernelGGL(k_map_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream, d->eb, map_const(d),
+                     n_time % d->vNf, d->vring);
+  return hip_status(hipGetLastError());
+}
+
+int mithra_dev_power_map_fetch(mithra_dev* d, double* p) {
+  if (!d || !p) return MITHRA_E_ARG;
+  if (d->vk < 0) return MITHRA_E_STATE;
+  const long n = (long)(d->g.N0 - 2) * (d->g.N1 - 2);
+  hipLaunchKernelGGL(k_map_power, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream, d->vring, d->vdft,
+                     map_const(d), d->vmap);
+  DIAG_TRY(hipGetLastError());
+  DIAG_TRY(hipMemcpyAsync(p, d->vmap, sizeof(double) * (size_t)d->N1N0, hipMemcpyDeviceToHost, d->stream));
+  return hip_status(hipStreamSynchronize(d->stream));
 }
 
 }  // extern "C"

@@ -1212,8 +1212,7 @@ hipError_t stencil_eb_impl(Domain* d) {
   int pb0 = 0, pb1 = -1;
   if (d->lazy_fields && d->zrange_valid) {
     zr = d->zrange;
-    pb0 = d->pk_lo;
-    pb1 = d->pk_lo <= d->pk_hi ? d->pk_hi + 1 : -1;
+    lazy_planes(d, &pb0, &pb1);
   }
   return d->g.solver == MITHRA_SOLVER_NSFD ? stencil_eb_variant<R, true>(d, zr, pb0, pb1)
                                            : stencil_eb_variant<R, false>(d, zr, pb0, pb1);
@@ -1308,7 +1307,8 @@ hipError_t launch_eval_eb(Domain* d) { return launch_eval_eb_range(d, 1, d->g.N2
 
 hipError_t launch_eval_eb_step(Domain* d) {
   if (!d->lazy_fields || !d->zrange_valid) return launch_eval_eb(d);
-  const int pb0 = d->pk_lo, pb1 = d->pk_lo <= d->pk_hi ? d->pk_hi + 1 : -1;
+  int pb0, pb1;
+  lazy_planes(d, &pb0, &pb1);
   return d->precision == MITHRA_PRECISION_FP64 ? eval_impl<double>(d, 1, d->g.N2 - 2, d->zrange, pb0, pb1)
                                                : eval_impl<float>(d, 1, d->g.N2 - 2, d->zrange, pb0, pb1);
 }

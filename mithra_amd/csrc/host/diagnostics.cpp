@@ -10,7 +10,8 @@
 // Solver::initializeScreenProfile (:2145-2199).  Writers: FdTd::fieldSample (src/fdtd.cpp:851-950,
 // FdTdSC src/fdtdSC.cpp:1147-1262), FdTd::fieldVisualizeAllDomain / InPlane{X,Y,Z}Normal
 // (src/fdtd.cpp:956-1545), FdTd::fieldProfile (:1546-1595), Solver::bunchSample /
-// bunchVisualize / bunchProfile (src/solver.cpp:1582-1792), Solver::screenProfile (:2205-2257).
+// bunchVisualize / bunchProfile (src/solver.cpp:1582-1792), Solver::screenProfile (:2205-2257),
+// Solver::initializePowerVisualize / powerVisualize (src/radiation.cpp:238-450).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -50,6 +51,57 @@ std::unique_ptr<std::ofstream> openFile(const std::string& name) {
 
 // One rank: the rank suffix of every per-rank file name is 0 and the slab is the whole mesh.
 const char* const kRank = "0";
+
+void vtsHeader(std::ostream& f, const std::string& extent) {
+  f << "<?xml version=\"1.0\"?>" << std::endl;
+  f << "<VTKFile type=\"StructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+       "compressor=\"vtkZLibDataCompressor\">"
+    << std::endl;
+  f << extent;
+  f << "<Points>" << std::endl;
+  f << "<DataArray type = \"Float64\" NumberOfComponents=\"3\" format=\"ascii\">" << std::endl;
+}
+
+void vtsMiddle(std::ostream& f, size_t components) {
+  f << "</DataArray>" << std::endl;
+  f << "</Points>" << std::endl;
+  f << "<CellData>" << std::endl;
+  f << "</CellData>" << std::endl;
+  f << "<PointData Vectors = \"field\">" << std::endl;
+  f << "<DataArray type=\"Float64\" Name=\"field\" NumberOfComponents=\"" << components << "\" format=\"ascii\">"
+    << std::endl;
+}
+
+void vtsFooter(std::ostream& f) {
+  f << "</DataArray>" << std::endl;
+  f << "</PointData>" << std::endl;
+  f << "</Piece>" << std::endl;
+  f << "</StructuredGrid>" << std::endl;
+  f << "</VTKFile>" << std::endl;
+}
+
+void pvts(const std::string& name, const std::string& whole, const std::string& piece, size_t components) {
+  auto f = openFile(name);
+  *f << "<?xml version=\"1.0\"?>" << std::endl;
+  *f << "<VTKFile type=\"PStructuredGrid\" version=\"0.1\" >" << std::endl;
+  *f << whole;
+  *f << "<PPoints>" << std::endl;
+  *f << "<DataArray type = \"Float64\" NumberOfComponents=\"3\" format=\"ascii\" />" << std::endl;
+  *f << "</PPoints>" << std::endl;
+  *f << "<PPointData>" << std::endl;
+  *f << "<DataArray type=\"Float64\" NumberOfComponents=\"" << components << "\" Name=\"field\" format=\"ascii\" />"
+     << std::endl;
+  *f << "</PPointData>" << std::endl;
+  *f << piece;
+  *f << "</PStructuredGrid>" << std::endl;
+  *f << "</VTKFile>" << std::endl;
+}
+
+void writeRow(std::ostream& f, const std::vector<Double>& v, size_t L, size_t n) {
+  f << v[L * n];
+  for (size_t l = 1; l < L; l++) f << " " << v[L * n + l];
+  f << std::endl;
+}
 
 }  // namespace
 
@@ -205,6 +257,86 @@ void Solver::initializeScreenProfile() {
   }
 }
 
+void Solver::initializePowerVisualize() {
+  vizFEL_ = -1;
+  for (unsigned jf = 0; jf < FEL_.size(); jf++) {
+    auto& V = FEL_[jf].vtkPower_;
+    if (!V.sampling_) continue;
+    if (V.rhythm_ == 0)
+      throw JobError("The power visualization rhythm of the field is zero although power visualization is "
+                     "activated !!!");
+    V.rhythm_ /= gamma_;
+    V.z_ *= gamma_;
+    if (!isabsolute(V.basename_)) V.basename_ = V.directory_ + V.basename_;
+    makeDirectory(V.basename_);
+    if (!(V.z_ < zp_[1] && V.z_ >= zp_[0])) continue;  // Nz = 0: the plane is not in the mesh
+    if (vizFEL_ >= 0) throw JobError("More than one power-visualization block is not supported by this build.");
+    if (undulator_.empty()) throw JobError("Power visualization needs an undulator for its wavelength.");
+    vizFEL_ = (int)jf;
+    const Double dt = undulator_[0].lu_ / V.lambda_ / (gamma_ * c0_);
+    vizNf_ = unsigned(3.0 * dt / mesh_.timeStep_);
+    vizW_ = 2 * PI / dt;
+    vizPc_ = 2.0 * mesh_.meshResolution_[0] * mesh_.meshResolution_[1] / (m0_ * vizNf_ * vizNf_) *
+             std::pow(mesh_.lengthScale_, 2) / std::pow(mesh_.timeScale_, 3);
+    if (vizNf_ == 0) throw JobError("The power visualization needs at least one time sample per period.");
+  }
+}
+
+void Solver::powerVisualizeSetup() {
+  if (vizFEL_ < 0) return;
+  const auto& V = FEL_[vizFEL_].vtkPower_;
+  Double c;
+  mithra_power_map_params p;
+  p.dzr = std::modf((V.z_ - zmin_) / mesh_.meshResolution_[2], &c);
+  p.k = (int)c - k0_;
+  p.Nf = vizNf_;
+  std::vector<double> cs(vizNf_), sn(vizNf_);
+  for (unsigned j = 0; j < vizNf_; j++) {  // src/radiation.cpp:311-318
+    cs[j] = std::cos(vizW_ * j * mesh_.timeStep_);
+    sn[j] = std::sin(vizW_ * j * mesh_.timeStep_);
+  }
+  p.cos_wt = cs.data();
+  p.sin_wt = sn.data();
+  p.pc = vizPc_;
+  p.gamma = gamma_;
+  p.beta = beta_;
+  p.c0 = c0_;
+  check(mithra_dev_power_map_setup(dev_, &p), "power visualization set-up");
+}
+
+void Solver::powerVisualize() {
+  if (vizFEL_ < 0) return;
+  const auto& V = FEL_[vizFEL_].vtkPower_;
+  check(mithra_dev_power_map_sample(dev_, nTime_), "power visualization sample");
+  if (!(std::fmod(time_, V.rhythm_) < mesh_.timeStep_)) return;
+  std::vector<double> pl((size_t)N1N0_);
+  check(mithra_dev_power_map_fetch(dev_, pl.data()), "power visualization");
+  Double c;
+  const Double dzr = std::modf((V.z_ - zmin_) / mesh_.meshResolution_[2], &c);
+  const int k = (int)c - k0_;
+  auto f = openFile(V.basename_ + "-" + std::to_string(nTime_) + ".vts");
+  f->setf(std::ios::scientific);
+  f->precision(4);
+  const std::string box = "0 " + std::to_string(N0_ - 1) + " 0 " + std::to_string(N1_ - 1) + " 0 0";
+  vtsHeader(*f, "<StructuredGrid WholeExtent=\"" + box + "\">\n<Piece Extent=\"" + box + "\">\n");
+  for (int j = 0; j < N1_; j++)
+    for (int i = 0; i < N0_; i++) {
+      const unsigned int m = k * N1_ * N0_ + i * N1_ + j;
+      const Vec3 r1 = rc(m), r2 = rc(m + N1N0_);
+      *f << r1[0] * (1.0 - dzr) + r2[0] * dzr << " " << r1[1] << " " << r1[2] << std::endl;
+    }
+  *f << "</DataArray>" << std::endl;
+  *f << "</Points>" << std::endl;
+  *f << "<CellData>" << std::endl;
+  *f << "</CellData>" << std::endl;
+  *f << "<PointData Vectors = \"power\">" << std::endl;
+  *f << "<DataArray type=\"Float64\" Name=\"power\" NumberOfComponents=\"" << 1 << "\" format=\"ascii\">" << std::endl;
+  for (int j = 0; j < N1_; j++)
+    for (int i = 0; i < N0_; i++) *f << pl[(size_t)i * N1_ + j] << std::endl;
+  vtsFooter(*f);
+  f->close();
+}
+
 /* ----------------------------------------------------------------------- rhythm tests ---- */
 
 // Whether a field writer fires at the current time_ (the tests of Solver::solve,
@@ -261,7 +393,7 @@ bool Solver::diagnosticsActive() const {
     if (v.sample_) return true;
   for (auto& f : FEL_)
     if (f.screenProfile_.sampling_) return true;
-  return false;
+  return vizFEL_ >= 0;
 }
 
 /* ---------------------------------------------------------------------- field writers ---- */
@@ -386,60 +518,6 @@ void Solver::fieldSample() {
   f << std::endl;
 }
 
-namespace {
-
-void vtsHeader(std::ostream& f, const std::string& extent) {
-  f << "<?xml version=\"1.0\"?>" << std::endl;
-  f << "<VTKFile type=\"StructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
-       "compressor=\"vtkZLibDataCompressor\">"
-    << std::endl;
-  f << extent;
-  f << "<Points>" << std::endl;
-  f << "<DataArray type = \"Float64\" NumberOfComponents=\"3\" format=\"ascii\">" << std::endl;
-}
-
-void vtsMiddle(std::ostream& f, size_t components) {
-  f << "</DataArray>" << std::endl;
-  f << "</Points>" << std::endl;
-  f << "<CellData>" << std::endl;
-  f << "</CellData>" << std::endl;
-  f << "<PointData Vectors = \"field\">" << std::endl;
-  f << "<DataArray type=\"Float64\" Name=\"field\" NumberOfComponents=\"" << components << "\" format=\"ascii\">"
-    << std::endl;
-}
-
-void vtsFooter(std::ostream& f) {
-  f << "</DataArray>" << std::endl;
-  f << "</PointData>" << std::endl;
-  f << "</Piece>" << std::endl;
-  f << "</StructuredGrid>" << std::endl;
-  f << "</VTKFile>" << std::endl;
-}
-
-void pvts(const std::string& name, const std::string& whole, const std::string& piece, size_t components) {
-  auto f = openFile(name);
-  *f << "<?xml version=\"1.0\"?>" << std::endl;
-  *f << "<VTKFile type=\"PStructuredGrid\" version=\"0.1\" >" << std::endl;
-  *f << whole;
-  *f << "<PPoints>" << std::endl;
-  *f << "<DataArray type = \"Float64\" NumberOfComponents=\"3\" format=\"ascii\" />" << std::endl;
-  *f << "</PPoints>" << std::endl;
-  *f << "<PPointData>" << std::endl;
-  *f << "<DataArray type=\"Float64\" NumberOfComponents=\"" << components << "\" Name=\"field\" format=\"ascii\" />"
-     << std::endl;
-  *f << "</PPointData>" << std::endl;
-  *f << piece;
-  *f << "</PStructuredGrid>" << std::endl;
-  *f << "</VTKFile>" << std::endl;
-}
-
-void writeRow(std::ostream& f, const std::vector<Double>& v, size_t L, size_t n) {
-  f << v[L * n];
-  for (size_t l = 1; l < L; l++) f << " " << v[L * n + l];
-  f << std::endl;
-}
-
-}  // namespace
 
 void Solver::fieldVisualizeAllDomain(unsigned ivtk) {
   const Seed::Vtk& V = seed_.vtk_[ivtk];

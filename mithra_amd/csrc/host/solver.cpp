@@ -533,8 +533,6 @@ void Solver::initialize() {
   for (auto& u : undulator_)
     if (u.type_ == OPTICAL) throw JobError("Optical undulators are not supported by the GPU time march.");
   if (!extField_.empty()) throw JobError("EXTERNAL-FIELD beams are not supported by the GPU time march.");
-  for (auto& f : FEL_)
-    if (f.vtkPower_.sampling_) throw JobError("Power visualization is not supported by this build.");
   setSimulationParameters();
   initializeBunch();
   timeBunch_ = time_;
@@ -547,6 +545,7 @@ void Solver::initialize() {
   if (seed_.profile_) initializeSeedProfile();
   initializeBunchUpdate();
   initializePowerSample();
+  initializePowerVisualize();
   initializeScreenProfile();
   shiftBackInTime();
   initialized_ = true;
@@ -698,6 +697,7 @@ long Solver::solve(long max_steps, int precision, bool quiet) {
     tbs.clear();
   };
   const bool diag = diagnosticsActive();
+  powerVisualizeSetup();
   timeval start, now;
   gettimeofday(&start, nullptr);
   // Particle-only march before time zero when a back shift is set (src/solver.cpp:1232-1291).
@@ -739,6 +739,7 @@ long Solver::solve(long max_steps, int precision, bool quiet) {
       tbs.push_back(timeBunch_);
       if (++pending == kRows) flush();
     }
+    if (diag) powerVisualize();
     check(mithra_dev_field_shift(dev_), "field shift");
     check(mithra_dev_current_reset(dev_), "current reset");
     check(mithra_dev_current_update(dev_), "current update");

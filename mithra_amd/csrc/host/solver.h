@@ -50,6 +50,8 @@ class Solver {
   void bunchVisualize();           // src/solver.cpp:1647-1757
   void bunchProfile();             // src/solver.cpp:1763-1792
   void screenProfile();            // src/solver.cpp:2205-2257
+  void initializePowerVisualize(); // src/radiation.cpp:238-318
+  void powerVisualize();           // src/radiation.cpp:324-450
   Vec3 rc(long m) const;           // src/solver.cpp:2302-2315
 
   // Device hand-over of the state built by initialize().
@@ -131,6 +133,10 @@ class Solver {
   std::vector<double> fullA_, fullF_, partCache_, scrRec_;
   std::vector<int> scrScreen_;
   std::vector<long> scrIndex_;
+  void powerVisualizeSetup();
+  int vizFEL_ = -1;  // the FEL block whose power map is written, or -1
+  unsigned vizNf_ = 0;
+  Double vizW_ = 0, vizPc_ = 0;
 };
 
 // The two plugins of the reference, selected by the space-charge flag; on the GPU both run the
