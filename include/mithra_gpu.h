@@ -48,9 +48,12 @@ enum {
 };
 enum {
   MITHRA_SEED_PLANEWAVE = 0, MITHRA_SEED_PLANEWAVE_TRUNCATED = 1, MITHRA_SEED_GAUSSIANBEAM = 2,
-  MITHRA_SEED_SUPERGAUSSIANBEAM = 3
-  /* the standing-wave seed types 4..7 are not handled by the reference's Seed::fields
-     (src/classes.cpp:758-851) and are rejected with MITHRA_E_UNSUPPORTED */
+  MITHRA_SEED_SUPERGAUSSIANBEAM = 3, MITHRA_SEED_STANDING_PLANEWAVE = 4,
+  MITHRA_SEED_STANDING_PLANEWAVE_TRUNCATED = 5, MITHRA_SEED_STANDING_GAUSSIANBEAM = 6,
+  MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM = 7
+  /* the standing-wave types 4..7 are beam types of optical undulators and external fields
+     (src/beam.cc:280-496); the reference's Seed::fields does not handle them
+     (src/classes.cpp:758-851) and a seed of those types is rejected with MITHRA_E_UNSUPPORTED */
 };
 enum { MITHRA_UNDULATOR_STATIC = 0, MITHRA_UNDULATOR_OPTICAL = 1 };
 enum { MITHRA_PRECISION_FP64 = 0, MITHRA_PRECISION_FP32_BULK = 1 };
@@ -95,12 +98,30 @@ typedef struct mithra_seed {
   double gamma, beta, dt_shift, c0;
 } mithra_seed;
 
-/* One undulator module (class Undulator, src/classes.h) after sorting and shifting. */
+/* One undulator module (class Undulator, src/classes.h) after sorting and shifting.  An
+ * optical undulator's laser beam is the next unused entry of the optical beams given to
+ * mithra_dev_set_beams, in table order. */
 typedef struct mithra_undulator {
-  int type;                       /* only MITHRA_UNDULATOR_STATIC is supported           */
+  int type;                       /* MITHRA_UNDULATOR_STATIC or MITHRA_UNDULATOR_OPTICAL  */
   unsigned length;                /* number of periods                                   */
   double k, lu, rb, dist, theta;
 } mithra_undulator;
+
+/* An analytic laser beam the bunch push adds in the lab frame: the beam of an optical undulator
+ * (class Undulator) or of an EXTERNAL-FIELD electromagnetic wave (class ExtField), after
+ * Undulator::initialize / ExtField::initialize (src/classes.cpp:1007-1102, :1162-1253) and the
+ * scaling of Solver::setSimulationParameters (src/solver.cpp:88-123). */
+typedef struct mithra_beam {
+  int beam_type;                  /* MITHRA_SEED_* (all eight types)                     */
+  int signal_type;                /* MITHRA_SIGNAL_*                                     */
+  int order[2];                   /* super-gaussian orders                               */
+  unsigned nR;                    /* flat-top rising cycles                              */
+  double amplitude;               /* field amplitude, amplitude_                          */
+  double position[3], direction[3], polarization[3];
+  double radius[2], zR[2], l;     /* radius_, Rayleigh lengths zR_, wavelength l_          */
+  double t0, s, f0, cep;          /* Signal members after scaling                        */
+  double sigma_invg[2];
+} mithra_beam;
 
 /* Constants of one bunch sub-step (Solver::bunchUpdate, src/solver.cpp:1424-1549, with the
  * UpdateBunch values of Solver::initializeBunchUpdate, src/solver.cpp:1050-1059). */
@@ -190,6 +211,21 @@ int mithra_dev_bunch_update(mithra_dev* dev, const mithra_bunch_params* p,
  * field step, so each particle is loaded once and advanced n_sub times in registers. */
 int mithra_dev_bunch_advance(mithra_dev* dev, const mithra_bunch_params* p, const mithra_undulator* und,
                              int n_und, double time_bunch, double bunch_time_step, int n_sub);
+
+/* The laser beams of the push: optical[q] is the beam of the q-th optical undulator of the
+ * undulator table, external[q] the q-th EXTERNAL-FIELD wave.  Each sub-step adds, per particle,
+ * every undulator's field in table order (static: Solver::staticUndulator; optical: the beam's
+ * lab-frame E and B transformed into the bunch frame, Solver::undulatorField,
+ * src/solver.cpp:1825-1877) and then every external wave's (Solver::externalField,
+ * src/solver.cpp:1886-1947), before the grid's E/B, with the beam fields of src/beam.cc:80-496.
+ * Replaces the previous beams; n = 0 removes them.  Two uses of state the reference leaves
+ * undefined are given their evident meaning: a standing-wave optical undulator takes its
+ * backward-wave time from its own beam (the reference reads the value the previous external
+ * wave left in UpdateBunchParallel::tlm, src/solver.cpp:1836-1838), and a standing super-gaussian
+ * beam's phase uses its wavelength l_ (the reference reads the never-set
+ * UpdateBunchParallel::l, src/beam.cc:466). */
+int mithra_dev_set_beams(mithra_dev* dev, const mithra_beam* optical, int n_optical, const mithra_beam* external,
+                         int n_external);
 
 /* Number of particles that were in the z-range but outside the transverse domain during
  * the bunch updates since the last call (UpdateBunch::nt, src/solver.cpp:1500-1505,

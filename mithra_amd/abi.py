@@ -69,6 +69,15 @@ class PowerParams(C.Structure):
                 ("z", _dp), ("w", _dp)]
 
 
+class Beam(C.Structure):
+    """mithra_beam: the laser beam of an optical undulator or an external field."""
+    _fields_ = [("beam_type", C.c_int), ("signal_type", C.c_int), ("order", C.c_int * 2), ("nR", C.c_uint),
+                ("amplitude", C.c_double), ("position", C.c_double * 3), ("direction", C.c_double * 3),
+                ("polarization", C.c_double * 3), ("radius", C.c_double * 2), ("zR", C.c_double * 2),
+                ("l", C.c_double), ("t0", C.c_double), ("s", C.c_double), ("f0", C.c_double), ("cep", C.c_double),
+                ("sigma_invg", C.c_double * 2)]
+
+
 class ScreenParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "beta", "c0", "time_bunch", "time_step", "dt_shift")]
 
@@ -184,6 +193,7 @@ def _declare(lib, prefix):
         "current_reset": (C.c_int, [vp]),
         "current_update": (C.c_int, [vp]),
         "bunching": (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)]),
+        "set_beams": (C.c_int, [vp, C.POINTER(Beam), C.c_int, C.POINTER(Beam), C.c_int]),
     }
     if prefix == "mithra_dev_":
         sig["set_reorder_interval"] = (C.c_int, [vp, C.c_int])
@@ -446,6 +456,12 @@ class Domain:
         ms = (C.c_float * (2 * TILE_VARIANTS))()
         self._check(self.f["autotune"](self.h, reps, chosen, ms))
         return (chosen[0], chosen[1]), np.array(ms[:]).reshape(2, TILE_VARIANTS)
+
+    def set_beams(self, optical, external):
+        """Laser beams of the optical undulators (table order) and of the external fields."""
+        o = (Beam * max(len(optical), 1))(*optical)
+        e = (Beam * max(len(external), 1))(*external)
+        self._check(self.f["set_beams"](self.h, o, len(optical), e, len(external)))
 
     def bunching(self, k):
         """|sum q exp(i k z)| / |sum q| over the particles."""

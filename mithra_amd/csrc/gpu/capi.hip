@@ -82,8 +82,12 @@ bool valid_grid(const mithra_grid* g) {
 
 int mithra::upload_undulators(mithra_dev* d, const mithra_undulator* und, int n, double c0) {
   if (n < 0 || (n > 0 && !und)) return MITHRA_E_ARG;
-  for (int u = 0; u < n; ++u)
-    if (und[u].type != MITHRA_UNDULATOR_STATIC) return MITHRA_E_UNSUPPORTED;
+  int optical = 0;
+  for (int u = 0; u < n; ++u) {
+    if (und[u].type == MITHRA_UNDULATOR_OPTICAL) ++optical;
+    else if (und[u].type != MITHRA_UNDULATOR_STATIC) return MITHRA_E_ARG;
+  }
+  if (optical > d->n_opt) return MITHRA_E_STATE;  // an optical undulator without its beam
   if ((int)d->und_host.size() == n && d->und_c0 == c0 &&
       (n == 0 || std::memcmp(d->und_host.data(), und, sizeof(*und) * n) == 0))
     return MITHRA_OK;
@@ -94,10 +98,13 @@ int mithra::upload_undulators(mithra_dev* d, const mithra_undulator* und, int n,
     d->und_cap = n;
   }
   std::vector<UndDev> t(n);
+  int beam = 0;
   for (int u = 0; u < n; ++u) {  // the push's expressions, src/solver.cpp:1805-1812
     const mithra_undulator& U = und[u];
     UndDev& v = t[u];
     v.k = U.k; v.lu = U.lu; v.rb = U.rb; v.dist = U.dist; v.theta = U.theta; v.length = U.length;
+    v.type = U.type;
+    v.beam = U.type == MITHRA_UNDULATOR_OPTICAL ? beam++ : -1;
     v.b0 = (U.lu != 0.0) ? kEM * c0 * 2 * kPI / U.lu * U.k / kEC : 0.0;
     v.ku = (U.lu != 0.0) ? 2 * kPI / U.lu : 0.0;
     v.ct = std::cos(U.theta);
@@ -279,6 +286,7 @@ int mithra_dev_destroy(mithra_dev* d) {
   hipFree(d->pid);
   free_sort(d);
   hipFree(d->und_dev);
+  hipFree(d->beams);
   free_power(d);
   hipFree(d->diag);
   hipFree(d->vring);
@@ -288,6 +296,48 @@ int mithra_dev_destroy(mithra_dev* d) {
     for (auto& e : d->ev) hipEventDestroy(e);
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
+  return MITHRA_OK;
+}
+
+int mithra_dev_set_beams(mithra_dev* d, const mithra_beam* optical, int n_optical, const mithra_beam* external,
+                         int n_external) {
+  if (!d || n_optical < 0 || n_external < 0 || (n_optical > 0 && !optical) || (n_external > 0 && !external))
+    return MITHRA_E_ARG;
+  std::vector<BeamDev> t;
+  for (int q = 0; q < n_optical + n_external; ++q) {
+    const mithra_beam& b = q < n_optical ? optical[q] : external[q - n_optical];
+    if (b.beam_type < MITHRA_SEED_PLANEWAVE || b.beam_type > MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM ||
+        b.signal_type < MITHRA_SIGNAL_NEUMANN || b.signal_type > MITHRA_SIGNAL_INVGAUSSIAN || b.order[0] < 0 ||
+        b.order[1] < 0)
+      return MITHRA_E_ARG;
+    BeamDev v;
+    v.beam_type = b.beam_type; v.signal_type = b.signal_type;
+    v.order[0] = b.order[0]; v.order[1] = b.order[1]; v.nR = b.nR;
+    v.amplitude = b.amplitude;
+    for (int c = 0; c < 3; ++c) {
+      v.position[c] = b.position[c]; v.direction[c] = b.direction[c]; v.polarization[c] = b.polarization[c];
+    }
+    // cross(direction_, polarization_) (src/fieldvector.h:197-203)
+    v.yv[0] = b.direction[1] * b.polarization[2] - b.direction[2] * b.polarization[1];
+    v.yv[1] = b.direction[2] * b.polarization[0] - b.direction[0] * b.polarization[2];
+    v.yv[2] = b.direction[0] * b.polarization[1] - b.direction[1] * b.polarization[0];
+    v.radius[0] = b.radius[0]; v.radius[1] = b.radius[1];
+    v.zR[0] = b.zR[0]; v.zR[1] = b.zR[1]; v.l = b.l;
+    v.t0 = b.t0; v.s = b.s; v.f0 = b.f0; v.cep = b.cep;
+    v.sigma_invg[0] = b.sigma_invg[0]; v.sigma_invg[1] = b.sigma_invg[1];
+    t.push_back(v);
+  }
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  MITHRA_TRY(hipFree(d->beams));
+  d->beams = nullptr;
+  d->n_opt = d->n_ext = 0;
+  d->und_host.clear();  // the undulator table is checked against the new beams at the next push
+  if (!t.empty()) {
+    if (hipMalloc(&d->beams, sizeof(BeamDev) * t.size()) != hipSuccess) return MITHRA_E_NOMEM;
+    MITHRA_TRY(hipMemcpy(d->beams, t.data(), sizeof(BeamDev) * t.size(), hipMemcpyHostToDevice));
+  }
+  d->n_opt = n_optical;
+  d->n_ext = n_external;
   return MITHRA_OK;
 }
 

@@ -99,8 +99,53 @@ struct EvalConst {  // -dt, twice the steps, and their correctly rounded recipro
 struct UndDev {
   double k, lu, rb, dist, theta;
   unsigned length;
+  int type;                 // MITHRA_UNDULATOR_*
+  int beam;                 // optical: index of its laser beam in Domain::beams
   double b0, ku, ct, st, L;
 };
+
+// A laser beam of an optical undulator or an external field (mithra_beam) with its transverse
+// axis yv = direction x polarization, computed once at upload as the reference computes it for
+// every particle (cross(direction_, polarization_), src/beam.cc:96, :121).
+struct BeamDev {
+  int beam_type, signal_type;
+  int order[2];
+  unsigned nR;
+  double amplitude;
+  double position[3], direction[3], polarization[3], yv[3];
+  double radius[2], zR[2], l;
+  double t0, s, f0, cep;
+  double sigma_invg[2];
+};
+
+// Signal::self (src/classes.cpp:534-575) of a seed or a beam: the carrier at phase `phase` times
+// the pulse envelope of the signal type.
+template <typename S>
+__device__ double signal(const S& s, double t, double phase) {
+  const double t0 = s.t0, ss = s.s, f0 = s.f0, cep = s.cep;
+  if (fabs(t - t0) > 10.0 * ss) return 0.0;
+  const double c = cos(2 * kPI * f0 * (t - t0) + cep + phase);
+  switch (s.signal_type) {
+    case MITHRA_SIGNAL_NEUMANN:
+      return -cos(2 * kPI * f0 * (t - t0) + cep + phase) * 2.7724 * (t - t0) / (ss * ss) *
+             exp(-1.3863 * (t - t0) * (t - t0) / (ss * ss));
+    case MITHRA_SIGNAL_GAUSSIAN:
+      return c * exp(-1.3863 * pow((t - t0) / ss, 2));
+    case MITHRA_SIGNAL_SECANT:
+      return c / cosh((t - t0) / ss);
+    case MITHRA_SIGNAL_FLATTOP:
+      if (t - t0 <= -ss / 2.0) return c * exp(-pow((t - t0 + ss / 2.0) * f0 / s.nR, 2));
+      if (t - t0 <= ss / 2.0) return c;
+      return c * exp(-pow((t - t0 - ss / 2.0) * f0 / s.nR, 2));
+    case MITHRA_SIGNAL_INVGAUSSIAN: {
+      double env = pow((1.0 + pow((t - t0) / s.sigma_invg[0], 2)) * (1.0 + pow((t - t0) / s.sigma_invg[1], 2)), 0.25);
+      if (t - t0 <= -ss / 2.0) return c * env * exp(-pow((t - t0 + ss / 2.0) * f0 / s.nR, 2));
+      if (t - t0 <= ss / 2.0) return c * env;
+      return c * env * exp(-pow((t - t0 - ss / 2.0) * f0 / s.nR, 2));
+    }
+  }
+  return 0.0;
+}
 
 struct PushConst {
   // mesh
@@ -190,6 +235,10 @@ struct Domain {
   double* history;          // device: [rows][N][Nl]
   int pblocks;
   int* plane_of;            // host: for each of N planes, its index among inside planes or -1
+  // laser beams of the push (mithra_dev_set_beams): n_opt optical-undulator beams, then n_ext
+  // external-field beams
+  BeamDev* beams;
+  int n_opt, n_ext;
   // staging of the diagnostics gathers (diagnostics.hip), grown on demand
   void* diag;
   size_t diag_bytes;

@@ -267,32 +267,6 @@ __global__ __launch_bounds__(kTileJ* TI) void k_interior_lds(
 
 // ---- analytic seed (Signal::self src/classes.cpp:534-575, Seed::fields :740-855) --------
 
-__device__ double signal(const mithra_seed& s, double t, double phase) {
-  const double t0 = s.t0, ss = s.s, f0 = s.f0, cep = s.cep;
-  if (fabs(t - t0) > 10.0 * ss) return 0.0;
-  const double c = cos(2 * kPI * f0 * (t - t0) + cep + phase);
-  switch (s.signal_type) {
-    case MITHRA_SIGNAL_NEUMANN:
-      return -cos(2 * kPI * f0 * (t - t0) + cep + phase) * 2.7724 * (t - t0) / (ss * ss) *
-             exp(-1.3863 * (t - t0) * (t - t0) / (ss * ss));
-    case MITHRA_SIGNAL_GAUSSIAN:
-      return c * exp(-1.3863 * pow((t - t0) / ss, 2));
-    case MITHRA_SIGNAL_SECANT:
-      return c / cosh((t - t0) / ss);
-    case MITHRA_SIGNAL_FLATTOP:
-      if (t - t0 <= -ss / 2.0) return c * exp(-pow((t - t0 + ss / 2.0) * f0 / s.nR, 2));
-      if (t - t0 <= ss / 2.0) return c;
-      return c * exp(-pow((t - t0 - ss / 2.0) * f0 / s.nR, 2));
-    case MITHRA_SIGNAL_INVGAUSSIAN: {
-      double env = pow((1.0 + pow((t - t0) / s.sigma_invg[0], 2)) * (1.0 + pow((t - t0) / s.sigma_invg[1], 2)), 0.25);
-      if (t - t0 <= -ss / 2.0) return c * env * exp(-pow((t - t0 + ss / 2.0) * f0 / s.nR, 2));
-      if (t - t0 <= ss / 2.0) return c * env;
-      return c * env * exp(-pow((t - t0 - ss / 2.0) * f0 / s.nR, 2));
-    }
-  }
-  return 0.0;
-}
-
 // Plane-wave, truncated plane-wave and Gaussian-beam seeds (the types whose output does not
 // depend on the previous call).  Writes the boosted-frame vector potential a[3].
 __device__ void seed_fields(const mithra_seed& s, double r0, double r1, double r2, double time, double a[3]) {

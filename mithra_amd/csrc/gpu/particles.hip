@@ -15,13 +15,176 @@ __device__ inline double pmod(double a, double b) {  // src/stdinclude.cpp:88-93
   return x;
 }
 
-// Solver::undulatorField for static undulators with Solver::staticUndulator
-// (src/solver.cpp:1798-1824, src/beam.cc:14-76): adds the boosted-frame fields at r.
-template <typename UT>
-__device__ void undulator_field(const PushConst& c, UT und, double x, double y, double z, double tb, double et[3],
-                                double bt[3]) {
+// ---- laser beams of optical undulators and external fields (src/beam.cc:80-496) ------------
+
+__device__ inline void set3(double v[3], double a, const double d[3]) {  // FieldVector::mv
+  v[0] = a * d[0];
+  v[1] = a * d[1];
+  v[2] = a * d[2];
+}
+__device__ inline void add3(double v[3], double a, const double d[3]) {  // FieldVector::pmv
+  v[0] += a * d[0];
+  v[1] += a * d[1];
+  v[2] += a * d[2];
+}
+
+// Lab-frame E (eT) and B (bT) of beam s at lab point rl and lab time t0: the propagation
+// distance z = (rl - position) . direction, the delayed times tl = t0 - z / c0 and, for the
+// standing waves, tlm = t0 + z / c0 (Solver::undulatorField / externalField,
+// src/solver.cpp:1829-1838, :1893-1904), then the type's fields in the reference's operation
+// order (Solver::planeWave ... standingSuperGaussianBeam).
+__device__ void beam_fields(const BeamDev& s, const double rl[3], double t0, double c0, double eT[3], double bT[3]) {
+  const double rv[3] = {rl[0] - s.position[0], rl[1] - s.position[1], rl[2] - s.position[2]};
+  const double z = rv[0] * s.direction[0] + rv[1] * s.direction[1] + rv[2] * s.direction[2];
+  const double tl = t0 - z / c0, tlm = t0 + z / c0;
+  const double p0 = 0.0;
+  for (int c = 0; c < 3; ++c) eT[c] = bT[c] = 0.0;
+  const int type = s.beam_type;
+  const bool standing = type >= MITHRA_SEED_STANDING_PLANEWAVE;
+  if (type == MITHRA_SEED_PLANEWAVE || type == MITHRA_SEED_PLANEWAVE_TRUNCATED ||
+      type == MITHRA_SEED_STANDING_PLANEWAVE || type == MITHRA_SEED_STANDING_PLANEWAVE_TRUNCATED) {
+    const bool truncated = type == MITHRA_SEED_PLANEWAVE_TRUNCATED || type == MITHRA_SEED_STANDING_PLANEWAVE_TRUNCATED;
+    double te, tb;  // signal of E and of B
+    if (standing) {
+      if (truncated) {  // the truncation test comes first for the standing wave (src/beam.cc:312-322)
+        const double x = rv[0] * s.polarization[0] + rv[1] * s.polarization[1] + rv[2] * s.polarization[2];
+        const double y = rv[0] * s.yv[0] + rv[1] * s.yv[1] + rv[2] * s.yv[2];
+        if (pow(x / s.radius[0], 2) + pow(y / s.radius[1], 2) > 1.0) return;
+      }
+      const double ts = signal(s, tl, p0), tsm = signal(s, tlm, p0);
+      te = ts - tsm;
+      tb = ts + tsm;
+      if (fabs(te) < 1.0e-6 && fabs(tb) < 1.0e-6) return;
+    } else {
+      const double ts = signal(s, tl, p0);
+      if (fabs(ts) < 1.0e-6) return;
+      if (truncated) {
+        const double x = rv[0] * s.polarization[0] + rv[1] * s.polarization[1] + rv[2] * s.polarization[2];
+        const double y = rv[0] * s.yv[0] + rv[1] * s.yv[1] + rv[2] * s.yv[2];
+        if (pow(x / s.radius[0], 2) + pow(y / s.radius[1], 2) > 1.0) return;
+      }
+      te = tb = ts;
+    }
+    set3(eT, s.amplitude * te, s.polarization);
+    set3(bT, s.amplitude * tb / c0, s.yv);
+    return;
+  }
+  // Gaussian and super-Gaussian beams
+  const double x = rv[0] * s.polarization[0] + rv[1] * s.polarization[1] + rv[2] * s.polarization[2];
+  const double y = rv[0] * s.yv[0] + rv[1] * s.yv[1] + rv[2] * s.yv[2];
+  const double wrp = sqrt(1.0 + z * z / (s.zR[0] * s.zR[0]));
+  const double wrs = sqrt(1.0 + z * z / (s.zR[1] * s.zR[1]));
+  const bool super = type == MITHRA_SEED_SUPERGAUSSIANBEAM || type == MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM;
+  if (super) {
+    if ((fabs(x) - s.order[0] * s.radius[0]) > 4.0 * s.radius[0] * wrp ||
+        (fabs(y) - s.order[1] * s.radius[1]) > 4.0 * s.radius[1] * wrs)
+      return;
+  } else if (fabs(x / wrp) > 4.0 * s.radius[0] || fabs(y / wrs) > 4.0 * s.radius[1]) {
+    return;
+  }
+  {
+    const double ts = signal(s, tl, p0);
+    if (standing) {
+      const double tsm = signal(s, tlm, p0);
+      if (fabs(ts - tsm) < 1.0e-6 && fabs(ts + tsm) < 1.0e-6) return;
+    } else if (fabs(ts) < 1.0e-6) {
+      return;
+    }
+  }
+  const double atanP = atan(z / s.zR[0]);
+  const double atanS = atan(z / s.zR[1]);
+  double ex[3] = {0.0, 0.0, 0.0}, by[3] = {0.0, 0.0, 0.0}, ez[3] = {0.0, 0.0, 0.0}, bz[3] = {0.0, 0.0, 0.0};
+  if (!super) {
+    const double x0 = x / wrp, y0 = y / wrs;
+    const double q0 = 0.5 * (atanP + atanS) - kPI * z / s.l * (pow(x0 / s.zR[0], 2) + pow(y0 / s.zR[1], 2));
+    double t = exp(-pow(x0 / s.radius[0], 2) - pow(y0 / s.radius[1], 2)) / sqrt(wrs * wrp);
+    t *= s.amplitude;
+    if (!standing) {  // Solver::gaussianBeam (src/beam.cc:142-197)
+      double ts = signal(s, tl, q0 - kPI / 2.0);
+      set3(ex, t * ts, s.polarization);
+      set3(by, t * ts / c0, s.yv);
+      ts = signal(s, tl, q0 + atanP);
+      set3(ez, t * (-x0 / s.zR[0]) * ts, s.direction);
+      ts = signal(s, tl, q0 + atanS);
+      set3(bz, t * (-y0 / s.zR[1]) / c0 * ts, s.direction);
+    } else {  // Solver::standingGaussianBeam (src/beam.cc:331-399)
+      double ts = signal(s, tl, q0 - kPI / 2.0), tsm = signal(s, tlm, q0 - kPI / 2.0);
+      set3(ex, t * (ts - tsm), s.polarization);
+      set3(by, t / c0 * (ts + tsm), s.yv);
+      double p1 = q0 + atanP;
+      ts = signal(s, tl, p1);
+      tsm = signal(s, tlm, -p1);
+      set3(ez, t * (-x0 / s.zR[0]) * (ts - tsm), s.direction);
+      p1 = q0 + atanS;
+      ts = signal(s, tl, p1);
+      tsm = signal(s, tlm, -p1);
+      set3(bz, t * (-y0 / s.zR[1]) / c0 * (ts - tsm), s.direction);
+    }
+  } else {  // Solver::superGaussianBeam / standingSuperGaussianBeam (src/beam.cc:201-275, :403-494)
+    const double af = s.amplitude / sqrt(wrs * wrp);
+    for (int i = -s.order[0]; i <= s.order[0]; i++)
+      for (int j = -s.order[1]; j <= s.order[1]; j++) {
+        const double x0 = (x - i * s.radius[0]) / wrp;
+        const double y0 = (y - j * s.radius[1]) / wrs;
+        if (fabs(x0) > 4.0 * s.radius[0] || fabs(y0) > 4.0 * s.radius[1]) continue;
+        const double q0 = 0.5 * (atanP + atanS) - kPI * z / s.l * (pow(x0 / s.zR[0], 2) + pow(y / s.zR[1], 2));
+        const double t = af * exp(-pow(x0 / s.radius[0], 2) - pow(y0 / s.radius[1], 2));
+        if (!standing) {
+          double ts = signal(s, tl, q0 - kPI / 2.0);
+          add3(ex, t * ts, s.polarization);
+          add3(by, t * ts / c0, s.yv);
+          ts = signal(s, tl, q0 + atanP);
+          add3(ez, t * (-x0 / s.zR[0]) * ts, s.direction);
+          ts = signal(s, tl, q0 + atanS);
+          add3(bz, t * (-y0 / s.zR[1]) * ts / c0, s.direction);
+        } else {
+          double ts = signal(s, tl, q0 - kPI / 2.0), tsm = signal(s, tlm, q0 - kPI / 2.0);
+          add3(ex, t * (ts - tsm), s.polarization);
+          add3(by, t * (ts + tsm) / c0, s.yv);
+          double p1 = q0 + atanP;
+          ts = signal(s, tl, p1);
+          tsm = signal(s, tlm, -p1);
+          add3(ez, t * (-x0 / s.zR[0]) * (ts - tsm), s.direction);
+          p1 = q0 + atanS;
+          ts = signal(s, tl, p1);
+          tsm = signal(s, tlm, -p1);
+          add3(bz, t * (-y0 / s.zR[1]) * (ts - tsm) / c0, s.direction);
+        }
+      }
+  }
+  for (int c = 0; c < 3; ++c) {
+    eT[c] = ex[c] + ez[c];
+    bT[c] = by[c] + bz[c];
+  }
+}
+
+// A beam's lab fields at the particle, transformed into the bunch frame and added
+// (src/solver.cpp:1868-1876, :1938-1946).
+__device__ inline void add_beam(const PushConst& c, const BeamDev& b, double x, double y, double z, double tb,
+                                double et[3], double bt[3]) {
+  const double rl[3] = {x, y, c.gamma * (z + c.beta * c.c0 * (tb + c.dt_shift))};
+  const double t0 = c.gamma * (tb + c.dt_shift + c.beta / c.c0 * z);
+  double eT[3], bT[3];
+  beam_fields(b, rl, t0, c.c0, eT, bT);
+  bt[0] += c.gamma * (bT[0] + c.beta / c.c0 * eT[1]);
+  bt[1] += c.gamma * (bT[1] - c.beta / c.c0 * eT[0]);
+  bt[2] += bT[2];
+  et[0] += c.gamma * (eT[0] - c.beta * c.c0 * bT[1]);
+  et[1] += c.gamma * (eT[1] + c.beta * c.c0 * bT[0]);
+  et[2] += eT[2];
+}
+
+// Solver::undulatorField (src/solver.cpp:1798-1877) with Solver::staticUndulator (src/beam.cc:14-76):
+// adds every undulator's boosted-frame fields at r in table order; BEAMS: some are optical.
+template <bool BEAMS, typename UT>
+__device__ void undulator_field(const PushConst& c, UT und, const BeamDev* beams, double x, double y, double z,
+                                double tb, double et[3], double bt[3]) {
   for (int u = 0; u < c.n_und; ++u) {
     const UndDev U = *(const UndDev*)(und + u);
+    if (BEAMS && U.type == MITHRA_UNDULATOR_OPTICAL) {
+      add_beam(c, beams[U.beam], x, y, z, tb, et, bt);
+      continue;
+    }
     const double b0 = U.b0, ku = U.ku, ct = U.ct, st = U.st, L = U.L;
     const double lz = c.gamma * (z + c.beta * c.c0 * (tb + c.dt_shift)) - U.rb;
     const double ly = x * ct + y * st;
@@ -62,9 +225,10 @@ __device__ void undulator_field(const PushConst& c, UT und, double x, double y, 
 // E/B grid does not change between sub-steps, so a particle is read from HBM once per field
 // step and advanced n_sub times in registers; the time of each sub-step is accumulated exactly
 // as Solver::solve() accumulates timeBunch_ (src/solver.cpp:1316-1321).
+template <bool BEAMS>
 __device__ inline void push_body(double* __restrict__ part, long cap, long n, const float* __restrict__ eb,
-                                 const UndDev* __restrict__ und, const PushConst& c,
-                                 unsigned long long* __restrict__ counters, long p) {
+                                 const UndDev* __restrict__ und, const BeamDev* __restrict__ beams, int n_opt,
+                                 int n_ext, const PushConst& c, unsigned long long* __restrict__ counters, long p) {
   if (p >= n) return;
   const gptr<double> X = G(part);
   const gptr<const f2> ebg = G(reinterpret_cast<const f2*>(eb));
@@ -87,7 +251,9 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
     const bool b1y = (y < c.ymax - c.dy && y > c.ymin + c.dy);
     const bool b1z = (z < c.zp1 && z >= c.zp0);
     double bt[3] = {0.0, 0.0, 0.0}, et[3] = {0.0, 0.0, 0.0};
-    undulator_field(c, undg, x, y, z, tb, et, bt);
+    undulator_field<BEAMS>(c, undg, beams, x, y, z, tb, et, bt);
+    if (BEAMS)
+      for (int q = 0; q < n_ext; ++q) add_beam(c, beams[n_opt + q], x, y, z, tb, et, bt);
     if (e == 1.0) {
       if (b1x && b1y && b1z) {
         double ip, jp, kp;
@@ -176,10 +342,13 @@ __device__ inline void push_body(double* __restrict__ part, long cap, long n, co
   if (ncross) atomicAdd(&counters[1], (unsigned long long)ncross);
 }
 
+template <bool BEAMS>
 __global__ __launch_bounds__(256) void k_push(double* __restrict__ part, long cap, long n,
                                               const float* __restrict__ eb, const UndDev* __restrict__ und,
-                                              PushConst c, unsigned long long* __restrict__ counters) {
-  push_body(part, cap, n, eb, und, c, counters, (long)blockIdx.x * blockDim.x + threadIdx.x);
+                                              const BeamDev* __restrict__ beams, int n_opt, int n_ext, PushConst c,
+                                              unsigned long long* __restrict__ counters) {
+  push_body<BEAMS>(part, cap, n, eb, und, beams, n_opt, n_ext, c, counters,
+                   (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // z extent of the bunch in cells (clamped to the mesh), for the lazy E/B evaluation: per
@@ -663,7 +832,7 @@ __global__ __launch_bounds__(256) void kb_push(BatchView* __restrict__ V, BatchT
     if (!v.active) continue;
     PushConst c = v.push;
     c.tb0 = v.time_bunch;
-    push_body(v.part, v.cap, v.np, v.eb, v.und, c, v.counters, (long)e.y * 256 + threadIdx.x);
+    push_body<false>(v.part, v.cap, v.np, v.eb, v.und, nullptr, 0, 0, c, v.counters, (long)e.y * 256 + threadIdx.x);
   }
 }
 
@@ -974,8 +1143,12 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const UndDev* un
   c.tb0 = time_bunch; c.dtb_time = bunch_time_step;
   c.n_sub = n_sub; c.n_und = n_und; c.save_rnm = save_rnm;
   c.dist0 = n_und > 0 ? d->und_dist0 : 0.0;  // kept at upload: `und` is device memory
-  hipLaunchKernelGGL(k_push, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap, d->np,
-                     d->eb, und, c, d->counters);
+  if (d->n_opt + d->n_ext > 0)
+    hipLaunchKernelGGL(k_push<true>, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap,
+                       d->np, d->eb, und, d->beams, d->n_opt, d->n_ext, c, d->counters);
+  else
+    hipLaunchKernelGGL(k_push<false>, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part,
+                       d->cap, d->np, d->eb, und, nullptr, 0, 0, c, d->counters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_zrange(d);

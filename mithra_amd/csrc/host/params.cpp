@@ -146,6 +146,24 @@ void normalise_beam(Vec3& direction, Vec3& polarization) {
   if (std::fabs(dot) > 1.0e-50) throw JobError("Polarization is not normal to the direction.");
 }
 
+// ExtField::initialize (src/classes.cpp:1188-1215) divides direction and polarization by the
+// square roots of their norms, not by the norms: a unit vector is kept as given, any other is
+// scaled to the square root of its length.
+void normalise_external(Vec3& direction, Vec3& polarization) {
+  if (direction.norm() == 0.0) throw JobError("The direction vector has length zero.");
+  Double vl = std::sqrt(direction.norm());
+  for (int c = 0; c < 3; ++c) direction[c] /= vl;
+  if (polarization.norm() == 0.0) throw JobError("The polarization vector has length zero.");
+  vl = std::sqrt(polarization.norm());
+  for (int c = 0; c < 3; ++c) polarization[c] /= vl;
+  Double dot = polarization[0] * direction[0] + polarization[1] * direction[1] + polarization[2] * direction[2];
+  if (std::fabs(dot) > 1.0e-50) throw JobError("Polarization is not normal to the direction.");
+}
+
+bool gaussian_type(SeedType t) {
+  return t == GAUSSIANBEAM || t == STANDINGGAUSSIANBEAM || t == SUPERGAUSSIANBEAM || t == STANDINGSUPERGAUSSIANBEAM;
+}
+
 void super_gaussian_setup(std::vector<int>& order, const std::vector<int>& given, std::vector<Double>& radius,
                           Double& a0) {
   order = given;
@@ -627,6 +645,8 @@ void ParseDarius::readUndulator(It& it) {
         else if (!k.take(name, *it, "beam-type")) throw JobError(name + " is not defined in the optical-undulator group.");
         next(it);
       } while (*it != "}");
+      // the signal, then Undulator::initialize (src/datainput.cpp:545-550, src/classes.cpp:1007-1102)
+      const Signal signal = k.signal();
       u.seedType_ = seedTypeOf(k.type);
       u.position_ = vec3(k.position);
       u.direction_ = vec3(k.direction);
@@ -635,7 +655,11 @@ void ParseDarius::readUndulator(It& it) {
       u.a0_ = k.a0;
       u.radius_ = k.radius;
       u.lu_ = k.wavelength;
-      u.signal_ = k.signal();
+      if (gaussian_type(u.seedType_) && u.radius_[0] * u.radius_[1] == 0.0)
+        throw JobError("One of the radii of the Gaussian beam is set to zero.");
+      u.signal_ = signal;
+      if (u.seedType_ == SUPERGAUSSIANBEAM || u.seedType_ == STANDINGSUPERGAUSSIANBEAM)
+        super_gaussian_setup(u.order_, k.order, u.radius_, u.a0_);
       undulator_.push_back(u);
     }
     next(it);
@@ -650,16 +674,28 @@ void ParseDarius::readExtField(It& it) {
       if (*next(it) != "{") throw JobError("The electromagnetic-field directory is empty");
       next(it);
       BeamKeys k;
+      k.wavelength = 1.0;  // 1 / Signal::f0_ of a default signal (src/datainput.cpp:584-588)
       do {
         const std::string name = parameterName(*it);
         if (!k.take(name, *it, "beam-type"))
           throw JobError(name + " is not defined in the electromagnetic external field group.");
         next(it);
       } while (*it != "}");
+      // the signal, then ExtField::initialize (src/datainput.cpp:615-620, src/classes.cpp:1162-1253)
+      const Signal signal = k.signal();
       ExtField e;
       e.seedType_ = seedTypeOf(k.type);
+      e.position_ = vec3(k.position);
+      e.direction_ = vec3(k.direction);
+      e.polarization_ = vec3(k.polarization);
+      normalise_external(e.direction_, e.polarization_);
       e.a0_ = k.a0;
-      e.signal_ = k.signal();
+      e.radius_ = k.radius;
+      if (gaussian_type(e.seedType_) && e.radius_[0] * e.radius_[1] == 0.0)
+        throw JobError("One of the radii of the Gaussian beam is set to zero.");
+      e.signal_ = signal;
+      if (e.seedType_ == SUPERGAUSSIANBEAM || e.seedType_ == STANDINGSUPERGAUSSIANBEAM)
+        super_gaussian_setup(e.order_, k.order, e.radius_, e.a0_);
       extField_.push_back(e);
     }
     next(it);

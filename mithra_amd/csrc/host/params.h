@@ -173,12 +173,13 @@ struct Undulator {
   std::vector<Double> zR_ = std::vector<Double>(2, 0.0);
 };
 
-struct ExtField {
+struct ExtField {  // src/classes.cpp:1145-1253
   SeedType seedType_ = PLANEWAVE;
   Double c0_ = 0.0, amplitude_ = 0.0, a0_ = 0.0, l_ = 0.0;
   Vec3 position_, direction_, polarization_;
   std::vector<Double> radius_ = std::vector<Double>(2, 0.0);
   Signal signal_;
+  std::vector<int> order_ = std::vector<int>(2, 0);
   std::vector<Double> zR_ = std::vector<Double>(2, 0.0);
 };
 

@@ -80,6 +80,9 @@ void Solver::setSimulationParameters() {
     e.signal_.f0_ *= c0_;
     e.signal_.s_ /= c0_;
     e.l_ = c0_ / e.signal_.f0_;
+    e.zR_.assign(2, 0.0);
+    e.zR_[0] = PI * e.radius_[0] * e.radius_[0] / e.l_;
+    e.zR_[1] = PI * e.radius_[1] * e.radius_[1] / e.l_;
   }
   seed_.amplitude_ = seed_.a0_ * EM * c0_ / EC;
   for (auto& u : undulator_) u.amplitude_ = u.a0_ * EM * c0_ * 2 * PI * u.signal_.f0_ / EC;
@@ -530,9 +533,6 @@ void Solver::shiftBackInTime() {
 
 void Solver::initialize() {
   if (initialized_) return;
-  for (auto& u : undulator_)
-    if (u.type_ == OPTICAL) throw JobError("Optical undulators are not supported by the GPU time march.");
-  if (!extField_.empty()) throw JobError("EXTERNAL-FIELD beams are not supported by the GPU time march.");
   setSimulationParameters();
   initializeBunch();
   timeBunch_ = time_;
@@ -634,6 +634,7 @@ void Solver::fill(mithra_dev* dev) {
     check(mithra_dev_set_potentials(dev, 1, anm1_.data(), mesh_.spaceCharge_ ? fnm1_.data() : nullptr),
           "A^n-1 upload");
   }
+  setBeams(dev);
   if (deviceBunch_) {
     check(mithra_dev_take_bunch(dev, dbunch_), "particle hand-over");
     return;
@@ -641,6 +642,50 @@ void Solver::fill(mithra_dev* dev) {
   std::vector<double> q(chargeVectorn_.size() * 11);
   particles(q.data(), (long)chargeVectorn_.size());
   check(mithra_dev_set_particles(dev, q.data(), (long)chargeVectorn_.size()), "particle upload");
+}
+
+namespace {
+
+template <typename B>
+mithra_beam beamOf(const B& b) {
+  mithra_beam m;
+  std::memset(&m, 0, sizeof m);
+  m.beam_type = (int)b.seedType_;
+  m.signal_type = (int)b.signal_.signalType_;
+  m.order[0] = b.order_[0];
+  m.order[1] = b.order_[1];
+  m.nR = b.signal_.nR_;
+  m.amplitude = b.amplitude_;
+  for (int c = 0; c < 3; ++c) {
+    m.position[c] = b.position_[c];
+    m.direction[c] = b.direction_[c];
+    m.polarization[c] = b.polarization_[c];
+  }
+  m.radius[0] = b.radius_[0];
+  m.radius[1] = b.radius_[1];
+  m.zR[0] = b.zR_[0];
+  m.zR[1] = b.zR_[1];
+  m.l = b.l_;
+  m.t0 = b.signal_.t0_;
+  m.s = b.signal_.s_;
+  m.f0 = b.signal_.f0_;
+  m.cep = b.signal_.cep_;
+  m.sigma_invg[0] = b.signal_.sigmaInvG_[0];
+  m.sigma_invg[1] = b.signal_.sigmaInvG_[1];
+  return m;
+}
+
+}  // namespace
+
+void Solver::setBeams(mithra_dev* dev) {
+  // the laser beams of optical undulators (table order) and of EXTERNAL-FIELD waves
+  std::vector<mithra_beam> optical, external;
+  for (auto& u : undulator_)
+    if (u.type_ == OPTICAL) optical.push_back(beamOf(u));
+  for (auto& e : extField_) external.push_back(beamOf(e));
+  if (optical.empty() && external.empty()) return;
+  check(mithra_dev_set_beams(dev, optical.data(), (int)optical.size(), external.data(), (int)external.size()),
+        "laser beams");
 }
 
 void Solver::upload(int precision) {

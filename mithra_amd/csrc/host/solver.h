@@ -108,6 +108,7 @@ class Solver {
   long particleCount() const;
   void fill(mithra_dev* dev);
   void particles(double* charge11, long n) const;
+  void setBeams(mithra_dev* dev);  // optical undulator and external-field beams of the push
   bool initialized_ = false;
   mithra_dev* dev_ = nullptr;
 

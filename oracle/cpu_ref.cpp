@@ -10,7 +10,8 @@
  *   Mur faces, edges, corners   src/database.cpp:137-288, loops src/fdtd.cpp:377-725
  *   E/B evaluation              src/fdtd.cpp:742-774, :818-845; src/fdtdSC.cpp:1110-1145
  *   bunch sub-step              src/solver.cpp:1424-1549, undulator src/solver.cpp:1798-1824,
- *                               src/beam.cc:14-76
+ *                               src/beam.cc:14-76; laser beams of optical undulators and
+ *                               external fields src/solver.cpp:1825-1947, src/beam.cc:80-496
  *   current deposition          src/fdtd.cpp:23-185, src/fdtdSC.cpp:23-210
  *   radiation power             src/radiation.cpp:18-232
  * Known divergence, shared with the HIP path: a particle whose periodic z coordinate leaves
@@ -73,6 +74,8 @@ struct oracle_dev {
   std::vector<std::vector<std::complex<double>>> ep, em;
   std::vector<double> fdt;    // [m][ni][4]
   std::vector<double> history;
+  // laser beams of optical undulators (table order) and external fields
+  std::vector<mithra_beam> optical, external;
 
   double* anp1() { return A[lvl_np1].data(); }
   double* an() { return A[lvl_n].data(); }
@@ -184,12 +187,300 @@ extern "C" void oracle_seed_fields(const mithra_seed* s, const double r[3], doub
   a[2] *= s->gamma;
 }
 
-/* Solver::undulatorField for static undulators + Solver::staticUndulator. */
+/* ---------------------------------------------------------------- laser beams -------- */
+
+namespace {
+
+// Signal::self of a beam's signal (the beam carries the Signal members).
+double beam_signal(const mithra_beam& b, double t, double phase) {
+  mithra_seed s;
+  std::memset(&s, 0, sizeof s);
+  s.signal_type = b.signal_type;
+  s.nR = b.nR;
+  s.t0 = b.t0;
+  s.s = b.s;
+  s.f0 = b.f0;
+  s.cep = b.cep;
+  s.sigma_invg[0] = b.sigma_invg[0];
+  s.sigma_invg[1] = b.sigma_invg[1];
+  return oracle_signal(&s, t, phase);
+}
+
+// UpdateBunchParallel's members that the beam routines read and write.
+struct Ubp {
+  double rv[3], z, tl, tlm, p0, p1, x, y, x0, y0, wrp, wrs, atanP, atanS, t, af, tsignal, tsignalm, tsignale,
+      tsignalb;
+  double yv[3], eT[3], bT[3], ex[3], ez[3], by[3], bz[3];
+};
+
+void zero(double v[3]) { v[0] = v[1] = v[2] = 0.0; }
+void mv(double v[3], double a, const double d[3]) { for (int c = 0; c < 3; ++c) v[c] = a * d[c]; }
+void pmv(double v[3], double a, const double d[3]) { for (int c = 0; c < 3; ++c) v[c] += a * d[c]; }
+double dot(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+void cross(double z[3], const double x[3], const double y[3]) {  // src/fieldvector.h:197-203
+  z[0] = x[1] * y[2] - x[2] * y[1];
+  z[1] = x[2] * y[0] - x[0] * y[2];
+  z[2] = x[0] * y[1] - x[1] * y[0];
+}
+void total(Ubp& u) {
+  for (int c = 0; c < 3; ++c) {
+    u.eT[c] = u.ex[c] + u.ez[c];
+    u.bT[c] = u.by[c] + u.bz[c];
+  }
+}
+
+// Solver::planeWave and planeWaveTruncated (src/beam.cc:80-130).
+void planeWave(Ubp& u, const mithra_beam& s, double c0, bool truncated) {
+  u.tsignal = beam_signal(s, u.tl, u.p0);
+  if (std::fabs(u.tsignal) < 1.0e-6) { zero(u.eT); zero(u.bT); return; }
+  if (truncated) {
+    u.x = dot(u.rv, s.polarization);
+    cross(u.yv, s.direction, s.polarization);
+    u.y = dot(u.rv, u.yv);
+    if (std::pow(u.x / s.radius[0], 2) + std::pow(u.y / s.radius[1], 2) > 1.0) { zero(u.eT); zero(u.bT); return; }
+  }
+  mv(u.eT, s.amplitude * u.tsignal, s.polarization);
+  cross(u.bT, s.direction, s.polarization);
+  const double b[3] = {u.bT[0], u.bT[1], u.bT[2]};
+  mv(u.bT, s.amplitude * u.tsignal / c0, b);
+}
+
+// Solver::gaussianBeam (src/beam.cc:132-198).
+void gaussianBeam(Ubp& u, const mithra_beam& s, double c0) {
+  u.x = dot(u.rv, s.polarization);
+  cross(u.yv, s.direction, s.polarization);
+  u.y = dot(u.rv, u.yv);
+  u.wrp = std::sqrt(1.0 + u.z * u.z / (s.zR[0] * s.zR[0]));
+  u.wrs = std::sqrt(1.0 + u.z * u.z / (s.zR[1] * s.zR[1]));
+  u.x0 = u.x / u.wrp;
+  u.y0 = u.y / u.wrs;
+  if (std::fabs(u.x0) > 4.0 * s.radius[0] || std::fabs(u.y0) > 4.0 * s.radius[1]) { zero(u.eT); zero(u.bT); return; }
+  u.tsignal = beam_signal(s, u.tl, u.p0);
+  if (std::fabs(u.tsignal) < 1.0e-6) { zero(u.eT); zero(u.bT); return; }
+  u.atanP = std::atan(u.z / s.zR[0]);
+  u.atanS = std::atan(u.z / s.zR[1]);
+  u.p0 = 0.5 * (u.atanP + u.atanS) - PI * u.z / s.l * (std::pow(u.x0 / s.zR[0], 2) + std::pow(u.y0 / s.zR[1], 2));
+  u.t = std::exp(-std::pow(u.x0 / s.radius[0], 2) - std::pow(u.y0 / s.radius[1], 2)) / std::sqrt(u.wrs * u.wrp);
+  u.t *= s.amplitude;
+  u.p1 = u.p0 - PI / 2.0;
+  u.tsignal = beam_signal(s, u.tl, u.p1);
+  mv(u.ex, u.t * u.tsignal, s.polarization);
+  mv(u.by, u.t * u.tsignal / c0, u.yv);
+  u.p1 = u.p0 + u.atanP;
+  u.tsignal = beam_signal(s, u.tl, u.p1);
+  mv(u.ez, u.t * (-u.x0 / s.zR[0]) * u.tsignal, s.direction);
+  u.p1 = u.p0 + u.atanS;
+  u.tsignal = beam_signal(s, u.tl, u.p1);
+  mv(u.bz, u.t * (-u.y0 / s.zR[1]) / c0 * u.tsignal, s.direction);
+  total(u);
+}
+
+// Solver::superGaussianBeam (src/beam.cc:200-276).
+void superGaussianBeam(Ubp& u, const mithra_beam& s, double c0) {
+  u.x = dot(u.rv, s.polarization);
+  cross(u.yv, s.direction, s.polarization);
+  u.y = dot(u.rv, u.yv);
+  u.wrp = std::sqrt(1.0 + u.z * u.z / (s.zR[0] * s.zR[0]));
+  u.wrs = std::sqrt(1.0 + u.z * u.z / (s.zR[1] * s.zR[1]));
+  if ((std::fabs(u.x) - s.order[0] * s.radius[0]) > 4.0 * s.radius[0] * u.wrp ||
+      (std::fabs(u.y) - s.order[1] * s.radius[1]) > 4.0 * s.radius[1] * u.wrs) {
+    zero(u.eT); zero(u.bT); return;
+  }
+  u.tsignal = beam_signal(s, u.tl, u.p0);
+  if (std::fabs(u.tsignal) < 1.0e-6) { zero(u.eT); zero(u.bT); return; }
+  u.atanP = std::atan(u.z / s.zR[0]);
+  u.atanS = std::atan(u.z / s.zR[1]);
+  u.af = s.amplitude / std::sqrt(u.wrs * u.wrp);
+  zero(u.ex); zero(u.ez); zero(u.bz); zero(u.by);
+  for (int i = -s.order[0]; i <= s.order[0]; i++)
+    for (int j = -s.order[1]; j <= s.order[1]; j++) {
+      u.x0 = (u.x - i * s.radius[0]) / u.wrp;
+      u.y0 = (u.y - j * s.radius[1]) / u.wrs;
+      if (std::fabs(u.x0) > 4.0 * s.radius[0] || std::fabs(u.y0) > 4.0 * s.radius[1]) continue;
+      // the parallel term uses x0, the perpendicular one y (not y0), src/beam.cc:247
+      u.p0 = 0.5 * (u.atanP + u.atanS) - PI * u.z / s.l * (std::pow(u.x0 / s.zR[0], 2) + std::pow(u.y / s.zR[1], 2));
+      u.t = u.af * std::exp(-std::pow(u.x0 / s.radius[0], 2) - std::pow(u.y0 / s.radius[1], 2));
+      u.p1 = u.p0 - PI / 2.0;
+      u.tsignal = beam_signal(s, u.tl, u.p1);
+      pmv(u.ex, u.t * u.tsignal, s.polarization);
+      pmv(u.by, u.t * u.tsignal / c0, u.yv);
+      u.p1 = u.p0 + u.atanP;
+      u.tsignal = beam_signal(s, u.tl, u.p1);
+      pmv(u.ez, u.t * (-u.x0 / s.zR[0]) * u.tsignal, s.direction);
+      u.p1 = u.p0 + u.atanS;
+      u.tsignal = beam_signal(s, u.tl, u.p1);
+      pmv(u.bz, u.t * (-u.y0 / s.zR[1]) * u.tsignal / c0, s.direction);
+    }
+  total(u);
+}
+
+// Solver::standingPlaneWave and standingPlaneWaveTruncated (src/beam.cc:278-327).
+void standingPlaneWave(Ubp& u, const mithra_beam& s, double c0, bool truncated) {
+  if (truncated) {
+    u.x = dot(u.rv, s.polarization);
+    cross(u.yv, s.direction, s.polarization);
+    u.y = dot(u.rv, u.yv);
+    if (std::pow(u.x / s.radius[0], 2) + std::pow(u.y / s.radius[1], 2) > 1.0) { zero(u.eT); zero(u.bT); return; }
+  }
+  u.tsignal = beam_signal(s, u.tl, u.p0);
+  u.tsignalm = beam_signal(s, u.tlm, u.p0);
+  u.tsignale = u.tsignal - u.tsignalm;
+  u.tsignalb = u.tsignal + u.tsignalm;
+  if (std::fabs(u.tsignale) < 1.0e-6 && std::fabs(u.tsignalb) < 1.0e-6) { zero(u.eT); zero(u.bT); return; }
+  mv(u.eT, s.amplitude * u.tsignale, s.polarization);
+  cross(u.bT, s.direction, s.polarization);
+  const double b[3] = {u.bT[0], u.bT[1], u.bT[2]};
+  mv(u.bT, s.amplitude * u.tsignalb / c0, b);
+}
+
+// Solver::standingGaussianBeam (src/beam.cc:329-400).
+void standingGaussianBeam(Ubp& u, const mithra_beam& s, double c0) {
+  u.x = dot(u.rv, s.polarization);
+  cross(u.yv, s.direction, s.polarization);
+  u.y = dot(u.rv, u.yv);
+  u.wrp = std::sqrt(1.0 + u.z * u.z / (s.zR[0] * s.zR[0]));
+  u.wrs = std::sqrt(1.0 + u.z * u.z / (s.zR[1] * s.zR[1]));
+  u.x0 = u.x / u.wrp;
+  u.y0 = u.y / u.wrs;
+  if (std::fabs(u.x0) > 4.0 * s.radius[0] || std::fabs(u.y0) > 4.0 * s.radius[1]) { zero(u.eT); zero(u.bT); return; }
+  u.tsignal = beam_signal(s, u.tl, u.p0);
+  u.tsignalm = beam_signal(s, u.tlm, u.p0);
+  u.tsignale = u.tsignal - u.tsignalm;
+  u.tsignalb = u.tsignal + u.tsignalm;
+  if (std::fabs(u.tsignale) < 1.0e-6 && std::fabs(u.tsignalb) < 1.0e-6) { zero(u.eT); zero(u.bT); return; }
+  u.atanP = std::atan(u.z / s.zR[0]);
+  u.atanS = std::atan(u.z / s.zR[1]);
+  u.p0 = 0.5 * (u.atanP + u.atanS) - PI * u.z / s.l * (std::pow(u.x0 / s.zR[0], 2) + std::pow(u.y0 / s.zR[1], 2));
+  u.t = std::exp(-std::pow(u.x0 / s.radius[0], 2) - std::pow(u.y0 / s.radius[1], 2)) / std::sqrt(u.wrs * u.wrp);
+  u.t *= s.amplitude;
+  u.p1 = u.p0 - PI / 2.0;
+  u.tsignal = beam_signal(s, u.tl, u.p1);
+  u.p1 = u.p0 - PI / 2.0;
+  u.tsignalm = beam_signal(s, u.tlm, u.p1);
+  mv(u.ex, u.t * (u.tsignal - u.tsignalm), s.polarization);
+  mv(u.by, u.t / c0 * (u.tsignal + u.tsignalm), u.yv);
+  u.p1 = u.p0 + u.atanP;
+  u.tsignal = beam_signal(s, u.tl, u.p1);
+  u.p1 = -u.p1;
+  u.tsignalm = beam_signal(s, u.tlm, u.p1);
+  mv(u.ez, u.t * (-u.x0 / s.zR[0]) * (u.tsignal - u.tsignalm), s.direction);
+  u.p1 = u.p0 + u.atanS;
+  u.tsignal = beam_signal(s, u.tl, u.p1);
+  u.p1 = -u.p1;
+  u.tsignalm = beam_signal(s, u.tlm, u.p1);
+  mv(u.bz, u.t * (-u.y0 / s.zR[1]) / c0 * (u.tsignal - u.tsignalm), s.direction);
+  total(u);
+}
+
+// Solver::standingSuperGaussianBeam (src/beam.cc:402-496), with the beam's wavelength in the
+// phase where the reference reads the never-set UpdateBunchParallel::l (src/beam.cc:466).
+void standingSuperGaussianBeam(Ubp& u, const mithra_beam& s, double c0) {
+  u.x = dot(u.rv, s.polarization);
+  cross(u.yv, s.direction, s.polarization);
+  u.y = dot(u.rv, u.yv);
+  u.wrp = std::sqrt(1.0 + u.z * u.z / (s.zR[0] * s.zR[0]));
+  u.wrs = std::sqrt(1.0 + u.z * u.z / (s.zR[1] * s.zR[1]));
+  if ((std::fabs(u.x) - s.order[0] * s.radius[0]) > 4.0 * s.radius[0] * u.wrp ||
+      (std::fabs(u.y) - s.order[1] * s.radius[1]) > 4.0 * s.radius[1] * u.wrs) {
+    zero(u.eT); zero(u.bT); return;
+  }
+  u.tsignal = beam_signal(s, u.tl, u.p0);
+  u.tsignalm = beam_signal(s, u.tlm, u.p0);
+  u.tsignale = u.tsignal - u.tsignalm;
+  u.tsignalb = u.tsignal + u.tsignalm;
+  if (std::fabs(u.tsignale) < 1.0e-6 && std::fabs(u.tsignalb) < 1.0e-6) { zero(u.eT); zero(u.bT); return; }
+  u.atanP = std::atan(u.z / s.zR[0]);
+  u.atanS = std::atan(u.z / s.zR[1]);
+  u.af = s.amplitude / std::sqrt(u.wrs * u.wrp);
+  zero(u.ex); zero(u.ez); zero(u.bz); zero(u.by);
+  for (int i = -s.order[0]; i <= s.order[0]; i++)
+    for (int j = -s.order[1]; j <= s.order[1]; j++) {
+      u.x0 = (u.x - i * s.radius[0]) / u.wrp;
+      u.y0 = (u.y - j * s.radius[1]) / u.wrs;
+      if (std::fabs(u.x0) > 4.0 * s.radius[0] || std::fabs(u.y0) > 4.0 * s.radius[1]) continue;
+      u.p0 = 0.5 * (u.atanP + u.atanS) - PI * u.z / s.l * (std::pow(u.x0 / s.zR[0], 2) + std::pow(u.y / s.zR[1], 2));
+      u.t = u.af * std::exp(-std::pow(u.x0 / s.radius[0], 2) - std::pow(u.y0 / s.radius[1], 2));
+      u.p1 = u.p0 - PI / 2.0;
+      u.tsignal = beam_signal(s, u.tl, u.p1);
+      u.p1 = u.p0 - PI / 2.0;
+      u.tsignalm = beam_signal(s, u.tlm, u.p1);
+      pmv(u.ex, u.t * (u.tsignal - u.tsignalm), s.polarization);
+      pmv(u.by, u.t * (u.tsignal + u.tsignalm) / c0, u.yv);
+      u.p1 = u.p0 + u.atanP;
+      u.tsignal = beam_signal(s, u.tl, u.p1);
+      u.p1 = -u.p1;
+      u.tsignalm = beam_signal(s, u.tlm, u.p1);
+      pmv(u.ez, u.t * (-u.x0 / s.zR[0]) * (u.tsignal - u.tsignalm), s.direction);
+      u.p1 = u.p0 + u.atanS;
+      u.tsignal = beam_signal(s, u.tl, u.p1);
+      u.p1 = -u.p1;
+      u.tsignalm = beam_signal(s, u.tlm, u.p1);
+      pmv(u.bz, u.t * (-u.y0 / s.zR[1]) * (u.tsignal - u.tsignalm) / c0, s.direction);
+    }
+  total(u);
+}
+
+// One beam's bunch-frame fields added to et, bt: the lab point and time, the beam's own
+// distance and delays, the type's fields, the boost (src/solver.cpp:1825-1877, :1886-1946).
+// The backward-wave delay tlm is the beam's own for optical undulators as for external
+// fields (the reference computes it only in Solver::externalField).
+void add_beam(const mithra_bunch_params* p, const mithra_beam& s, const double r[3], double tb, double et[3],
+              double bt[3]) {
+  const double gamma = p->gamma, beta = p->beta, c0 = p->c0;
+  double rl[3] = {r[0], r[1], gamma * (r[2] + beta * c0 * (tb + p->dt_shift))};
+  const double t0 = gamma * (tb + p->dt_shift + beta / c0 * r[2]);
+  Ubp u;
+  std::memset(&u, 0, sizeof u);
+  for (int c = 0; c < 3; ++c) u.rv[c] = rl[c] - s.position[c];
+  u.z = dot(u.rv, s.direction);
+  u.tl = t0 - u.z / c0;
+  u.tlm = t0 + u.z / c0;
+  u.p0 = 0.0;
+  switch (s.beam_type) {
+    case MITHRA_SEED_PLANEWAVE: planeWave(u, s, c0, false); break;
+    case MITHRA_SEED_PLANEWAVE_TRUNCATED: planeWave(u, s, c0, true); break;
+    case MITHRA_SEED_GAUSSIANBEAM: gaussianBeam(u, s, c0); break;
+    case MITHRA_SEED_SUPERGAUSSIANBEAM: superGaussianBeam(u, s, c0); break;
+    case MITHRA_SEED_STANDING_PLANEWAVE: standingPlaneWave(u, s, c0, false); break;
+    case MITHRA_SEED_STANDING_PLANEWAVE_TRUNCATED: standingPlaneWave(u, s, c0, true); break;
+    case MITHRA_SEED_STANDING_GAUSSIANBEAM: standingGaussianBeam(u, s, c0); break;
+    case MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM: standingSuperGaussianBeam(u, s, c0); break;
+  }
+  bt[0] += gamma * (u.bT[0] + beta / c0 * u.eT[1]);
+  bt[1] += gamma * (u.bT[1] - beta / c0 * u.eT[0]);
+  bt[2] += u.bT[2];
+  et[0] += gamma * (u.eT[0] - beta * c0 * u.bT[1]);
+  et[1] += gamma * (u.eT[1] + beta * c0 * u.bT[0]);
+  et[2] += u.eT[2];
+}
+
+}  // namespace
+
+extern "C" void oracle_beam_field(const mithra_bunch_params* p, const mithra_beam* beam, const double r[3],
+                                  double tb, double et[3], double bt[3]) {
+  add_beam(p, *beam, r, tb, et, bt);
+}
+
+/* Solver::undulatorField with Solver::staticUndulator (static undulators; an optical one is
+ * skipped here and added by the caller from its beam, in table order). */
+static void undulator_field(const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+                            const mithra_beam* optical, const double r[3], double tb, double et[3], double bt[3]);
+
 extern "C" void oracle_undulator_field(const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
                                        const double r[3], double tb, double et[3], double bt[3]) {
+  undulator_field(p, und, n_und, nullptr, r, tb, et, bt);
+}
+
+static void undulator_field(const mithra_bunch_params* p, const mithra_undulator* und, int n_und,
+                            const mithra_beam* optical, const double r[3], double tb, double et[3], double bt[3]) {
   const double gamma = p->gamma, beta = p->beta, c0 = p->c0;
+  int beam = 0;
   for (int u = 0; u < n_und; ++u) {
     const mithra_undulator& U = und[u];
+    if (U.type == MITHRA_UNDULATOR_OPTICAL) {
+      if (optical) add_beam(p, optical[beam++], r, tb, et, bt);
+      continue;
+    }
     double b0 = (U.lu != 0.0) ? EM * c0 * 2 * PI / U.lu * U.k / EC : 0.0;
     double ku = (U.lu != 0.0) ? 2 * PI / U.lu : 0.0;
     double ct = std::cos(U.theta), st = std::sin(U.theta);
@@ -620,6 +911,14 @@ extern "C" int oracle_field_update(oracle_dev* d, double time, const mithra_seed
 
 /* ---------------------------------------------------------------- particles ---------- */
 
+extern "C" int oracle_set_beams(oracle_dev* d, const mithra_beam* optical, int n_optical, const mithra_beam* external,
+                                int n_external) {
+  if (!d || n_optical < 0 || n_external < 0) return MITHRA_E_ARG;
+  d->optical.assign(optical, optical + n_optical);
+  d->external.assign(external, external + n_external);
+  return MITHRA_OK;
+}
+
 extern "C" int oracle_bunch_begin(oracle_dev* d) {
   const long n = (long)(d->q.size() / 11);
 #pragma omp parallel for
@@ -645,7 +944,8 @@ extern "C" int oracle_bunch_update(oracle_dev* d, const mithra_bunch_params* bp,
     bool b1y = (r[1] < g.ymax - g.dy && r[1] > g.ymin + g.dy);
     bool b1z = (r[2] < g.zp1 && r[2] >= g.zp0);
     double bt[3] = {0.0, 0.0, 0.0}, et[3] = {0.0, 0.0, 0.0};
-    oracle_undulator_field(bp, und, n_und, r, tb, et, bt);
+    undulator_field(bp, und, n_und, d->optical.data(), r, tb, et, bt);
+    for (const mithra_beam& b : d->external) add_beam(bp, b, r, tb, et, bt);
     if (Q[10] == 1.0) {
       if (b1x && b1y && b1z) {
         double d1, dxr, dyr, dzr;

@@ -36,6 +36,10 @@ int oracle_get_particles(oracle_dev* dev, double* charge11, long n);
 long oracle_particle_count(const oracle_dev* dev);
 int oracle_field_update(oracle_dev* dev, double time, const mithra_seed* seed);
 int oracle_field_evaluate(oracle_dev* dev);
+int oracle_set_beams(oracle_dev* dev, const mithra_beam* optical, int n_optical, const mithra_beam* external,
+                     int n_external);
+void oracle_beam_field(const mithra_bunch_params* p, const mithra_beam* beam, const double r[3], double time_bunch,
+                       double et[3], double bt[3]);
 int oracle_bunch_begin(oracle_dev* dev);
 int oracle_bunch_update(oracle_dev* dev, const mithra_bunch_params* p, const mithra_undulator* und,
                         int n_und, double time_bunch);

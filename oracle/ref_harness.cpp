@@ -192,6 +192,26 @@ class Harness : public Base {
       und.push_back(u.theta_);
     }
     o.dv("undulators", und.data(), und.size());
+    /* laser beams of the optical undulators (table order), then of the external fields, after
+       the scaling of setSimulationParameters: 27 values each */
+    std::vector<double> beams;
+    auto beam = [&](double kind, int seedType, const Signal& g, const std::vector<int>& order, double amp,
+                    const FieldVector<Double>& pos, const FieldVector<Double>& dir, const FieldVector<Double>& pol,
+                    const std::vector<Double>& radius, const std::vector<Double>& zR, double l) {
+      double v[27] = {kind, (double)seedType, (double)g.signalType_, (double)order[0], (double)order[1],
+                      (double)g.nR_, amp, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2], pol[0], pol[1], pol[2],
+                      radius[0], radius[1], zR[0], zR[1], l, g.t0_, g.s_, g.f0_, g.cep_, g.sigmaInvG_[0],
+                      g.sigmaInvG_[1]};
+      beams.insert(beams.end(), v, v + 27);
+    };
+    for (auto& u : s.undulator_)
+      if (u.type_ == OPTICAL)
+        beam(0.0, u.seedType_, u.signal_, u.order_, u.amplitude_, u.position_, u.direction_, u.polarization_,
+             u.radius_, u.zR_, u.l_);
+    for (auto& e : s.extField_)
+      beam(1.0, e.seedType_, e.signal_, e.order_, e.amplitude_, e.position_, e.direction_, e.polarization_,
+           e.radius_, e.zR_, e.l_);
+    if (!beams.empty()) o.dv("beams", beams.data(), beams.size());
     /* radiation power sampling parameters */
     std::vector<double> rp;
     for (size_t jf = 0; jf < s.rp_.size(); ++jf) {

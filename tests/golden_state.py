@@ -21,6 +21,9 @@ CASES = sorted(f[:-4] for f in os.listdir(GOLDEN)
 # and the oracle still run them; the GPU tests check the refusal.
 DEVICE_REFUSED = ("nsfd_o2_supergauss",)
 GPU_CASES = [c for c in CASES if c not in DEVICE_REFUSED]
+# Cases with optical undulators or external-field waves (laser beams in the push)
+BEAM_CASES = [c for c in CASES if "optical-undulator" in open(os.path.join(GOLDEN, "jobs", c + ".job")).read() or
+              "EXTERNAL-FIELD" in open(os.path.join(GOLDEN, "jobs", c + ".job")).read()]
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(REPO, "oracle", "_build", "libcpu_ref.so")
 
@@ -81,6 +84,18 @@ class Case:
                            for u in und]
         self.bunch = abi.BunchParams(s("gamma"), s("beta"), s("c0"), s("dt_shift"), s("ub_r1"),
                                      s("ub_r2"), s("ub_dtb"))
+        # laser beams of optical undulators and external fields (27 values each, ref_harness.cpp)
+        self.optical, self.external = [], []
+        for v in (d["beams"].reshape(-1, 27) if "beams" in d else []):
+            b = abi.Beam()
+            b.beam_type, b.signal_type = int(v[1]), int(v[2])
+            b.order[:] = [int(v[3]), int(v[4])]
+            b.nR, b.amplitude = int(v[5]), v[6]
+            b.position[:], b.direction[:], b.polarization[:] = list(v[7:10]), list(v[10:13]), list(v[13:16])
+            b.radius[:], b.zR[:], b.l = list(v[16:18]), list(v[18:20]), v[20]
+            b.t0, b.s, b.f0, b.cep = v[21], v[22], v[23], v[24]
+            b.sigma_invg[:] = list(v[25:27])
+            (self.optical if v[0] == 0 else self.external).append(b)
         pw = np.asarray(d["power"], dtype=np.float64)
         N, Nz, Nl, Nf = (int(v) for v in pw[:4])
         self.power_N, self.power_Nl, self.power_Nf = N, Nl, Nf
@@ -110,6 +125,8 @@ class Case:
         dom.set_potentials(2, self.arr("pre0_jn"), self.arr("pre0_rn") if sc else None)
         dom.set_particles(self.arr("pre0_particles"))
         dom.power_setup(self.power, max(self.n_steps, 1))
+        if self.optical or self.external:
+            dom.set_beams(self.optical, self.external)
         return dom
 
 
