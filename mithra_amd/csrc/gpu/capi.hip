@@ -148,9 +148,8 @@ int run_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_b
              unsigned n_time, int power_row, int more, float* ms) {
   if (!p || n_sub < 0) return MITHRA_E_ARG;
   const bool seeded = seed && std::fabs(seed->amplitude) > 1.0e-50;
-  if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
-      seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
-    return MITHRA_E_UNSUPPORTED;
+  if (seeded && (seed->seed_type < MITHRA_SEED_PLANEWAVE || seed->seed_type > MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM))
+    return MITHRA_E_ARG;
   if (power_row >= 0 && (!d->prows || (unsigned)power_row >= d->prows)) return MITHRA_E_STATE;
   int rc = upload_undulators(d, und, n_und, p->c0);
   if (rc) return rc;
@@ -289,6 +288,7 @@ int mithra_dev_destroy(mithra_dev* d) {
   hipFree(d->beams);
   free_power(d);
   hipFree(d->diag);
+  hipFree(d->sg_buf);
   hipFree(d->vring);
   hipFree(d->vdft);
   hipFree(d->vmap);
@@ -501,9 +501,8 @@ long mithra_dev_particle_count(const mithra_dev* d) { return d ? d->np : -1; }
 int mithra_dev_field_update(mithra_dev* d, double time, const mithra_seed* seed) {
   if (!d) return MITHRA_E_ARG;
   const bool seeded = seed && std::fabs(seed->amplitude) > 1.0e-50;
-  if (seeded && seed->seed_type != MITHRA_SEED_PLANEWAVE && seed->seed_type != MITHRA_SEED_PLANEWAVE_TRUNCATED &&
-      seed->seed_type != MITHRA_SEED_GAUSSIANBEAM)
-    return MITHRA_E_UNSUPPORTED;
+  if (seeded && (seed->seed_type < MITHRA_SEED_PLANEWAVE || seed->seed_type > MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM))
+    return MITHRA_E_ARG;
   if (int rc = settle(d)) return rc;
   const bool tfsf = seeded && d->g.N0 > 4 && d->g.N1 > 4 && d->g.N2 > 4;
   MITHRA_TRY(launch_interior(d, !tfsf));

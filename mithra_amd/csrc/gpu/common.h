@@ -239,6 +239,9 @@ struct Domain {
   // external-field beams
   BeamDev* beams;
   int n_opt, n_ext;
+  // super-gaussian TF/SF: per call its beam term, running vector and reset flag (field.hip)
+  void* sg_buf;
+  long sg_cap;
   // staging of the diagnostics gathers (diagnostics.hip), grown on demand
   void* diag;
   size_t diag_bytes;

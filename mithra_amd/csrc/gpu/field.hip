@@ -372,6 +372,147 @@ __global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid
   }
 }
 
+// ---- the super-gaussian seed's TF/SF injection ------------------------------------------------
+// Seed::fields adds the super-gaussian beam into its output vector instead of overwriting it
+// (src/classes.cpp:814-851), and FdTd::fieldUpdate passes the same vector, zeroed once per
+// step, to every call of the injection (src/fdtd.cpp:239, :307-373).  Call c therefore injects
+// the running result of calls 0..c in the reference's loop order: reset to zero by a call whose
+// signal is below the threshold, otherwise the beam's (2 o0 + 1)(2 o1 + 1) equal terms added one
+// by one, then the z component times gamma.  The calls are numbered in that loop order:
+//   x planes  (j outer, k inner, four planes each), y planes (i outer, k inner, four planes each),
+//   z = 1, 2 planes (i outer, j inner, two planes each), z = np-2, np-3 planes (likewise).
+struct SgCall {
+  long m;            // node that receives the injection
+  double r[3];       // point the seed is evaluated at (the neighbour across the TF/SF plane)
+  int block;         // 0: x planes, 1: y planes, 2: z = 1, 2, 3: z = np-2, np-3
+  bool minus;        // mmv (subtract) or pmv (add)
+};
+
+__device__ inline SgCall sg_call(const mithra_grid& g, long c) {
+  const int N0 = g.N0, N1 = g.N1, N2 = g.N2;
+  const long N1N0 = (long)N1 * N0, ni = N0 - 4, nj = N1 - 4, nk = N2 - 4;
+  SgCall q;
+  const long cx = 4 * nj * nk, cy = 4 * ni * nk, cz = 2 * ni * nj;
+  if (c < cx) {
+    const long t = c / 4, s = c % 4, j = 2 + t / nk, k = 2 + t % nk;
+    const long i = s == 0 ? 1 : s == 1 ? 2 : s == 2 ? N0 - 2 : N0 - 3;
+    const long ip = s == 0 ? 2 : s == 1 ? 1 : s == 2 ? N0 - 3 : N0 - 2;  // i of the evaluated node
+    q.m = N1N0 * k + N1 * i + j;
+    q.r[0] = g.xmin + ip * g.dx; q.r[1] = g.ymin + j * g.dy; q.r[2] = g.zmin + k * g.dz;
+    q.block = 0;
+    q.minus = (s % 2) == 0;
+  } else if (c < cx + cy) {
+    const long u = c - cx, t = u / 4, s = u % 4, i = 2 + t / nk, k = 2 + t % nk;
+    const long j = s == 0 ? 1 : s == 1 ? 2 : s == 2 ? N1 - 2 : N1 - 3;
+    const long jp = s == 0 ? 2 : s == 1 ? 1 : s == 2 ? N1 - 3 : N1 - 2;
+    q.m = N1N0 * k + N1 * i + j;
+    q.r[0] = g.xmin + i * g.dx; q.r[1] = g.ymin + jp * g.dy; q.r[2] = g.zmin + k * g.dz;
+    q.block = 1;
+    q.minus = (s % 2) == 0;
+  } else {
+    const long u = c - cx - cy, upper = u >= cz, v = upper ? u - cz : u, t = v / 2, s = v % 2;
+    const long i = 2 + t / nj, j = 2 + t % nj;
+    const long k = upper ? (s == 0 ? N2 - 2 : N2 - 3) : (s == 0 ? 1 : 2);
+    const long kp = upper ? (s == 0 ? N2 - 3 : N2 - 2) : (s == 0 ? 2 : 1);
+    q.m = N1N0 * k + N1 * i + j;
+    q.r[0] = g.xmin + i * g.dx; q.r[1] = g.ymin + j * g.dy; q.r[2] = g.zmin + kp * g.dz;
+    q.block = upper ? 3 : 2;
+    q.minus = s == 0;
+  }
+  return q;
+}
+
+// One call's beam term t * tsignal at boosted-frame point r (every one of its (2 o0 + 1)(2 o1 + 1)
+// terms is the same), or a reset (returns true) when the signal is below the threshold
+// (src/classes.cpp:742-755, :816-850).
+__device__ bool sg_term(const mithra_seed& s, double r0, double r1, double r2, double time, double* term) {
+  const double rl2 = s.gamma * (r2 + s.beta * s.c0 * (time + s.dt_shift));
+  double tl = s.gamma * (time + s.dt_shift + s.beta / s.c0 * r2);
+  const double rv[3] = {r0 - s.position[0], r1 - s.position[1], rl2 - s.position[2]};
+  const double z = rv[0] * s.direction[0] + rv[1] * s.direction[1] + rv[2] * s.direction[2];
+  tl -= z / s.c0;
+  double ts = signal(s, tl, 0.0);
+  if (fabs(ts) < 1.0e-6) {
+    *term = 0.0;
+    return true;
+  }
+  const double* pol = s.polarization;
+  const double* dir = s.direction;
+  const double x = rv[0] * pol[0] + rv[1] * pol[1] + rv[2] * pol[2];
+  const double yv0 = dir[1] * pol[2] - dir[2] * pol[1];
+  const double yv1 = dir[2] * pol[0] - dir[0] * pol[2];
+  const double yv2 = dir[0] * pol[1] - dir[1] * pol[0];
+  const double y = rv[0] * yv0 + rv[1] * yv1 + rv[2] * yv2;
+  const double l = s.c0 / s.f0;
+  const double zRp = kPI * s.radius[0] * s.radius[0] / l;
+  const double wrp = sqrt(1.0 + z * z / (zRp * zRp));
+  const double zRs = kPI * s.radius[1] * s.radius[1] / l;
+  const double wrs = sqrt(1.0 + z * z / (zRs * zRs));
+  const double p = 0.5 * (atan(z / zRp) + atan(z / zRs) - kPI) -
+                   kPI * z / l * (pow(x / (zRp * wrp), 2) + pow(y / (zRs * wrs), 2));
+  ts = signal(s, tl, p);
+  const double t = exp(-pow(x / (s.radius[0] * wrp), 2) - pow(y / (s.radius[1] * wrs), 2)) / sqrt(wrs * wrp) *
+                   s.amplitude;
+  *term = t * ts;
+  return false;
+}
+
+__global__ void k_sg_terms(mithra_grid g, mithra_seed s, double time, long n, double* __restrict__ term,
+                           int* __restrict__ reset) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const SgCall q = sg_call(g, c);
+  reset[c] = sg_term(s, q.r[0], q.r[1], q.r[2], time, &term[c]) ? 1 : 0;
+}
+
+// The same terms at listed points and times (the point evaluator's call order).
+__global__ void k_sg_point_terms(mithra_seed s, const double* __restrict__ r, const double* __restrict__ time,
+                                 long n, double* __restrict__ term, int* __restrict__ reset) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  reset[c] = sg_term(s, r[3 * c], r[3 * c + 1], r[3 * c + 2], time[c], &term[c]) ? 1 : 0;
+}
+
+// The running vector in call order: strictly sequential, one lane.
+__global__ void k_sg_running(mithra_seed s, long n, const double* __restrict__ term, const int* __restrict__ reset,
+                             double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int reps = (2 * s.order[0] + 1) * (2 * s.order[1] + 1);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (long c = 0; c < n; ++c) {
+    if (reset[c]) {
+      a0 = a1 = a2 = 0.0;
+    } else {
+      const double v = term[c];
+      for (int r = 0; r < reps; ++r) {
+        a0 += v * s.polarization[0];
+        a1 += v * s.polarization[1];
+        a2 += v * s.polarization[2];
+      }
+    }
+    a2 *= s.gamma;
+    out[3 * c] = a0;
+    out[3 * c + 1] = a1;
+    out[3 * c + 2] = a2;
+  }
+}
+
+// The injection of one block of calls (nodes are distinct within a block; the blocks run in
+// the reference's order, so a node two blocks touch gets its two updates in sequence).
+template <typename R>
+__global__ void k_sg_apply(FieldSet A, mithra_grid g, long first, long count, const double* __restrict__ run,
+                           double coef) {
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= count) return;
+  const long c = first + u;
+  const SgCall q = sg_call(g, c);
+  for (int k = 0; k < 3; ++k) {
+    R* p = (R*)A.c[k];
+    const double v = (double)p[q.m];
+    p[q.m] = (R)(q.minus ? v - coef * run[3 * c + k] : v + coef * run[3 * c + k]);
+  }
+}
+
 // The analytic seed on its own, at listed points: one thread per point.
 __global__ void k_signal_eval(mithra_seed s, const double* __restrict__ t, const double* __restrict__ phase, long n,
                               double* __restrict__ out) {
@@ -1203,8 +1344,50 @@ hipError_t interior_impl(Domain* d, bool fuse_yfaces) {
 }
 
 template <typename R>
+hipError_t tfsf_sg_impl(Domain* d, double time, const mithra_seed* seed) {
+  const mithra_grid& g = d->g;
+  const long ni = g.N0 - 4, nj = g.N1 - 4, nk = g.N2 - 4;
+  const long blocks[4] = {4 * nj * nk, 4 * ni * nk, 2 * ni * nj, 2 * ni * nj};
+  const long n = blocks[0] + blocks[1] + blocks[2] + blocks[3];
+  if (n <= 0) return hipSuccess;
+  if (d->sg_cap < n) {
+    hipError_t e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess) return e;
+    (void)hipFree(d->sg_buf);
+    d->sg_buf = nullptr;
+    d->sg_cap = 0;
+    if ((e = hipMalloc(&d->sg_buf, (size_t)n * (4 * sizeof(double) + sizeof(int)))) != hipSuccess) return e;
+    d->sg_cap = n;
+  }
+  double* term = (double*)d->sg_buf;
+  double* run = term + n;
+  int* reset = (int*)(run + 3 * n);
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_sg_terms, dim3(grid), dim3(256), 0, d->stream, g, *seed, time, n, term, reset);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sg_running, dim3(1), dim3(64), 0, d->stream, *seed, n, term, reset, run);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const double coef[4] = {d->co.a[1], d->co.a[2], d->co.a[3], d->co.a[3]};
+  long first = 0;
+  for (int b = 0; b < 4; ++b) {
+    if (blocks[b] > 0) {
+      hipLaunchKernelGGL((k_sg_apply<R>), dim3((unsigned)((blocks[b] + 255) / 256)), dim3(256), 0, d->stream,
+                         d->lvl[d->i_np1], g, first, blocks[b], run, coef[b]);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    first += blocks[b];
+  }
+  return hipSuccess;
+}
+
+template <typename R>
 hipError_t tfsf_impl(Domain* d, double time, const mithra_seed* seed) {
   const mithra_grid& g = d->g;
+  // the standing types are not handled by Seed::fields: its output stays the zero vector the
+  // update starts from (src/classes.cpp:758-851, src/fdtd.cpp:239), so nothing is injected
+  if (seed->seed_type >= MITHRA_SEED_STANDING_PLANEWAVE) return hipSuccess;
+  if (seed->seed_type == MITHRA_SEED_SUPERGAUSSIANBEAM) return tfsf_sg_impl<R>(d, time, seed);
   const long n[3] = {(long)(g.N1 - 4) * (g.N2 - 4), (long)(g.N0 - 4) * (g.N2 - 4), (long)(g.N0 - 4) * (g.N1 - 4)};
   const double coef[3] = {d->co.a[1], d->co.a[2], d->co.a[3]};
   for (int axis = 0; axis < 3; ++axis) {
@@ -1402,11 +1585,6 @@ int eval_points(long n, std::initializer_list<std::pair<const double*, int>> in,
   return e == hipSuccess ? MITHRA_OK : MITHRA_E_HIP - (int)e;
 }
 
-bool seed_supported(const mithra_seed* s) {
-  return s->seed_type == MITHRA_SEED_PLANEWAVE || s->seed_type == MITHRA_SEED_PLANEWAVE_TRUNCATED ||
-         s->seed_type == MITHRA_SEED_GAUSSIANBEAM;
-}
-
 bool signal_valid(const mithra_seed* s) { return s->signal_type >= MITHRA_SIGNAL_NEUMANN && s->signal_type <= MITHRA_SIGNAL_INVGAUSSIAN; }
 
 }  // namespace
@@ -1423,8 +1601,32 @@ int mithra_signal_evaluate(const mithra_seed* seed, const double* t, const doubl
 
 int mithra_seed_evaluate(const mithra_seed* seed, const double* r, const double* time, long n, double* a) {
   if (!seed || !signal_valid(seed)) return MITHRA_E_ARG;
-  if (!seed_supported(seed)) return MITHRA_E_UNSUPPORTED;
+  if (seed->seed_type < MITHRA_SEED_PLANEWAVE || seed->seed_type > MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM)
+    return MITHRA_E_ARG;
+  if (n < 0 || (n > 0 && (!r || !time || !a))) return MITHRA_E_ARG;
+  if (seed->seed_type >= MITHRA_SEED_STANDING_PLANEWAVE) {  // Seed::fields leaves the zero vector as it is
+    for (long q = 0; q < 3 * n; ++q) a[q] = 0.0;
+    return MITHRA_OK;
+  }
   const mithra_seed s = *seed;
+  if (s.seed_type == MITHRA_SEED_SUPERGAUSSIANBEAM) {
+    // one output vector, zeroed once and passed to every point in turn, as the injection loop
+    // passes it (src/fdtd.cpp:239, :307-373)
+    if (n == 0) return MITHRA_OK;
+    double* term = nullptr;
+    int* reset = nullptr;
+    if (hipMalloc(&term, sizeof(double) * n) != hipSuccess || hipMalloc(&reset, sizeof(int) * n) != hipSuccess) {
+      (void)hipFree(term);
+      return MITHRA_E_NOMEM;
+    }
+    const int rc = eval_points(n, {{r, 3}, {time, 1}}, 3, a, [&](double** in, double* o, unsigned blocks) {
+      hipLaunchKernelGGL(mithra::k_sg_point_terms, dim3(blocks), dim3(256), 0, 0, s, in[0], in[1], n, term, reset);
+      hipLaunchKernelGGL(mithra::k_sg_running, dim3(1), dim3(64), 0, 0, s, n, term, reset, o);
+    });
+    (void)hipFree(term);
+    (void)hipFree(reset);
+    return rc;
+  }
   return eval_points(n, {{r, 3}, {time, 1}}, 3, a, [&](double** in, double* o, unsigned blocks) {
     hipLaunchKernelGGL(mithra::k_seed_eval, dim3(blocks), dim3(256), 0, 0, s, in[0], in[1], n, o);
   });

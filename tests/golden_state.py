@@ -16,11 +16,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # the march fixtures: one per job file (other .npz files hold known-answer tables)
 CASES = sorted(f[:-4] for f in os.listdir(GOLDEN)
                if f.endswith(".npz") and os.path.exists(os.path.join(GOLDEN, "jobs", f[:-4] + ".job")))
-# Cases the device refuses: the super-gaussian seed, whose reference form adds every TF/SF call
-# into the previous call's output (src/classes.cpp:837-849, src/fdtd.cpp:239).  The host set-up
-# and the oracle still run them; the GPU tests check the refusal.
-DEVICE_REFUSED = ("nsfd_o2_supergauss",)
-GPU_CASES = [c for c in CASES if c not in DEVICE_REFUSED]
+GPU_CASES = list(CASES)
 # Cases with optical undulators or external-field waves (laser beams in the push)
 BEAM_CASES = [c for c in CASES if "optical-undulator" in open(os.path.join(GOLDEN, "jobs", c + ".job")).read() or
               "EXTERNAL-FIELD" in open(os.path.join(GOLDEN, "jobs", c + ".job")).read()]
@@ -124,7 +120,7 @@ class Case:
         dom.set_potentials(1, self.arr("pre0_anm1"), self.arr("pre0_fnm1") if sc else None)
         dom.set_potentials(2, self.arr("pre0_jn"), self.arr("pre0_rn") if sc else None)
         dom.set_particles(self.arr("pre0_particles"))
-        dom.power_setup(self.power, max(self.n_steps, 1))
+        dom.power_setup(self.power, max(self.n_steps, 64))  # room for the tests' own marches
         if self.optical or self.external:
             dom.set_beams(self.optical, self.external)
         return dom
