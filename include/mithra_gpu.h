@@ -411,11 +411,12 @@ int mithra_dev_take_bunch(mithra_dev* dev, const mithra_bunch* bunch);
  * it replays as one captured HIP graph.  Each instance is an ordinary mithra_dev, created and
  * filled through the calls above (potentials, particles, power set-up) and read back through
  * them afterwards (power fetch, particles, potentials); the batch does not own the handles,
- * which must outlive it and must not be used while it runs.  Supported: no space charge, no
- * TF/SF seed, static undulators, rows of the mesh that fit the row-tiled deposit.  Each
- * instance computes exactly its single-domain march (mithra_dev_step with a fused push and
- * power row = step): only the deposit's summation order differs.  No counterpart in the
- * reference (one job per process run). */
+ * which must outlive it and must not be used while it runs.  Supported: with or without space
+ * charge, TF/SF seeds of every type but the super-gaussian (mithra_batch_seed), static
+ * undulators, rows of the mesh that fit the row-tiled deposit.  Each instance computes exactly
+ * its single-domain march (mithra_dev_step with a fused push and power row = step): only the
+ * deposit's summation order differs.  No counterpart in the reference (one job per process
+ * run). */
 typedef struct mithra_batch mithra_batch;
 int mithra_batch_create(mithra_dev* const* devs, int n, mithra_batch** out);
 int mithra_batch_destroy(mithra_batch* batch);
@@ -426,6 +427,11 @@ int mithra_batch_destroy(mithra_batch* batch);
 int mithra_batch_instance(mithra_batch* batch, int i, double time, double time_bunch, double bunch_time_step,
                           int n_sub, long n_steps, const mithra_bunch_params* p, const mithra_undulator* und,
                           int n_und, double bunching_k, long bunching_every, int bunching_slots);
+/* Instance i's analytic seed, injected through the TF/SF planes every step at the instance's
+ * own time (FdTd::fieldUpdate, src/fdtd.cpp:307-373), as mithra_dev_field_update does; after
+ * mithra_batch_instance, before the first run.  The super-gaussian type is
+ * MITHRA_E_UNSUPPORTED here (its running vector is a single-domain path). */
+int mithra_batch_seed(mithra_batch* batch, int i, const mithra_seed* seed);
 /* Tile shape of the field kernels (a variant of mithra_tile_shape) and whether steps replay as
  * a captured graph (1) or are launched one by one (0); before the first run. */
 int mithra_batch_set_tiles(mithra_batch* batch, int variant, int use_graph);

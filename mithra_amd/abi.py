@@ -597,6 +597,7 @@ class Batch:
             ("mithra_batch_instance", C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                                 C.c_long, C.POINTER(BunchParams), C.POINTER(Undulator), C.c_int,
                                                 C.c_double, C.c_long, C.c_int]),
+            ("mithra_batch_seed", C.c_int, [vp, C.c_int, C.POINTER(Seed)]),
             ("mithra_batch_set_tiles", C.c_int, [vp, C.c_int, C.c_int]),
             ("mithra_batch_autotune", C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
             ("mithra_batch_run", C.c_int, [vp, C.c_long, C.POINTER(C.c_long)]),
@@ -622,6 +623,10 @@ class Batch:
                                                    C.byref(bunch_params), und, len(undulators), bunching_k,
                                                    bunching_every, bunching_slots))
         self.slots[i] = bunching_slots
+
+    def seed(self, i, seed):
+        """Instance i's TF/SF seed (after instance(), before the first run)."""
+        self._check(self.lib.mithra_batch_seed(self.h, i, C.byref(seed)))
 
     def set_tiles(self, variant, use_graph=True):
         self._check(self.lib.mithra_batch_set_tiles(self.h, variant, 1 if use_graph else 0))

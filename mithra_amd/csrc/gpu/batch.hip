@@ -95,6 +95,7 @@ struct mithra_batch {
   int2* tiles = nullptr;
   size_t tiles_cap = 0;
   BatchTiles t_int[2], t_eval, t_faces, t_edges, t_corners, t_part, t_rows, t_inst, t_dep, t_pnodes, t_pred;
+  BatchTiles t_tfsf[3];             // TF/SF lines of the seeded instances, per axis
   size_t dep_lds = 0;
   bool prepared = false;
   std::vector<long> step;           // host mirror of each instance's step
@@ -170,6 +171,13 @@ int build_tables(mithra_batch* b) {
   table(b->t_pred, [&](int i, const BatchView& v) {
     if (v.prows > 0) add(i, ((long)v.pN * v.pNl + 63) / 64);
   });
+  for (int axis = 0; axis < 3; ++axis)
+    table(b->t_tfsf[axis], [&](int i, const BatchView& v) {
+      if (!v.seeded) return;
+      const long lines = axis == 0 ? (long)(v.N1 - 4) * (v.N2 - 4)
+                                   : axis == 1 ? (long)(v.N0 - 4) * (v.N2 - 4) : (long)(v.N0 - 4) * (v.N1 - 4);
+      add(i, (lines + 255) / 256);
+    });
   if (all.size() > b->tiles_cap) {
     (void)hipFree(b->tiles);
     b->tiles = nullptr;
@@ -181,8 +189,9 @@ int build_tables(mithra_batch* b) {
     BATCH_TRY(hipMemcpyAsync(b->tiles, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice, b->stream));
     BATCH_TRY(hipStreamSynchronize(b->stream));
   }
-  BatchTiles* ts[] = {&b->t_int[0], &b->t_int[1], &b->t_eval, &b->t_faces, &b->t_edges, &b->t_corners,
-                      &b->t_part, &b->t_rows, &b->t_inst, &b->t_dep, &b->t_pnodes, &b->t_pred};
+  BatchTiles* ts[] = {&b->t_int[0], &b->t_int[1], &b->t_eval, &b->t_faces,    &b->t_edges,   &b->t_corners,
+                      &b->t_part,   &b->t_rows,   &b->t_inst, &b->t_dep,      &b->t_pnodes,  &b->t_pred,
+                      &b->t_tfsf[0], &b->t_tfsf[1], &b->t_tfsf[2]};
   for (BatchTiles* T : ts) T->t = b->tiles + (size_t)T->t;
   return MITHRA_OK;
 }
@@ -193,6 +202,7 @@ int enqueue_step(mithra_batch* b) {
   const int p = b->precision;
   BATCH_TRY(launch_batch_interior(V, b->t_int[1], b->variant, p, 1, s));
   BATCH_TRY(launch_batch_interior(V, b->t_int[0], b->variant, p, 0, s));
+  BATCH_TRY(launch_batch_tfsf(V, b->t_tfsf, p, s));
   BATCH_TRY(launch_batch_mur(V, b->t_faces, b->t_edges, b->t_corners, p, s));
   BATCH_TRY(launch_batch_eval(V, b->t_eval, b->variant, p, s));
   BATCH_TRY(launch_batch_push(V, b->t_part, s));
@@ -240,7 +250,8 @@ void set_tile_geometry(BatchView& v, int variant) {
   v.tilesJ = (v.N1 + 63) / 64;
   v.tilesI = (v.N0 - 2 + TI - 1) / TI;
   v.chunks = (v.N2 - 2 + K - 1) / K;
-  v.yfaces = yface_mask(variant, v.N1);  // no TF/SF in a batch: the interior kernels take the y faces
+  // without a TF/SF write between them, the interior kernels take the y faces
+  v.yfaces = v.seeded ? 0 : yface_mask(variant, v.N1);
 }
 
 int prepare(mithra_batch* b) {
@@ -297,7 +308,7 @@ int mithra_batch_create(mithra_dev* const* devs, int n, mithra_batch** out) {
     const mithra_dev* d = devs[i];
     if (!d) return MITHRA_E_ARG;
     if (d->precision != devs[0]->precision) return MITHRA_E_ARG;
-    if (d->g.space_charge || !deposit_tiled(d) || d->g.N0 < 5 || d->g.N1 < 5 || d->g.N2 < 5 || d->np >= (1L << 31))
+    if (!deposit_tiled(d) || d->g.N0 < 5 || d->g.N1 < 5 || d->g.N2 < 5 || d->np >= (1L << 31))
       return MITHRA_E_UNSUPPORTED;
   }
   mithra_batch* b = new (std::nothrow) mithra_batch();
@@ -316,6 +327,23 @@ int mithra_batch_create(mithra_dev* const* devs, int n, mithra_batch** out) {
     return rc;
   }
   *out = b;
+  return MITHRA_OK;
+}
+
+int mithra_batch_seed(mithra_batch* b, int i, const mithra_seed* seed) {
+  if (!b || i < 0 || i >= b->n || !seed) return MITHRA_E_ARG;
+  if (b->prepared || !b->ready[i]) return MITHRA_E_STATE;
+  if (seed->seed_type < MITHRA_SEED_PLANEWAVE || seed->seed_type > MITHRA_SEED_STANDING_SUPERGAUSSIANBEAM ||
+      seed->signal_type < MITHRA_SIGNAL_NEUMANN || seed->signal_type > MITHRA_SIGNAL_INVGAUSSIAN)
+    return MITHRA_E_ARG;
+  // the super-gaussian seed's running vector is a single-domain path
+  if (std::fabs(seed->amplitude) > 1.0e-50 && seed->seed_type == MITHRA_SEED_SUPERGAUSSIANBEAM)
+    return MITHRA_E_UNSUPPORTED;
+  BatchView& v = b->hv[i];
+  v.seed = *seed;
+  // as the single-domain field update decides (src/fdtd.cpp:307); the standing types inject nothing
+  v.seeded = std::fabs(seed->amplitude) > 1.0e-50 && v.N0 > 4 && v.N1 > 4 && v.N2 > 4 &&
+             seed->seed_type < MITHRA_SEED_STANDING_PLANEWAVE;
   return MITHRA_OK;
 }
 
@@ -361,6 +389,8 @@ int mithra_batch_instance(mithra_batch* b, int i, double time, double time_bunch
   v.N1 = g.N1;
   v.N2 = g.N2;
   v.ncomp = d->ncomp;
+  v.grid = g;
+  v.seeded = 0;  // mithra_batch_seed sets the seed
   v.nsfd = g.solver == MITHRA_SOLVER_NSFD;
   v.trunc2 = g.truncation_order == 2;
   for (int l = 0; l < 3; ++l) v.lvl[l] = d->lvl[l];

@@ -291,6 +291,9 @@ struct BatchView {
   double time, time_bunch;
   // mesh, potentials, E/B
   int N0, N1, N2, ncomp, nsfd, trunc2;
+  mithra_grid grid;         // the instance's mesh (TF/SF node positions)
+  mithra_seed seed;         // its analytic seed, injected when `seeded`
+  int seeded;
   FieldSet lvl[3];
   float* eb;
   int tilesJ, tilesI, chunks;  // interior / E-B tiles of the batch's tile shape
@@ -383,6 +386,7 @@ struct BatchTiles {
 hipError_t launch_batch_interior(BatchView* V, BatchTiles T, int variant, int precision, int nsfd, hipStream_t s);
 hipError_t launch_batch_mur(BatchView* V, BatchTiles faces, BatchTiles edges, BatchTiles corners, int precision,
                             hipStream_t s);
+hipError_t launch_batch_tfsf(BatchView* V, const BatchTiles* axes, int precision, hipStream_t s);  // axes[3]
 hipError_t launch_batch_eval(BatchView* V, BatchTiles T, int variant, int precision, hipStream_t s);
 hipError_t launch_batch_push(BatchView* V, BatchTiles T, hipStream_t s);
 hipError_t launch_batch_bunching(BatchView* V, BatchTiles particles, BatchTiles inst, hipStream_t s);

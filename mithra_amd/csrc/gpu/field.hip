@@ -335,10 +335,9 @@ __device__ inline void tfsf_apply(FieldSet A, long m, double px, double py, doub
 // four plane cases of its line in the reference's order, so a node touched twice (tiny meshes)
 // receives its contributions in the same sequence.
 template <typename R>
-__global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid g, mithra_seed s, double time,
-                       double coef) {
+__device__ inline void tfsf_body(FieldSet A, int axis, int N0, int N1, int N2, const mithra_grid& g,
+                                 const mithra_seed& s, double time, double coef, long t) {
   const long N1N0 = (long)N1 * N0;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (axis == 0) {
     const long nj = N1 - 4, nk = N2 - 4;
     if (t >= nj * nk) return;
@@ -370,6 +369,12 @@ __global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid
     tfsf_apply<R>(A, base + N1N0 * (N2 - 2), x, y, g.zmin + (N2 - 3) * g.dz, time, s, coef, true);
     tfsf_apply<R>(A, base + N1N0 * (N2 - 3), x, y, g.zmin + (N2 - 2) * g.dz, time, s, coef, false);
   }
+}
+
+template <typename R>
+__global__ void k_tfsf(FieldSet A, int axis, int N0, int N1, int N2, mithra_grid g, mithra_seed s, double time,
+                       double coef) {
+  tfsf_body<R>(A, axis, N0, N1, N2, g, s, time, coef, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- the super-gaussian seed's TF/SF injection ------------------------------------------------
@@ -1164,14 +1169,39 @@ __global__ __launch_bounds__(kTileJ* TI) void kb_eval(const BatchView* __restric
     const BatchView& v = V[e.x];
     if (v.active) {
       const EvalConst q = v.ev;  // a local copy: stores to eb cannot alias it
-      if constexpr (LDS)
-        eval_lds_body<R, false, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
-                                       v.N2 - 2, q, nullptr, 0, -1, e.y);
-      else
-        eval_body<R, false, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
-                                   v.N2 - 2, q, nullptr, 0, -1, e.y);
+      if constexpr (LDS) {
+        if (v.ncomp == 4)  // space charge: E includes -grad phi^n
+          eval_lds_body<R, true, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
+                                        v.N2 - 2, q, nullptr, 0, -1, e.y);
+        else
+          eval_lds_body<R, false, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI,
+                                         1, v.N2 - 2, q, nullptr, 0, -1, e.y);
+      } else {
+        if (v.ncomp == 4)
+          eval_body<R, true, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
+                                    v.N2 - 2, q, nullptr, 0, -1, e.y);
+        else
+          eval_body<R, false, TI, K>(v.lvl[v.i_n], v.lvl[v.i_np1], v.eb, v.N0, v.N1, v.N2, v.tilesJ, v.tilesI, 1,
+                                     v.N2 - 2, q, nullptr, 0, -1, e.y);
+      }
     }
     if constexpr (LDS) __syncthreads();
+  }
+}
+
+// TF/SF injection of the seeded instances, one axis per launch (the reference's order);
+// entry .y = block of 256 lines of the axis.
+template <typename R>
+__global__ __launch_bounds__(256) void kb_tfsf(const BatchView* __restrict__ V, BatchTiles T, int axis) {
+  for (int r = 0;; ++r) {
+    const int t = batch_entry(r);
+    if (t >= T.n) break;
+    const int2 e = T.t[t];
+    const BatchView& v = V[e.x];
+    if (!v.active || !v.seeded) continue;
+    const double coef = axis == 0 ? v.a[1] : axis == 1 ? v.a[2] : v.a[3];
+    tfsf_body<R>(v.lvl[v.i_np1], axis, v.N0, v.N1, v.N2, v.grid, v.seed, v.time, coef,
+                 (long)e.y * 256 + threadIdx.x);
   }
 }
 
@@ -1529,6 +1559,20 @@ hipError_t launch_batch_interior(BatchView* V, BatchTiles T, int variant, int pr
 hipError_t launch_batch_eval(BatchView* V, BatchTiles T, int variant, int precision, hipStream_t s) {
   return precision == MITHRA_PRECISION_FP64 ? batch_field_variant<double>(V, T, 1, variant, 0, s)
                                             : batch_field_variant<float>(V, T, 1, variant, 0, s);
+}
+
+hipError_t launch_batch_tfsf(BatchView* V, const BatchTiles* axes, int precision, hipStream_t s) {
+  for (int axis = 0; axis < 3; ++axis) {
+    const BatchTiles& T = axes[axis];
+    if (!T.n) continue;
+    if (precision == MITHRA_PRECISION_FP64)
+      hipLaunchKernelGGL((kb_tfsf<double>), dim3(batch_grid(T.n, 8)), dim3(256), 0, s, V, T, axis);
+    else
+      hipLaunchKernelGGL((kb_tfsf<float>), dim3(batch_grid(T.n, 8)), dim3(256), 0, s, V, T, axis);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <typename R>

@@ -1010,8 +1010,12 @@ __global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* _
       const CellOrder po = v.ord[v.parity];
       const CellOrder mo = v.ord[1 - v.parity];
       const DepositConst c = v.dep;  // a local copy: the LDS atomics cannot alias it
-      deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
-                                  v.dtilesI, kDepositChunk, 0, 0.0, 0, e.y);
+      if (v.ncomp == 4)  // space charge: rho rides as the fourth component
+        deposit_rows_body<R, true>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
+                                   v.dtilesI, kDepositChunk, 0, 0.0, 0, e.y);
+      else
+        deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
+                                    v.dtilesI, kDepositChunk, 0, 0.0, 0, e.y);
     }
     __syncthreads();  // the LDS tile is reused by the next entry
   }

@@ -41,7 +41,7 @@ JOB = """MESH
   mesh-center             = ( 0.0, 0.0, 0.0 )
   total-time              = 1.0e6
   mesh-truncation-order   = 2
-  space-charge            = false
+  space-charge            = {sc}
   solver                  = NSFD
 }}
 BUNCH
@@ -64,7 +64,7 @@ BUNCH
     bunching-factor-phase = {bfp:.10g}
   }}
 }}
-UNDULATOR
+{field}UNDULATOR
 {{
   static-undulator
   {{
@@ -88,6 +88,26 @@ FEL-OUTPUT
 }}
 """
 
+SEED = """FIELD
+{{
+  field-initialization
+  {{
+    type                  = gaussian-beam
+    position              = ( 0.0, 0.0, {zseed:.10g} )
+    direction             = ( 0.0, 0.0, 1.0 )
+    polarization          = ( 0.0, 1.0, 0.0 )
+    radius-parallel       = {w0:.10g}
+    radius-perpendicular  = {w0:.10g}
+    strength-parameter    = {a0:.10g}
+    signal-type           = gaussian
+    offset                = 0.0
+    pulse-length          = 1.0e12
+    wavelength            = {lam:.10g}
+    CEP                   = 0.0
+  }}
+}}
+"""
+
 
 @dataclasses.dataclass
 class Instance:
@@ -105,6 +125,9 @@ class Instance:
     sz_frac: float = 0.125    # sigma_z / longitudinal mesh length
     bunching: float = 0.0     # imposed initial bunching at lambda_r (amplitude, with `phase`); 0: shot noise
     phase: float = 0.0
+    space_charge: bool = False
+    seed_power: float = 0.0   # peak power (W) of a Gaussian-beam seed at lambda_r; 0: no seed
+    seed_w0: float = 10.0     # the seed's waist radius in units of sigma_r
 
     @property
     def lambda_r(self):
@@ -154,15 +177,31 @@ def bunch_sizes(inst):
     return sr, sz, sgb, electrons
 
 
+def seed_strength(power, w0, lam):
+    """Strength parameter a0 = e E0 lambda / (2 pi m c^2) of a Gaussian beam of peak power `power`
+    (W) and waist w0 (um) at wavelength lam (um): E0 = sqrt(4 Z0 P / (pi w0^2))."""
+    z0 = 376.730313668
+    e0 = math.sqrt(4.0 * z0 * power / (math.pi * (w0 * 1e-6) ** 2))
+    return 1.602e-19 * e0 * lam * 1e-6 / (2.0 * math.pi * 9.109e-31 * 2.99792458e8 ** 2)
+
+
 def job_text(inst, idx, workdir):
     """The instance as a job file: a Gaussian ellipsoid loaded by the reference's own quiet start
     (halton numbers, four beamlets lambda_r / 4 apart), with shot noise, or, for an instance with
-    an imposed bunching, with that bunching factor and phase instead."""
+    an imposed bunching, with that bunching factor and phase instead; space charge and a
+    Gaussian-beam seed at lambda_r (waist seed_w0 sigma_r, a quarter of the window ahead of the
+    mesh centre) when the instance asks for them."""
     # the reference makes (int)(L / d) + 2 nodes of a mesh length L (src/solver.cpp:610-612)
     lx, lz = (inst.n_perp - 1.5) * inst.dx, (inst.n_z - 1.5) * inst.dz
     sr, sz, sgb, electrons = bunch_sizes(inst)
     bunched = inst.bunching > 0.0
-    return JOB.format(lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=inst.n_particles // 4 * 4,
+    field = ""
+    if inst.seed_power > 0.0:
+        w0 = inst.seed_w0 * sr
+        field = SEED.format(zseed=lz / 4.0, w0=w0, a0=seed_strength(inst.seed_power, w0, inst.lambda_r),
+                            lam=inst.lambda_r)
+    return JOB.format(sc="true" if inst.space_charge else "false", field=field,
+                      lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=inst.n_particles // 4 * 4,
                       gamma=inst.gamma, sr=sr, sz=sz, sgb=sgb, sgbz=inst.spread * math.sqrt(inst.gamma ** 2 - 1.0),
                       tt=4.0 * sr, lt=3.5 * sz, shot="false" if bunched else "true",
                       bf=inst.bunching if bunched else 0.0, bfp=math.degrees(inst.phase) if bunched else 0.0,
@@ -282,6 +321,7 @@ def make_batch(prepared, variant=None, use_graph=True, autotune=True):
     for i, p in enumerate(prepared):
         b.instance(i, p.t["time"], p.t["time_bunch"], p.t["bunch_time_step"], p.t["n_sub"], p.inst.n_steps, p.bp,
                    p.und, p.k, p.inst.every, SAMPLES)
+        b.seed(i, p.seed)
     tuned = None
     if autotune:
         tuned = b.autotune()

@@ -44,6 +44,35 @@ def test_batch_equals_single_instance_runs(precision, graph, tol):
             assert rel_l2(qb, qs) <= tol
 
 
+@pytest.mark.parametrize("precision,tol", [(abi.PRECISION_FP64, 1e-12), (abi.PRECISION_FP32_BULK, 1e-5)],
+                         ids=["fp64", "fp32bulk"])
+def test_batch_with_space_charge_and_seed(precision, tol):
+    """Instances with space charge, with a Gaussian-beam TF/SF seed, with both and with neither in
+    one batch: each equals its single-domain run (mithra_dev_step with its seed)."""
+    insts = small_scan(4, seed=13)
+    for i, inst in enumerate(insts):
+        inst.space_charge = i % 2 == 0
+        inst.seed_power = 1.0e6 if i >= 2 else 0.0
+    print("scan", [(i.n_perp, i.n_z, i.n_particles, i.space_charge, i.seed_power) for i in insts], flush=True)
+    with tempfile.TemporaryDirectory() as w1, tempfile.TemporaryDirectory() as w2:
+        single = scan.prepare(insts, w1, precision)
+        batch = scan.prepare(insts, w2, precision)
+        assert [abs(p.seed.amplitude) > 0 for p in single] == [False, False, True, True]
+        res_s, _ = scan.run_single(single)
+        res_b, _, _ = scan.run_batch(batch)
+        for i, ((ps, bs), (pb, bb)) in enumerate(zip(res_s, res_b)):
+            print(f"instance {i}: |P| {np.max(np.abs(ps)):.3e} dP {rel_l2(pb, ps):.2e} "
+                  f"db {np.max(np.abs(bb - bs)):.2e}", flush=True)
+            assert np.max(np.abs(ps)) > 0.0
+            assert rel_l2(pb, ps) <= tol
+            assert np.max(np.abs(bb - bs)) <= tol * max(1e-3, np.max(np.abs(bs)))
+            a_s, f_s = single[i].dom.get_potentials(0)
+            a_b, f_b = batch[i].dom.get_potentials(0)
+            assert rel_l2(a_b, a_s) <= tol
+            if insts[i].space_charge:
+                assert rel_l2(f_b, f_s) <= tol
+
+
 def test_batch_continues_in_single_domain_calls():
     """After a batch run stops part-way, each domain continues through mithra_dev_step from
     where the batch left it, exactly like an uninterrupted single-domain march."""
