@@ -466,11 +466,22 @@ int mithra_mlp_gradients(mithra_mlp* mlp, int member, const float* x, const floa
                          float* loss);
 /* one Adam step (0.9, 0.999, 1e-8) of every member on one mini-batch; loss[members] before it */
 int mithra_mlp_train_step(mithra_mlp* mlp, const float* x, const float* t, long n, float lr, float* loss);
+/* the training set kept on the device: n rows of x (n x dims[0], stored as BF16) and targets t
+ * (n x dims[layers]); replaces any earlier set */
+int mithra_mlp_set_data(mithra_mlp* mlp, const float* x, const float* t, long n);
+/* `epochs` epochs of Adam on the resident set: each epoch a fresh permutation (from `seed`) cut
+ * into n / batch mini-batches, every member stepping on each; learning rate
+ * max(lr_max (1 + cos(pi epoch / epochs)) / 2, lr_min).  An epoch is one captured HIP graph
+ * replayed without the host.  loss[epochs] (optional): each epoch's mean loss over members and
+ * mini-batches.  MITHRA_E_STATE without a data set or with batch > n. */
+int mithra_mlp_train_epochs(mithra_mlp* mlp, int epochs, int batch, float lr_max, float lr_min,
+                            unsigned long long seed, float* loss);
 /* one product of the surrogate's GEMM on host data (BF16 as 16-bit patterns): C (M x N, FP32,
  * row-major) = op(A) op(B) with op(A)(m,k) = A[m*sam + k*sak], op(B)(k,n) = B[k*sbk + n*sbn];
- * reps timed launches, mean time in *ms */
+ * tile 0 (chosen by size), 1 (64 x 64 workgroups) or 2 (128 x 128); reps timed launches, mean
+ * time in *ms.  MITHRA_E_ARG when op(A) or op(B) reaches past a_len or b_len elements. */
 int mithra_mlp_gemm(const unsigned short* A, long sam, long sak, long a_len, const unsigned short* B, long sbk,
-                    long sbn, long b_len, float* C, int M, int N, int K, int reps, float* ms);
+                    long sbn, long b_len, float* C, int M, int N, int K, int tile, int reps, float* ms);
 
 /* Bunching factor b = |sum q exp(i k z)| / |sum q| over all particles at their current boosted-
  * frame z.  With k = 2 pi / lambda', lambda' = lambda_u / (gamma lambda_norm) the radiation

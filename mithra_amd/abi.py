@@ -691,6 +691,8 @@ class MLP:
             ("mithra_mlp_forward", C.c_int, [vp, fp, C.c_long, fp]),
             ("mithra_mlp_gradients", C.c_int, [vp, C.c_int, fp, fp, C.c_long, fp, fp]),
             ("mithra_mlp_train_step", C.c_int, [vp, fp, fp, C.c_long, C.c_float, fp]),
+            ("mithra_mlp_set_data", C.c_int, [vp, fp, fp, C.c_long]),
+            ("mithra_mlp_train_epochs", C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, fp]),
         ):
             f = getattr(self.lib, name)
             f.restype, f.argtypes = res, args
@@ -743,6 +745,19 @@ class MLP:
                                                    lr, loss.ctypes.data_as(fp)))
         return loss
 
+    def set_data(self, x, t):
+        """The training set, kept on the device for train_epochs."""
+        x, t = self._f(x), self._f(t)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.mithra_mlp_set_data(self.h, x.ctypes.data_as(fp), t.ctypes.data_as(fp), x.shape[0]))
+
+    def train_epochs(self, epochs, batch, lr_max, lr_min=1e-6, seed=0):
+        """Adam over the resident set, one captured graph per epoch; returns each epoch's mean loss."""
+        loss = np.empty(epochs, np.float32)
+        self._check(self.lib.mithra_mlp_train_epochs(self.h, epochs, batch, lr_max, lr_min, seed,
+                                                     loss.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss
+
     def close(self):
         if self.h:
             self.lib.mithra_mlp_destroy(self.h)
@@ -755,15 +770,15 @@ class MLP:
             pass
 
 
-def gemm_bf16(A, B, transpose_a=False, transpose_b=False, reps=1):
-    """C = op(A) op(B) through the surrogate's MFMA GEMM (inputs rounded to BF16); returns
-    (C, mean ms per launch)."""
+def gemm_bf16(A, B, transpose_a=False, transpose_b=False, reps=1, tile=0):
+    """C = op(A) op(B) through the surrogate's MFMA GEMM (inputs rounded to BF16; tile 0 by size,
+    1 for 64 x 64 workgroups, 2 for 128 x 128); returns (C, mean ms per launch)."""
     lib = load_gpu_library()
     f = lib.mithra_mlp_gemm
     u16p, fp = C.POINTER(C.c_ushort), C.POINTER(C.c_float)
     f.restype = C.c_int
     f.argtypes = [u16p, C.c_long, C.c_long, C.c_long, u16p, C.c_long, C.c_long, C.c_long, fp, C.c_int, C.c_int,
-                  C.c_int, C.c_int, fp]
+                  C.c_int, C.c_int, C.c_int, fp]
     A = np.ascontiguousarray(A, np.float32)
     B = np.ascontiguousarray(B, np.float32)
     M, K = (A.shape[1], A.shape[0]) if transpose_a else A.shape
@@ -775,7 +790,7 @@ def gemm_bf16(A, B, transpose_a=False, transpose_b=False, reps=1):
     out = np.empty((M, N), np.float32)
     ms = C.c_float()
     rc = f(a16.ctypes.data_as(u16p), sam, sak, a16.size, b16.ctypes.data_as(u16p), sbk, sbn, b16.size,
-           out.ctypes.data_as(fp), M, N, K, reps, C.byref(ms))
+           out.ctypes.data_as(fp), M, N, K, tile, reps, C.byref(ms))
     if rc != 0:
         raise MithraError(f"mithra_mlp_gemm: status {rc}")
     return out, ms.value

@@ -16,7 +16,7 @@ ordinary job file (so any instance can be run, or checked, on its own):
   n_particles macro-particles of a beam of the given peak current (1 kA by default), normalised
   transverse momentum spread emittance / sigma_r (2e-3 by default);
 * the march runs n_z field steps (with dt = dz/c in the frame, the light-crossing time of the
-  window), and P(z) and the bunching factor are sampled at SAMPLES evenly spaced steps, the
+  window) times `passes`, and P(z) and the bunching factor are sampled at SAMPLES evenly spaced steps, the
   power on one plane ahead of the bunch at the resonant frequency.
 """
 import dataclasses
@@ -128,6 +128,8 @@ class Instance:
     space_charge: bool = False
     seed_power: float = 0.0   # peak power (W) of a Gaussian-beam seed at lambda_r; 0: no seed
     seed_w0: float = 10.0     # the seed's waist radius in units of sigma_r
+    passes: int = 1           # field steps in units of n_z (window light-crossing times)
+    quiet: bool = False       # without an imposed bunching: no shot noise either (pure quiet start)
 
     @property
     def lambda_r(self):
@@ -143,7 +145,7 @@ class Instance:
 
     @property
     def n_steps(self):
-        return self.n_z
+        return self.n_z * self.passes
 
     @property
     def every(self):
@@ -203,7 +205,7 @@ def job_text(inst, idx, workdir):
     return JOB.format(sc="true" if inst.space_charge else "false", field=field,
                       lx=lx, lz=lz, dx=inst.dx, dz=inst.dz, charge=electrons, n=inst.n_particles // 4 * 4,
                       gamma=inst.gamma, sr=sr, sz=sz, sgb=sgb, sgbz=inst.spread * math.sqrt(inst.gamma ** 2 - 1.0),
-                      tt=4.0 * sr, lt=3.5 * sz, shot="false" if bunched else "true",
+                      tt=4.0 * sr, lt=3.5 * sz, shot="false" if bunched or inst.quiet else "true",
                       bf=inst.bunching if bunched else 0.0, bfp=math.degrees(inst.phase) if bunched else 0.0,
                       K=inst.K, lu=inst.lambda_u, dir=workdir, idx=idx, zp=0.375 * lz)
 
