@@ -295,7 +295,7 @@ class March:
         self.n += 1
 
 
-def run_gpu(job, steps, warmup, precision, reorder=16, pipelined=True):
+def run_gpu(job, steps, warmup, precision, reorder=16, pipelined=True, deterministic=False):
     """Warm-up, then `steps` timed steps, then a few steps with per-phase HIP events.  Pipelined:
     each group of steps is one complete pipelined run (mithra_dev_step_pipelined, more = 1 on all
     but its last step), so the timed region starts and ends in the reference's between-step state
@@ -303,6 +303,8 @@ def run_gpu(job, steps, warmup, precision, reorder=16, pipelined=True):
     dom = open_domain(job, precision)
     dom.set_lazy_fields(1)   # E/B where the bunch and the sampler read them (the reference's rule)
     dom.set_reorder_interval(reorder)
+    if deterministic:
+        dom.set_deterministic(True)
     (ci, ce), tune_ms = dom.autotune(reps=2)
     log("autotune (ms per launch, tile shapes rows x planes): " + "; ".join(
         f"{abi.tile_shape(v)[0]}x{abi.tile_shape(v)[1]} interior {tune_ms[0, v]:.3f} eval {tune_ms[1, v]:.3f}"
@@ -497,6 +499,8 @@ def main():
                          "Gaussian seed; the plain step, as the pipelined form covers runs without space charge)")
     ap.add_argument("--plain", action="store_true",
                     help="time the plain step (mithra_dev_step) instead of the pipelined one")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="fixed-order current deposition (bit-for-bit repeatable runs; plain steps)")
     ap.add_argument("--scan", type=int, default=64,
                     help="instances of the batched parameter scan measured for scan points/hour (0 skips it)")
     args = ap.parse_args()
@@ -516,13 +520,14 @@ def main():
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
         # the library pipelines FP64 without space charge only; FP32-bulk takes the plain step
-        pipelined = not args.plain and precision == abi.PRECISION_FP64 and args.workload == "production"
+        pipelined = (not args.plain and not args.deterministic and precision == abi.PRECISION_FP64
+                     and args.workload == "production")
         if args.workload == "production" and args.warmup + args.steps < 1500:
             log(f"note: steps {args.warmup}..{args.warmup + args.steps - 1} end before most particles emit "
                 "(about step 1500); the push then skips its grid gather and the step is cheaper than the "
                 "run's average")
         elapsed, phases, counters, P, tiles, emitting = run_gpu(job, args.steps, args.warmup, precision,
-                                                                args.reorder, pipelined)
+                                                                args.reorder, pipelined, args.deterministic)
         log(f"GPU: {1e3 * elapsed / args.steps:.3f} ms/step; phases {np.round(phases, 3).tolist()}")
         ms_step = 1e3 * elapsed / args.steps
         cell_rate = nodes * args.steps / elapsed
@@ -565,6 +570,7 @@ def main():
                    "nodes": nodes, "particles": n_part, "sub_steps": n_sub,
                    "precision": args.precision, "parallelism": "single GPU, whole domain resident",
                    "reorder_interval": args.reorder,
+                   "deposit": "fixed order" if args.deterministic else "row tiles, LDS adds",
                    "timed_steps": [args.warmup, args.warmup + args.steps - 1],
                    "emitting_fraction_after": emitting,
                    "step": "pipelined (E/B fused with the next update's stencil)" if pipelined else "plain"},

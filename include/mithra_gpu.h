@@ -328,6 +328,19 @@ int mithra_dev_set_lazy_fields(mithra_dev* dev, int lazy);
  * src/solver.h:271). */
 int mithra_dev_set_reorder_interval(mithra_dev* dev, int interval);
 
+/* Fixed-order current deposition (on = 1; default 0).  The default deposit sums each node's
+ * contributions in LDS with atomic adds, so FP64 runs agree to rounding but not bit for bit.
+ * In fixed order every node sums the weights of the halves in its eight cells in an order set
+ * by the particles alone (cell planes K-1, K, rows I-1, I, columns J-1, J; in each cell the
+ * rnp halves, then the rnm halves, in particle-array order), so a run repeated from the same
+ * state with the same reorder interval repeats every bit.  It costs a cell-keyed sort, the
+ * weights of every half stored (12-20 doubles) and a pass over every node; pipelined steps run
+ * as plain steps meanwhile.  MITHRA_E_STATE mid-way through a pipelined run, MITHRA_E_UNSUPPORTED
+ * for a mesh of 2^32 cells or more; a batch refuses such a domain as an instance.  Replaces the
+ * reference's particle-order sums (FdTd::currentUpdate, src/fdtd.cpp:38-185) by another fixed
+ * order; no counterpart switch in the reference, whose sums are always in list order. */
+int mithra_dev_set_deterministic(mithra_dev* dev, int on);
+
 /* Tile shapes of the two marched field kernels (interior update, E/B evaluation): variant v
  * has `rows` x-rows of 64 y-nodes per workgroup and marches `planes` z-planes; variants 7-9
  * stage the interior update's A^n planes in LDS.  When no TF/SF seed is injected, the interior

@@ -197,6 +197,7 @@ def _declare(lib, prefix):
     }
     if prefix == "mithra_dev_":
         sig["set_reorder_interval"] = (C.c_int, [vp, C.c_int])
+        sig["set_deterministic"] = (C.c_int, [vp, C.c_int])
         sig["set_tiles"] = (C.c_int, [vp, C.c_int, C.c_int])
         sig["set_lazy_fields"] = (C.c_int, [vp, C.c_int])
         sig["autotune"] = (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)])
@@ -440,6 +441,10 @@ class Domain:
     def set_reorder_interval(self, n):
         """Deposits between reorders of the particle arrays (0: never); device library only."""
         self._check(self.f["set_reorder_interval"](self.h, n))
+
+    def set_deterministic(self, on=True):
+        """Fixed-order current deposition (bit-for-bit repeatable FP64 runs); device library only."""
+        self._check(self.f["set_deterministic"](self.h, 1 if on else 0))
 
     def set_lazy_fields(self, lazy):
         """E/B only where the bunch and the power sampler read them (1), or everywhere (0)."""

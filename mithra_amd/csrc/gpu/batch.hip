@@ -365,8 +365,8 @@ int mithra_batch_instance(mithra_batch* b, int i, double time, double time_bunch
   // The batched sort and deposit walk every instance's particle orders, which exist only for
   // a domain holding particles; an instance without a bunch radiates nothing either.
   if (d->np <= 0) return MITHRA_E_UNSUPPORTED;
-  // the batched push adds static undulators only
-  if (d->n_opt + d->n_ext > 0) return MITHRA_E_UNSUPPORTED;
+  // the batched push adds static undulators only; the batched deposit is the row-tiled one
+  if (d->n_opt + d->n_ext > 0 || d->deterministic) return MITHRA_E_UNSUPPORTED;
   BATCH_TRY(hipStreamSynchronize(d->stream));
   if (int rc = settle(d)) return rc;
   if (int rc = upload_undulators(d, und, n_und, p->c0)) return rc;

@@ -158,7 +158,7 @@ int run_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_b
   // FP64 only: in FP32 the LDS-staged field kernels run at half the speed of the register
   // forms, and the fused kernel stages in LDS
   const bool pipe_out = more && d->precision == MITHRA_PRECISION_FP64 && d->ncomp == 3 && d->np > 0 &&
-                        deposit_tiled(d);
+                        deposit_tiled(d) && !d->deterministic;
   if (ms && !d->ev_ready) {
     for (auto& e : d->ev) MITHRA_TRY(hipEventCreate(&e));
     d->ev_ready = true;
@@ -286,6 +286,7 @@ int mithra_dev_destroy(mithra_dev* d) {
   free_sort(d);
   hipFree(d->und_dev);
   hipFree(d->beams);
+  hipFree(d->det_block);
   free_power(d);
   hipFree(d->diag);
   hipFree(d->sg_buf);
@@ -670,6 +671,25 @@ int mithra_dev_set_reorder_interval(mithra_dev* d, int interval) {
   if (!d || interval < 0) return MITHRA_E_ARG;
   d->sort_interval = interval;
   d->sort_count = 0;
+  return MITHRA_OK;
+}
+
+int mithra_dev_set_deterministic(mithra_dev* d, int on) {
+  if (!d || (on != 0 && on != 1)) return MITHRA_E_ARG;
+  if (d->pipe) return MITHRA_E_STATE;  // a pipelined run holds X + a4 J, not J
+  if (on && fixed_order_cells(d) == 0) return MITHRA_E_UNSUPPORTED;
+  if (on == d->deterministic) return MITHRA_OK;
+  MITHRA_TRY(hipStreamSynchronize(d->stream));
+  d->deterministic = on;
+  // the orders are keyed by cell in one mode and by row in the other: list them afresh
+  d->rm_order_valid = 0;
+  d->order1_is_rnp = 0;
+  if (!on) {
+    (void)hipFree(d->det_block);
+    d->det_block = nullptr;
+    d->det_bytes = 0;
+    d->ord[0].cell = d->ord[1].cell = nullptr;
+  }
   return MITHRA_OK;
 }
 

@@ -170,6 +170,8 @@ struct CellOrder {          // particle halves sorted by the (k, i) row of their
   unsigned* key;            // row k*N0 + i, or `rows` for a half that deposits nothing
   unsigned* idx;            // particle index
   unsigned* row;            // row r starts at row[r] in key/idx; rows + 1 entries
+  unsigned* cell;           // fixed-order deposit: keys are cells (k N0 + i)(N1 - 1) + j, and
+                            // cell c starts at cell[c]; cells + 1 entries (else unused)
 };
 
 struct Domain {
@@ -200,6 +202,10 @@ struct Domain {
   size_t sort_tmp_bytes;
   long sort_cap;            // capacity the order buffers were sized for
   unsigned sort_rows;
+  int deterministic;        // fixed-order deposit (mithra_dev_set_deterministic)
+  void* det_block;          // its cell starts (2 orders) and per-half weights (2 halves)
+  size_t det_bytes;
+  double* det_w[2];         // weights of the listed rnp / rnm halves, weight t of entry s at [t cap + s]
   int rm_order_valid;       // ord[1] lists the current rnm halves
   int order1_is_rnp;        // ord[1] lists the current rnp (true right after a deposit)
   double* part_alt;         // target of a reorder, swapped with part
@@ -368,13 +374,14 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const UndDev* un
 // interior node's X with X + a4 J (the pipelined step) and leaves the boundary nodes alone.
 constexpr int kDepositSource = 2;
 hipError_t launch_deposit(Domain* d, int accumulate);
+unsigned long long fixed_order_cells(const Domain* d);  // cells of the fixed-order deposit, 0: too many
 bool deposit_tiled(const Domain* d);
 hipError_t launch_bunching(Domain* d, double k, double* partial, int blocks);
 hipError_t launch_power(Domain* d, unsigned n_time, unsigned row);
 hipError_t launch_particle_ids(Domain* d);
 void free_sort(Domain* d);
 hipError_t ensure_orders(Domain* d, unsigned rows);
-hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows);
+hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows, unsigned span);
 hipError_t reorder_particles(Domain* d, CellOrder& o);
 
 // Batched forms of the kernels (persistent workgroups walking a table of (instance, local tile)

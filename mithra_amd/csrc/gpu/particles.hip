@@ -563,18 +563,20 @@ __global__ __launch_bounds__(256) void k_deposit(const double* __restrict__ part
 
 // Row key of one half of every particle: the (k, i) row k*N0 + i of the cell the half deposits
 // from (rnp or rnm, the cell test of k_deposit), or `rows` when that half deposits nothing.
+// With span = N1 - 1 the key is the cell, (k N0 + i) span + j, and the sentinel rows span.
 __global__ __launch_bounds__(256) void k_row_keys(const double* __restrict__ part, long cap, long n, DepositConst c,
-                                                  int use_rm, unsigned rows, unsigned* __restrict__ key,
+                                                  int use_rm, unsigned rows, unsigned span, unsigned* __restrict__ key,
                                                   unsigned* __restrict__ val) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int o = use_rm ? PXM : PX;
   const double x = part[(o + 0) * cap + p], y = part[(o + 1) * cap + p], z = part[(o + 2) * cap + p];
-  unsigned k = rows;
+  unsigned k = rows * span;
   if (inside(c, x, y, z)) {
     const int i = (int)floor(div_rn(x - c.xmin, c.dx, c.ix)), j = (int)floor(div_rn(y - c.ymin, c.dy, c.iy)),
               kk = (int)floor(div_rn(z - c.zmin, c.dz, c.iz));
-    if (cell_ok(c, i, j, kk)) k = (unsigned)kk * (unsigned)c.N0 + (unsigned)i;
+    if (cell_ok(c, i, j, kk))
+      k = ((unsigned)kk * (unsigned)c.N0 + (unsigned)i) * span + (span > 1 ? (unsigned)j : 0u);
   }
   key[p] = k;
   val[p] = (unsigned)p;
@@ -674,6 +676,107 @@ __device__ inline void deposit_item(const HalfItem& it, const DepositConst& c, d
     double* t = tile + (k & 1) * psz;
     if (rho) add_plane<SC>(t, TI, c.N1, up, i0, ti, ic, jc, v);
     else add_plane<false>(t, TI, c.N1, up, i0, ti, ic, jc, v);
+  }
+}
+
+// ---- fixed-order deposition ------------------------------------------------------------------
+//
+// The row-tiled deposit adds into LDS with atomics, so a node's sum is taken in whatever order
+// the wavefronts reach it: FP64 runs agree to rounding, not bit for bit.  The fixed-order form
+// sums every node in an order that depends on the particles alone.  Halves are listed by cell
+// (a stable sort, so the halves of one cell keep their particle-array order).  A first pass
+// computes each listed half's weights once (the arithmetic of deposit_item) into its list slot;
+// a second pass gives every node one thread, which adds the weights it receives from its eight
+// cells in a fixed order: cell planes k = K-1, K, rows i = I-1, I, columns j = J-1, J, in each
+// cell the rnp listings, then the rnm listings, in list order.
+
+// Weights of the halves listed in one order (half 0: rnp, with rho under space charge; half
+// 1: rnm): entry s of the list gets its 12 (20) weights at w[t cap + s].  Entries past the
+// listed cells (the sentinel key) are left alone.
+template <bool SC>
+__global__ __launch_bounds__(256) void k_half_weights(const double* __restrict__ part, long cap, long n,
+                                                      DepositConst c, const unsigned* __restrict__ key,
+                                                      const unsigned* __restrict__ idx, unsigned cells, int half,
+                                                      double* __restrict__ w) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || key[s] >= cells) return;
+  const long p = idx[s];
+  double rp[3], rm[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rp[a] = part[(PX + a) * cap + p];
+    rm[a] = part[(PXM + a) * cap + p];
+  }
+  const double q = part[PQ * cap + p];
+  const int ip = (int)floor(div_rn(rp[0] - c.xmin, c.dx, c.ix)), jp = (int)floor(div_rn(rp[1] - c.ymin, c.dy, c.iy)),
+            kp = (int)floor(div_rn(rp[2] - c.zmin, c.dz, c.iz));
+  const int im = (int)floor(div_rn(rm[0] - c.xmin, c.dx, c.ix)), jm = (int)floor(div_rn(rm[1] - c.ymin, c.dy, c.iy)),
+            km = (int)floor(div_rn(rm[2] - c.zmin, c.dz, c.iz));
+  double r[3];
+  r[0] = fmin(min(im, ip) * c.dx + c.dx + c.xmin, fmax(max(im, ip) * c.dx + c.xmin, 0.5 * (rm[0] + rp[0])));
+  r[1] = fmin(min(jm, jp) * c.dy + c.dy + c.ymin, fmax(max(jm, jp) * c.dy + c.ymin, 0.5 * (rm[1] + rp[1])));
+  r[2] = fmin(min(km, kp) * c.dz + c.dz + c.zmin, fmax(max(km, kp) * c.dz + c.zmin, 0.5 * (rm[2] + rp[2])));
+  double v[SC ? 20 : 12];
+  if (half == 0) {
+    const double jc3[3] = {rp[0] - r[0], rp[1] - r[1], rp[2] - r[2]};
+    half_weights(c, q, rp, r, jc3, v);
+    if (SC) rho_weights(c, q, rp, v + 12);
+  } else {
+    const double jc3[3] = {r[0] - rm[0], r[1] - rm[1], r[2] - rm[2]};
+    half_weights(c, q, rm, r, jc3, v);
+  }
+  const int nw = (SC && half == 0) ? 20 : 12;
+#pragma unroll
+  for (int t = 0; t < (SC ? 20 : 12); ++t)
+    if (t < nw) w[t * cap + s] = v[t];
+}
+
+// One thread per node (I, J, K): J (and rho) = the sum of the weights its eight cells' listed
+// halves give it, in the fixed order above; `accumulate` adds the sum to what the buffer holds.
+// Weight of a half of cell (i, j, k) at node offset (a, b, e) = (I - i, J - j, K - k), with the
+// value layout of half_weights / rho_weights (as add_plane places them): J_x takes
+// v[e ? 2 + b : b], J_y v[e ? 6 + a : 4 + a], J_z v[8 + a + 2 b], rho v[12 + 4 e + a + 2 b].
+template <typename R, bool SC>
+__global__ __launch_bounds__(256) void k_deposit_fixed(FieldSet J, DepositConst c, const unsigned* __restrict__ pcell,
+                                                       const unsigned* __restrict__ mcell,
+                                                       const double* __restrict__ wp, const double* __restrict__ wm,
+                                                       long cap, int accumulate) {
+  const long N1N0 = (long)c.N1 * c.N0;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N1N0 * c.N2) return;
+  const int K = (int)(t / N1N0);
+  const long rem = t - (long)K * N1N0;
+  const int I = (int)(rem / c.N1), Jn = (int)(rem - (long)I * c.N1);
+  const unsigned span = (unsigned)(c.N1 - 1);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = K - 1; k <= K; ++k) {
+    if (k < 0 || k > c.N2 - 2) continue;
+    const int e = K - k;
+    for (int i = I - 1; i <= I; ++i) {
+      if (i < 0 || i > c.N0 - 2) continue;
+      const int a = I - i;
+      for (int j = Jn - 1; j <= Jn; ++j) {
+        if (j < 0 || j > c.N1 - 2) continue;
+        const int b = Jn - j;
+        const unsigned cell = ((unsigned)k * (unsigned)c.N0 + (unsigned)i) * span + (unsigned)j;
+        for (int h = 0; h < 2; ++h) {
+          const unsigned* cs = h ? mcell : pcell;
+          const double* w = h ? wm : wp;
+          const unsigned s1 = cs[cell + 1];
+          for (unsigned s = cs[cell]; s < s1; ++s) {
+            acc[0] += w[(e ? 2 + b : b) * cap + s];
+            acc[1] += w[(e ? 6 + a : 4 + a) * cap + s];
+            acc[2] += w[(8 + a + 2 * b) * cap + s];
+            if (SC && h == 0) acc[3] += w[(12 + 4 * e + a + 2 * b) * cap + s];
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int comp = 0; comp < (SC ? 4 : 3); ++comp) {
+    R* out = (R*)component(J, comp);
+    out[t] = accumulate ? (R)((double)out[t] + acc[comp]) : (R)acc[comp];
   }
 }
 
@@ -1158,6 +1261,13 @@ hipError_t launch_push(Domain* d, const mithra_bunch_params* p, const UndDev* un
   return launch_zrange(d);
 }
 
+// Cells of the fixed-order deposit's keys: (N2 - 1) N0 rows of N1 - 1 cells (0 when they do
+// not fit a 32-bit key with its sentinel).
+unsigned long long fixed_order_cells(const Domain* d) {
+  const unsigned long long cells = (unsigned long long)(d->g.N2 - 1) * d->g.N0 * (d->g.N1 - 1);
+  return cells < 0xffffffffull ? cells : 0ull;
+}
+
 namespace {
 
 DepositConst deposit_const(const Domain* d) {
@@ -1172,11 +1282,74 @@ DepositConst deposit_const(const Domain* d) {
 
 hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows) {
   const DepositConst c = deposit_const(d);
+  const unsigned span = d->deterministic ? (unsigned)(c.N1 - 1) : 1u;  // cells, or rows
   hipLaunchKernelGGL(k_row_keys, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap,
-                     d->np, c, use_rm, rows, o.key, o.idx);
+                     d->np, c, use_rm, rows, span, o.key, o.idx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return sort_order(d, o, d->np, rows);
+  return sort_order(d, o, d->np, rows, span);
+}
+
+// The fixed-order deposit's buffers: both orders' cell starts and both halves' weights.
+hipError_t ensure_fixed_order(Domain* d) {
+  const unsigned long long cells = fixed_order_cells(d);
+  const size_t starts = (size_t)(cells + 1 + 63) / 64 * 64;
+  const size_t nw = d->ncomp == 4 ? 32 : 24;  // weights per particle: rnp (12 or 20) + rnm (12)
+  const size_t bytes = 2 * starts * sizeof(unsigned) + nw * (size_t)d->cap * sizeof(double);
+  if (d->det_block && d->det_bytes == bytes && d->ord[0].cell && d->ord[1].cell) return hipSuccess;
+  if (!d->det_block || d->det_bytes != bytes) {
+    (void)hipFree(d->det_block);
+    d->det_block = nullptr;
+    d->det_bytes = 0;
+    hipError_t e = hipMalloc(&d->det_block, bytes);
+    if (e != hipSuccess) return e;
+    d->det_bytes = bytes;
+  }
+  unsigned* b = (unsigned*)d->det_block;
+  d->ord[0].cell = b;
+  d->ord[1].cell = b + starts;
+  d->det_w[0] = (double*)(b + 2 * starts);
+  d->det_w[1] = d->det_w[0] + (d->ncomp == 4 ? 20 : 12) * (size_t)d->cap;
+  d->rm_order_valid = 0;  // the starts were (re)assigned: list both orders afresh
+  return hipSuccess;
+}
+
+// Fixed-order deposition of J (and rho): both orders listed by cell, the weights of every listed
+// half, then one thread per node.  The reorder and the order swap follow as in launch_deposit.
+hipError_t deposit_fixed_order(Domain* d, int accumulate, unsigned rows) {
+  const unsigned cells = (unsigned)fixed_order_cells(d);
+  hipError_t e;
+  if ((e = ensure_fixed_order(d)) != hipSuccess) return e;
+  if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows)) != hipSuccess) return e;
+  if ((e = build_order(d, d->ord[0], 0, rows)) != hipSuccess) return e;
+  const DepositConst c = deposit_const(d);
+  const dim3 gp((unsigned)((d->np + 255) / 256));
+  for (int h = 0; h < 2; ++h) {
+    if (c.sc)
+      hipLaunchKernelGGL((k_half_weights<true>), gp, dim3(256), 0, d->stream, d->part, d->cap, d->np, c,
+                         d->ord[h].key, d->ord[h].idx, cells, h, d->det_w[h]);
+    else
+      hipLaunchKernelGGL((k_half_weights<false>), gp, dim3(256), 0, d->stream, d->part, d->cap, d->np, c,
+                         d->ord[h].key, d->ord[h].idx, cells, h, d->det_w[h]);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  const FieldSet J = d->lvl[d->i_np1];
+  const dim3 gn((unsigned)((d->nodes + 255) / 256));
+  const unsigned *pc = d->ord[0].cell, *mc = d->ord[1].cell;
+  const bool f64 = d->precision == MITHRA_PRECISION_FP64;
+  if (f64 && c.sc)
+    hipLaunchKernelGGL((k_deposit_fixed<double, true>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+                       d->det_w[1], d->cap, accumulate);
+  else if (f64)
+    hipLaunchKernelGGL((k_deposit_fixed<double, false>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+                       d->det_w[1], d->cap, accumulate);
+  else if (c.sc)
+    hipLaunchKernelGGL((k_deposit_fixed<float, true>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+                       d->det_w[1], d->cap, accumulate);
+  else
+    hipLaunchKernelGGL((k_deposit_fixed<float, false>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+                       d->det_w[1], d->cap, accumulate);
+  return hipGetLastError();
 }
 
 // Rows per workgroup tile of the row-tiled deposit: its two-plane LDS ring
@@ -1223,10 +1396,20 @@ bool deposit_tiled(const Domain* d) {
   return tile_rows(d) > 0 && rows < 0xFFFFFFF0L;
 }
 
+hipError_t finish_deposit(Domain* d);
+
 hipError_t launch_deposit(Domain* d, int accumulate) {
   const mithra_grid& g = d->g;
-  if (accumulate == kDepositSource && (!deposit_tiled(d) || d->np == 0 || d->ncomp != 3)) return hipErrorInvalidValue;
+  if (accumulate == kDepositSource && (!deposit_tiled(d) || d->np == 0 || d->ncomp != 3 || d->deterministic))
+    return hipErrorInvalidValue;
   const double src_coef = d->co.a[4];
+  if (d->deterministic && d->np > 0) {
+    const unsigned rows = (unsigned)((g.N2 - 1) * (long)g.N0);
+    hipError_t e;
+    if ((e = ensure_orders(d, rows)) != hipSuccess) return e;
+    if ((e = deposit_fixed_order(d, accumulate, rows)) != hipSuccess) return e;
+    return finish_deposit(d);
+  }
   if (!deposit_tiled(d)) {
     if (!accumulate) {
       for (int c = 0; c < d->ncomp; ++c) {
@@ -1282,11 +1465,15 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
     hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
                        mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  // Periodic reorder of the particle arrays into the rnp row order.
+  return finish_deposit(d);
+}
+
+// After a sorted deposit: the periodic reorder of the particle arrays into the rnp order, and
+// the rnp order becomes the rnm order of the next deposit once rnm is set from rnp.
+hipError_t finish_deposit(Domain* d) {
   if (d->sort_interval > 0 && d->sort_count % d->sort_interval == 0)
-    if ((e = reorder_particles(d, d->ord[0])) != hipSuccess) return e;
+    if (hipError_t e = reorder_particles(d, d->ord[0]); e != hipSuccess) return e;
   ++d->sort_count;
-  // The rnp order becomes the rnm order of the next deposit once rnm is set from rnp.
   CellOrder t = d->ord[0];
   d->ord[0] = d->ord[1];
   d->ord[1] = t;

@@ -6,6 +6,8 @@
 // plus the start of each row in that list; a workgroup then reads two contiguous ranges per
 // half.  Sorting is rocPRIM's LSD radix sort of 32-bit (row, index) pairs over the key's
 // significant bits only; it is stable, so the halves of one row keep their particle-array order.
+// The fixed-order deposit keys by cell instead ((k N0 + i)(N1 - 1) + j) and lists the start of
+// every cell.
 //
 // The particle arrays themselves are permuted into row order now and then, so the sorted
 // indices walk memory almost sequentially; the original index travels with each particle in
@@ -34,7 +36,8 @@ __global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ id, long n)
   if (p < n) id[p] = (unsigned)p;
 }
 
-// row_start[r] = number of sorted keys below r, for r in [0, rows]
+// row_start[r] = number of sorted keys below r, for r in [0, rows] (rows: of the key's unit,
+// cell rows or cells)
 __global__ __launch_bounds__(256) void k_row_start(const unsigned* __restrict__ key, long n, unsigned rows,
                                                    unsigned* __restrict__ row_start) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,7 +71,7 @@ void free_sort(Domain* d) {
   d->sort_tmp_bytes = 0;
   d->sort_cap = 0;
   for (int s = 0; s < 2; ++s) {
-    d->ord[s].key = d->ord[s].idx = d->ord[s].row = nullptr;
+    d->ord[s].key = d->ord[s].idx = d->ord[s].row = d->ord[s].cell = nullptr;
   }
   d->key_alt = d->idx_alt = nullptr;
   d->rm_order_valid = 0;
@@ -95,9 +98,13 @@ hipError_t ensure_orders(Domain* d, unsigned rows) {
   return hipSuccess;
 }
 
-hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows) {
+// Sorts the keys of order o (key units: rows, or cells when span > 1 with `rows` * span cells)
+// and lists the start of each unit: rows into o.row, cells into o.cell.
+hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows, unsigned span) {
+  const unsigned long long units = (unsigned long long)rows * span;
+  if (units >= 0xffffffffull || (span > 1 && !o.cell)) return hipErrorInvalidValue;
   int bits = 1;
-  while (bits < 32 && (rows >> bits) != 0) ++bits;  // the sentinel `rows` must fit
+  while (bits < 32 && (units >> bits) != 0) ++bits;  // the sentinel `units` must fit
   rocprim::double_buffer<unsigned> k(o.key, d->key_alt), v(o.idx, d->idx_alt);
   size_t bytes = 0;
   hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, k, v, (unsigned)n, 0, bits, d->stream);
@@ -120,8 +127,9 @@ hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows) {
     d->idx_alt = o.idx;
     o.idx = v.current();
   }
-  hipLaunchKernelGGL(k_row_start, dim3((unsigned)(((long)rows + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, n,
-                     rows, o.row);
+  unsigned* starts = span > 1 ? o.cell : o.row;
+  hipLaunchKernelGGL(k_row_start, dim3((unsigned)((units + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, n,
+                     (unsigned)units, starts);
   return hipGetLastError();
 }
 
