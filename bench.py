@@ -27,7 +27,10 @@ roofline of the dominant kernel (the fused stencil + E/B kernel; the interior st
 oracle timed on a bounded sample of the same workload on this host.  scan_points_per_hour is
 the batched parameter scan's rate (64 seeded instances of uneven size, each a job taken to its
 P(z) and bunching at 128 positions, host set-up included), measured in the same run beside the
-same scan through the single-instance path.
+same scan through the single-instance path.  surrogate_points_per_hour is the surrogate with
+FDTD/PIC fallback on the space-charge, seeded generator's held-out points, counting only points
+whose returned P(z) is within 2 % of their FP64 solve (a reduced workflow: 1024 training points,
+600 epochs, 128 held-out points; the data generation and training are reported, not counted).
 """
 import argparse
 import json
@@ -352,6 +355,22 @@ def run_gpu(job, steps, warmup, precision, reorder=16, pipelined=True, determini
     return elapsed, phases, counters, P, tiles, emitting
 
 
+def run_surrogate(n_train, work):
+    """The surrogate workflow at a reduced size (see mithra_amd/surrogate.py:run_workflow): its
+    points/hour within 2 % of FP64 beside the single-instance and batched solvers'."""
+    from mithra_amd import surrogate
+    n_hold = max(32, n_train // 8)
+    log(f"surrogate: {n_train} training points, {n_hold} held out, 600 epochs")
+    out = surrogate.run_workflow(work, n_train=n_train, n_hold=n_hold, epochs=600, n_single=8, n_outside=0)
+    r = out["points_per_hour"]
+    log(f"surrogate: {r['surrogate_with_fallback_within_2pct']:.3g} points/h within 2 %, "
+        f"{r['gain_vs_single']:.0f}x the single-instance path, P(z) error median {out['surrogate_p_err']['median']:.4f}")
+    return {"n_train": n_train, "n_hold": n_hold, "epochs": 600, "train_s": out["train_s"],
+            "data_solve_s": out["solve_s"]["train_setup"] + out["solve_s"]["train_march"],
+            "p_err_median": out["surrogate_p_err"]["median"], "p_err_within_2pct": out["surrogate_p_err"]["within_2pct"],
+            "points_per_hour": r}
+
+
 def run_scan(n, work):
     """Batched parameter scan (seed 4: meshes 32-128 transverse x 128-1024 longitudinal cells,
     1e4-1e6 macro-particles, FP32-bulk fields, FP64 particles): the scan once as one batch and
@@ -499,6 +518,8 @@ def main():
                          "Gaussian seed; the plain step, as the pipelined form covers runs without space charge)")
     ap.add_argument("--plain", action="store_true",
                     help="time the plain step (mithra_dev_step) instead of the pipelined one")
+    ap.add_argument("--surrogate", type=int, default=1024,
+                    help="training points of the surrogate workflow measured for its points/hour (0 skips it)")
     ap.add_argument("--deterministic", action="store_true",
                     help="fixed-order current deposition (bit-for-bit repeatable runs; plain steps)")
     ap.add_argument("--scan", type=int, default=64,
@@ -544,6 +565,8 @@ def main():
         t_k = phases[abi.PHASES.index(phase)] * 1e-3
         achieved = bytes_per_node * interior / t_k / 1e9
         scan_out = run_scan(args.scan, work) if args.scan > 0 and args.workload == "production" else None
+        sur_out = (run_surrogate(args.surrogate, work) if args.surrogate > 0 and args.workload == "production"
+                   else None)
         copy_gbs = abi.copy_bandwidth()
         stream_gbs = abi.stream_bandwidth()
         cpu = None
@@ -577,6 +600,9 @@ def main():
         "particle_pushes_per_s": push_rate,
         "scan_points_per_hour": scan_out["points_per_hour_batched"] if scan_out else None,
         "scan": scan_out,
+        "surrogate_points_per_hour": (sur_out["points_per_hour"]["surrogate_with_fallback_within_2pct"]
+                                      if sur_out else None),
+        "surrogate": sur_out,
         "phase_ms": {k: round(float(v), 4) for k, v in zip(abi.PHASES, phases)},
         "roofline": {"bound": "hbm", "kernel": kernel_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
