@@ -13,7 +13,8 @@ fixed order every node sums its eight cells' listed halves in an order set by th
    record and P(z) sample; that march stays within the golden march's bars of the oracle and
    the reference's P(z) file;
 3. a pipelined request runs plain steps while the mode is on, the mode cannot change mid-way
-   through a pipelined run, and a batch refuses a fixed-order domain."""
+   through a pipelined run, and a batch refuses a fixed-order domain;
+4. at the production size two fixed-order marches from one set-up agree in every bit."""
 import numpy as np
 import pytest
 
@@ -127,3 +128,31 @@ def test_fixed_order_runs_plain_steps_and_guards_its_switch():
         b.instance(0, t, tb, case.dtb, case.n_sub, 4, case.bunch, case.undulators, 1.0, 1, 4)
     b.close()
     dom.close()
+
+
+def test_fixed_order_repeats_at_production_size():
+    """256 x 256 x 2048 nodes, 16,777,216 particles (the bench's set-up): two domains from one
+    set-up march 3 fixed-order steps each and agree in every bit of J, A^n and the particles."""
+    import os
+    import sys
+    import tempfile
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    with tempfile.TemporaryDirectory() as work:
+        job = bench.set_up(work)
+        out = []
+        for _ in range(2):
+            dom = bench.open_domain(job, abi.PRECISION_FP64, rows=3)
+            dom.set_deterministic(True)
+            m = bench.March(job)
+            for _ in range(3):
+                dom.step(m.time, m.seed, m.bp, m.und, m.tb, m.t["bunch_time_step"], m.t["n_sub"], m.n, m.n)
+                m.advance()
+            out.append((dom.get_potentials(2)[0], dom.get_potentials(0)[0], dom.get_particles()))
+            dom.close()
+        job.close()
+    (ja, aa, qa), (jb, ab, qb) = out
+    print(f"production size, fixed order: J non-zero {np.count_nonzero(ja)}, identical "
+          f"{np.array_equal(ja, jb) and np.array_equal(aa, ab) and np.array_equal(qa, qb)}", flush=True)
+    assert np.count_nonzero(ja) > 0
+    assert np.array_equal(ja, jb) and np.array_equal(aa, ab) and np.array_equal(qa, qb)
