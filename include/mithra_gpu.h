@@ -336,9 +336,10 @@ int mithra_dev_set_reorder_interval(mithra_dev* dev, int interval);
  * state with the same reorder interval repeats every bit.  It costs a cell-keyed sort, the
  * weights of every half stored (12-20 doubles) and a pass over every node; pipelined steps run
  * as plain steps meanwhile.  MITHRA_E_STATE mid-way through a pipelined run, MITHRA_E_UNSUPPORTED
- * for a mesh of 2^32 cells or more; a batch refuses such a domain as an instance.  Replaces the
- * reference's particle-order sums (FdTd::currentUpdate, src/fdtd.cpp:38-185) by another fixed
- * order; no counterpart switch in the reference, whose sums are always in list order. */
+ * for a mesh of 2^32 cells or more.  A batch refuses a
+ * fixed-order domain as an instance.  Replaces the reference's particle-order sums
+ * (FdTd::currentUpdate, src/fdtd.cpp:38-185) by another fixed order; no counterpart switch in
+ * the reference, whose sums are always in list order. */
 int mithra_dev_set_deterministic(mithra_dev* dev, int on);
 
 /* Tile shapes of the two marched field kernels (interior update, E/B evaluation): variant v

@@ -733,12 +733,16 @@ __global__ __launch_bounds__(256) void k_half_weights(const double* __restrict__
 
 // One thread per node (I, J, K): J (and rho) = the sum of the weights its eight cells' listed
 // halves give it, in the fixed order above; `accumulate` adds the sum to what the buffer holds.
-// Weight of a half of cell (i, j, k) at node offset (a, b, e) = (I - i, J - j, K - k), with the
-// value layout of half_weights / rho_weights (as add_plane places them): J_x takes
-// v[e ? 2 + b : b], J_y v[e ? 6 + a : 4 + a], J_z v[8 + a + 2 b], rho v[12 + 4 e + a + 2 b].
+// A cell row no half lists (in either order) adds nothing, which the row starts show at once:
+// most rows of the mesh.  Weight of a half of cell (i, j, k) at node offset (a, b, e) =
+// (I - i, J - j, K - k), with the value layout of half_weights / rho_weights (as add_plane places
+// them): J_x takes v[e ? 2 + b : b], J_y v[e ? 6 + a : 4 + a], J_z v[8 + a + 2 b],
+// rho v[12 + 4 e + a + 2 b].
 template <typename R, bool SC>
 __global__ __launch_bounds__(256) void k_deposit_fixed(FieldSet J, DepositConst c, const unsigned* __restrict__ pcell,
                                                        const unsigned* __restrict__ mcell,
+                                                       const unsigned* __restrict__ prow,
+                                                       const unsigned* __restrict__ mrow,
                                                        const double* __restrict__ wp, const double* __restrict__ wm,
                                                        long cap, int accumulate) {
   const long N1N0 = (long)c.N1 * c.N0;
@@ -755,6 +759,8 @@ __global__ __launch_bounds__(256) void k_deposit_fixed(FieldSet J, DepositConst 
     for (int i = I - 1; i <= I; ++i) {
       if (i < 0 || i > c.N0 - 2) continue;
       const int a = I - i;
+      const long row = (long)k * c.N0 + i;
+      if (prow[row] == prow[row + 1] && mrow[row] == mrow[row + 1]) continue;
       for (int j = Jn - 1; j <= Jn; ++j) {
         if (j < 0 || j > c.N1 - 2) continue;
         const int b = Jn - j;
@@ -1335,19 +1341,19 @@ hipError_t deposit_fixed_order(Domain* d, int accumulate, unsigned rows) {
   }
   const FieldSet J = d->lvl[d->i_np1];
   const dim3 gn((unsigned)((d->nodes + 255) / 256));
-  const unsigned *pc = d->ord[0].cell, *mc = d->ord[1].cell;
+  const unsigned *pc = d->ord[0].cell, *mc = d->ord[1].cell, *pr = d->ord[0].row, *mr = d->ord[1].row;
   const bool f64 = d->precision == MITHRA_PRECISION_FP64;
   if (f64 && c.sc)
-    hipLaunchKernelGGL((k_deposit_fixed<double, true>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+    hipLaunchKernelGGL((k_deposit_fixed<double, true>), gn, dim3(256), 0, d->stream, J, c, pc, mc, pr, mr, d->det_w[0],
                        d->det_w[1], d->cap, accumulate);
   else if (f64)
-    hipLaunchKernelGGL((k_deposit_fixed<double, false>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+    hipLaunchKernelGGL((k_deposit_fixed<double, false>), gn, dim3(256), 0, d->stream, J, c, pc, mc, pr, mr, d->det_w[0],
                        d->det_w[1], d->cap, accumulate);
   else if (c.sc)
-    hipLaunchKernelGGL((k_deposit_fixed<float, true>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+    hipLaunchKernelGGL((k_deposit_fixed<float, true>), gn, dim3(256), 0, d->stream, J, c, pc, mc, pr, mr, d->det_w[0],
                        d->det_w[1], d->cap, accumulate);
   else
-    hipLaunchKernelGGL((k_deposit_fixed<float, false>), gn, dim3(256), 0, d->stream, J, c, pc, mc, d->det_w[0],
+    hipLaunchKernelGGL((k_deposit_fixed<float, false>), gn, dim3(256), 0, d->stream, J, c, pc, mc, pr, mr, d->det_w[0],
                        d->det_w[1], d->cap, accumulate);
   return hipGetLastError();
 }

@@ -36,19 +36,41 @@ __global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ id, long n)
   if (p < n) id[p] = (unsigned)p;
 }
 
-// row_start[r] = number of sorted keys below r, for r in [0, rows] (rows: of the key's unit,
-// cell rows or cells)
+// row_start[r] = number of sorted keys below r span (the first key of row r), for r in [0, rows]
 __global__ __launch_bounds__(256) void k_row_start(const unsigned* __restrict__ key, long n, unsigned rows,
-                                                   unsigned* __restrict__ row_start) {
+                                                   unsigned span, unsigned* __restrict__ row_start) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r > (long)rows) return;
+  const unsigned long long first = (unsigned long long)r * span;
   long lo = 0, hi = n;
   while (lo < hi) {
     const long mid = (lo + hi) >> 1;
-    if (key[mid] < (unsigned)r) lo = mid + 1;
+    if ((unsigned long long)key[mid] < first) lo = mid + 1;
     else hi = mid;
   }
   row_start[r] = (unsigned)lo;
+}
+
+// cell_start[c] = number of sorted cell keys below c, for c in [0, rows span]: a search within
+// the keys of c's row only (row_start from k_row_start with the same span).
+__global__ __launch_bounds__(256) void k_cell_start(const unsigned* __restrict__ key,
+                                                    const unsigned* __restrict__ row_start, unsigned rows,
+                                                    unsigned span, unsigned* __restrict__ cell_start) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long cells = (long)rows * span;
+  if (c > cells) return;
+  if (c == cells) {
+    cell_start[c] = row_start[rows];
+    return;
+  }
+  const long r = c / span;
+  long lo = row_start[r], hi = row_start[r + 1];
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if ((long)key[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  cell_start[c] = (unsigned)lo;
 }
 
 }  // namespace
@@ -99,7 +121,7 @@ hipError_t ensure_orders(Domain* d, unsigned rows) {
 }
 
 // Sorts the keys of order o (key units: rows, or cells when span > 1 with `rows` * span cells)
-// and lists the start of each unit: rows into o.row, cells into o.cell.
+// and lists the start of each row into o.row and, keyed by cell, of each cell into o.cell.
 hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows, unsigned span) {
   const unsigned long long units = (unsigned long long)rows * span;
   if (units >= 0xffffffffull || (span > 1 && !o.cell)) return hipErrorInvalidValue;
@@ -127,9 +149,11 @@ hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows, unsigned s
     d->idx_alt = o.idx;
     o.idx = v.current();
   }
-  unsigned* starts = span > 1 ? o.cell : o.row;
-  hipLaunchKernelGGL(k_row_start, dim3((unsigned)((units + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, n,
-                     (unsigned)units, starts);
+  hipLaunchKernelGGL(k_row_start, dim3((unsigned)(((long)rows + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, n,
+                     rows, span, o.row);
+  if (span > 1)
+    hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((units + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, o.row,
+                       rows, span, o.cell);
   return hipGetLastError();
 }
 
