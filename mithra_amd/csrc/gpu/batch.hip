@@ -176,7 +176,7 @@ int build_tables(mithra_batch* b) {
       if (!v.seeded) return;
       const long lines = axis == 0 ? (long)(v.N1 - 4) * (v.N2 - 4)
                                    : axis == 1 ? (long)(v.N0 - 4) * (v.N2 - 4) : (long)(v.N0 - 4) * (v.N1 - 4);
-      add(i, (lines + 255) / 256);
+      add(i, (lines + 63) / 64);  // 256 work items (four injections of 64 lines) per entry
     });
   if (all.size() > b->tiles_cap) {
     (void)hipFree(b->tiles);

@@ -331,43 +331,53 @@ __device__ inline void tfsf_apply(FieldSet A, long m, double px, double py, doub
 
 // TF/SF injection (src/fdtd.cpp:307-373).  axis 0: planes i = 1, 2, N0-2, N0-3 over (j, k);
 // axis 1: planes j = 1, 2, N1-2, N1-3 over (i, k); axis 2: planes k = 1, 2, np-2, np-3 over (i, j).
-// The three axes run as three launches in the reference's loop order; one thread applies the
-// four plane cases of its line in the reference's order, so a node touched twice (tiny meshes)
-// receives its contributions in the same sequence.
+// The three axes run as three launches in the reference's loop order.  Launches carry four work
+// items per line.  Where the axis has 6 nodes or more its four planes are distinct, and work item
+// t = 64 (4 g + q) + l applies injection q of line 64 g + l alone (q is uniform over a
+// wavefront, so each wavefront follows one branch).  On a shorter axis two planes coincide, and
+// work item t < lines applies all four injections of line t in the reference's order, so the node
+// touched twice receives its contributions in the same sequence.
 template <typename R>
 __device__ inline void tfsf_body(FieldSet A, int axis, int N0, int N1, int N2, const mithra_grid& g,
                                  const mithra_seed& s, double time, double coef, long t) {
   const long N1N0 = (long)N1 * N0;
+  const bool split = (axis == 0 ? N0 : axis == 1 ? N1 : N2) >= 6;
+  int q = 0;
+  if (split) {
+    q = (int)((t >> 6) & 3);
+    t = ((t >> 8) << 6) + (t & 63);
+  }
+  const bool q0 = !split || q == 0, q1 = !split || q == 1, q2 = !split || q == 2, q3 = !split || q == 3;
   if (axis == 0) {
     const long nj = N1 - 4, nk = N2 - 4;
     if (t >= nj * nk) return;
     const long j = 2 + t % nj, k = 2 + t / nj;
     const double y = g.ymin + j * g.dy, z = g.zmin + k * g.dz;
     const long base = N1N0 * k + j;
-    tfsf_apply<R>(A, base + N1 * 1, g.xmin + 2 * g.dx, y, z, time, s, coef, true);
-    tfsf_apply<R>(A, base + N1 * 2, g.xmin + 1 * g.dx, y, z, time, s, coef, false);
-    tfsf_apply<R>(A, base + (long)N1 * (N0 - 2), g.xmin + (N0 - 3) * g.dx, y, z, time, s, coef, true);
-    tfsf_apply<R>(A, base + (long)N1 * (N0 - 3), g.xmin + (N0 - 2) * g.dx, y, z, time, s, coef, false);
+    if (q0) tfsf_apply<R>(A, base + N1 * 1, g.xmin + 2 * g.dx, y, z, time, s, coef, true);
+    if (q1) tfsf_apply<R>(A, base + N1 * 2, g.xmin + 1 * g.dx, y, z, time, s, coef, false);
+    if (q2) tfsf_apply<R>(A, base + (long)N1 * (N0 - 2), g.xmin + (N0 - 3) * g.dx, y, z, time, s, coef, true);
+    if (q3) tfsf_apply<R>(A, base + (long)N1 * (N0 - 3), g.xmin + (N0 - 2) * g.dx, y, z, time, s, coef, false);
   } else if (axis == 1) {
     const long ni = N0 - 4, nk = N2 - 4;
     if (t >= ni * nk) return;
     const long i = 2 + t % ni, k = 2 + t / ni;
     const double x = g.xmin + i * g.dx, z = g.zmin + k * g.dz;
     const long base = N1N0 * k + (long)N1 * i;
-    tfsf_apply<R>(A, base + 1, x, g.ymin + 2 * g.dy, z, time, s, coef, true);
-    tfsf_apply<R>(A, base + 2, x, g.ymin + 1 * g.dy, z, time, s, coef, false);
-    tfsf_apply<R>(A, base + N1 - 2, x, g.ymin + (N1 - 3) * g.dy, z, time, s, coef, true);
-    tfsf_apply<R>(A, base + N1 - 3, x, g.ymin + (N1 - 2) * g.dy, z, time, s, coef, false);
+    if (q0) tfsf_apply<R>(A, base + 1, x, g.ymin + 2 * g.dy, z, time, s, coef, true);
+    if (q1) tfsf_apply<R>(A, base + 2, x, g.ymin + 1 * g.dy, z, time, s, coef, false);
+    if (q2) tfsf_apply<R>(A, base + N1 - 2, x, g.ymin + (N1 - 3) * g.dy, z, time, s, coef, true);
+    if (q3) tfsf_apply<R>(A, base + N1 - 3, x, g.ymin + (N1 - 2) * g.dy, z, time, s, coef, false);
   } else {
     const long ni = N0 - 4, nj = N1 - 4;
     if (t >= ni * nj) return;
     const long j = 2 + t % nj, i = 2 + t / nj;
     const double x = g.xmin + i * g.dx, y = g.ymin + j * g.dy;
     const long base = (long)N1 * i + j;
-    tfsf_apply<R>(A, base + N1N0 * 1, x, y, g.zmin + 2 * g.dz, time, s, coef, true);
-    tfsf_apply<R>(A, base + N1N0 * 2, x, y, g.zmin + 1 * g.dz, time, s, coef, false);
-    tfsf_apply<R>(A, base + N1N0 * (N2 - 2), x, y, g.zmin + (N2 - 3) * g.dz, time, s, coef, true);
-    tfsf_apply<R>(A, base + N1N0 * (N2 - 3), x, y, g.zmin + (N2 - 2) * g.dz, time, s, coef, false);
+    if (q0) tfsf_apply<R>(A, base + N1N0 * 1, x, y, g.zmin + 2 * g.dz, time, s, coef, true);
+    if (q1) tfsf_apply<R>(A, base + N1N0 * 2, x, y, g.zmin + 1 * g.dz, time, s, coef, false);
+    if (q2) tfsf_apply<R>(A, base + N1N0 * (N2 - 2), x, y, g.zmin + (N2 - 3) * g.dz, time, s, coef, true);
+    if (q3) tfsf_apply<R>(A, base + N1N0 * (N2 - 3), x, y, g.zmin + (N2 - 2) * g.dz, time, s, coef, false);
   }
 }
 
@@ -1422,7 +1432,7 @@ hipError_t tfsf_impl(Domain* d, double time, const mithra_seed* seed) {
   const double coef[3] = {d->co.a[1], d->co.a[2], d->co.a[3]};
   for (int axis = 0; axis < 3; ++axis) {
     if (n[axis] <= 0) continue;
-    hipLaunchKernelGGL((k_tfsf<R>), dim3((unsigned)((n[axis] + 255) / 256)), dim3(256), 0, d->stream,
+    hipLaunchKernelGGL((k_tfsf<R>), dim3((unsigned)((n[axis] + 63) / 64)), dim3(256), 0, d->stream,
                        d->lvl[d->i_np1], axis, g.N0, g.N1, g.N2, g, *seed, time, coef[axis]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
