@@ -10,6 +10,7 @@ namespace {
 
 
 __device__ inline double pmod(double a, double b) {  // src/stdinclude.cpp:88-93
+  if (a >= 0.0 && a < b) return a;  // fmod(a, b) == a exactly here, and a is not negative
   double x = fmod(a, b);
   x += (x < 0.0) ? b : 0.0;
   return x;

@@ -49,6 +49,9 @@ __device__ inline void power_nodes_body(const float* __restrict__ eb, double* __
       }
       double e1r = 0.0, e1i = 0.0, b1r = 0.0, b1i = 0.0, e2r = 0.0, e2i = 0.0, b2r = 0.0, b2i = 0.0;
       const gptr<const double> w = dftg + (size_t)l * Nf * 2;
+      // few workgroups (one per 256 nodes of a plane): unrolled, a thread keeps eight slots'
+      // loads in flight; the sums still run over m in order
+#pragma unroll 8
       for (unsigned m = 0; m < Nf; ++m) {
         const gptr<const double> r = ringg + (size_t)m * ring_slot;
         const double f0 = r[0 * Nz * N1N0 + base], f1 = r[1 * Nz * N1N0 + base];
