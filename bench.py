@@ -515,7 +515,7 @@ def main():
                     help="deposits between reorders of the particle arrays into cell-row order (0: never)")
     ap.add_argument("--workload", choices=("production", "c3"), default="production",
                     help="production (256x256x2048, no space charge) or c3 (128x128x1024, space charge, "
-                         "Gaussian seed; the plain step, as the pipelined form covers runs without space charge)")
+                         "Gaussian seed)")
     ap.add_argument("--plain", action="store_true",
                     help="time the plain step (mithra_dev_step) instead of the pipelined one")
     ap.add_argument("--surrogate", type=int, default=1024,
@@ -540,9 +540,8 @@ def main():
         n_part = job.particles().shape[0]
         n_sub = job.times()["n_sub"]
         log(f"set-up done in {t_setup:.1f} s; timing {args.steps} GPU steps after {args.warmup} warm-up")
-        # the library pipelines FP64 without space charge only; FP32-bulk takes the plain step
-        pipelined = (not args.plain and not args.deterministic and precision == abi.PRECISION_FP64
-                     and args.workload == "production")
+        # the library pipelines FP64 runs (with or without space charge); FP32-bulk takes the plain step
+        pipelined = not args.plain and not args.deterministic and precision == abi.PRECISION_FP64
         if args.workload == "production" and args.warmup + args.steps < 1500:
             log(f"note: steps {args.warmup}..{args.warmup + args.steps - 1} end before most particles emit "
                 "(about step 1500); the push then skips its grid gather and the step is cheaper than the "
@@ -556,9 +555,10 @@ def main():
         # the dominant field kernel and its algorithmic bytes per interior node
         real = 8 if precision == abi.PRECISION_FP64 else 4
         ncomp = 4 if args.workload == "c3" else 3
-        if pipelined:   # reads A^{n+1}, A^n; writes X and E/B (6 floats)
-            kernel, phase, bytes_per_node = "stencil_eb", "eval_eb", 9 * real + 24
-            kernel_name = "k_stencil_eb (E/B fused with the next NSFD update's stencil)"
+        if pipelined:   # reads A^{n+1}, A^n (and phi's two levels); writes X (and phi's) and E/B (6 floats)
+            kernel, phase, bytes_per_node = "stencil_eb", "eval_eb", 3 * ncomp * real + 24
+            kernel_name = ("k_stencil_eb (E/B fused with the next NSFD update's stencil"
+                           + (", A and phi)" if ncomp == 4 else ")"))
         else:           # reads A^n, A^{n-1}, J (and phi's three levels with space charge); writes A^{n+1}
             kernel, phase, bytes_per_node = "interior", "interior", 4 * ncomp * real
             kernel_name = "k_interior (NSFD stencil" + (", A and phi)" if ncomp == 4 else ")")

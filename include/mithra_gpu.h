@@ -534,8 +534,10 @@ int mithra_dev_step_timed(mithra_dev* dev, double time, const mithra_seed* seed,
  * the reference's between-step state (J is not held), so mithra_dev_get/set_potentials,
  * field_update, field_evaluate, field_shift, current_reset, current_update, autotune and
  * batch_add return MITHRA_E_STATE; particles, E/B, power samples and counters stay readable.
- * Pipelining needs FP64, no space charge, particles and the row-tiled deposit; otherwise
- * more = 1 runs as more = 0 (and nothing is pending).  Its results are bit-identical to
+ * Under space charge phi is a fourth component: its X takes the same stencil, the fused E/B
+ * adds the gradient of phi^n, and the deposit writes X + a5 rho for it (src/fdtdSC.cpp).
+ * Pipelining needs FP64, particles, the row-tiled deposit and no fixed-order deposit;
+ * otherwise more = 1 runs as more = 0 (and nothing is pending).  Its results are bit-identical to
  * mithra_dev_step's (the deposit's summation order aside).  ms: NULL, or MITHRA_STEP_PHASES durations as
  * mithra_dev_step_timed gives them (phase 0 is zero after a pipelined step; phase 3 is the
  * fused kernel when more = 1).  No counterpart in the reference: its loop body reordered. */

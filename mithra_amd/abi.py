@@ -381,8 +381,9 @@ class Domain:
         """step() pipelined across calls (device library only): with more=True the E/B pass is
         fused with the next interior update and the deposit adds its J to that update, so the
         next call skips the interior update; the potentials are unavailable until a call with
-        more=False ends the run.  FP64 without space charge only (elsewhere more=True runs the
-        plain step).  Returns the phase times in ms when timed, else None."""
+        more=False ends the run.  FP64 only, with or without space charge (FP32-bulk, no
+        particles or a fixed-order deposit: more=True runs the plain step).  Returns the phase
+        times in ms when timed, else None."""
         arr = (Undulator * max(1, len(undulators)))(*undulators)
         ms = (C.c_float * 8)() if timed else None
         self._check(self.f["step_pipelined"](self.h, time, None if seed is None else C.byref(seed),

@@ -157,7 +157,7 @@ int run_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_b
   if (!pipe_in && (rc = settle(d))) return rc;
   // FP64 only: in FP32 the LDS-staged field kernels run at half the speed of the register
   // forms, and the fused kernel stages in LDS
-  const bool pipe_out = more && d->precision == MITHRA_PRECISION_FP64 && d->ncomp == 3 && d->np > 0 &&
+  const bool pipe_out = more && d->precision == MITHRA_PRECISION_FP64 && d->np > 0 &&
                         deposit_tiled(d) && !d->deterministic;
   if (ms && !d->ev_ready) {
     for (auto& e : d->ev) MITHRA_TRY(hipEventCreate(&e));
@@ -837,7 +837,6 @@ int mithra_fused_tile_shape(int variant, int* rows, int* planes) {
 
 int mithra_dev_autotune_fused(mithra_dev* d, int reps, int* chosen, float* ms) {
   if (!d || reps < 1) return MITHRA_E_ARG;
-  if (d->ncomp != 3) return MITHRA_E_UNSUPPORTED;
   if (int rc = settle(d)) return rc;
   hipStream_t s = d->stream;
   // the kernel writes its X into the A^{n-1} level and E/B into the field array: time it there

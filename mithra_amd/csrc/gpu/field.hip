@@ -965,14 +965,16 @@ struct StencilConst {
 #endif
 constexpr int kStencilWaves = MITHRA_STENCIL_WAVES;  // waves per SIMD the register budget allows
 
-template <typename R, bool NSFD, int TI, int K>
-__global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldSet An1, FieldSet An, FieldSet X,
-                                                           float* __restrict__ eb, int N0, int N1, int N2, int tilesJ,
-                                                           int tilesI, StencilConst s, EvalConst q,
-                                                           const int* __restrict__ zr, int pb0, int pb1) {
+template <typename R, bool NSFD, int TI, int K, bool SC = false>
+__global__ __launch_bounds__(kTileJ* TI, SC ? 3 : kStencilWaves) void k_stencil_eb(
+    FieldSet An1, FieldSet An, FieldSet X, float* __restrict__ eb, int N0, int N1, int N2, int tilesJ, int tilesI,
+    StencilConst s, EvalConst q, const int* __restrict__ zr, int pb0, int pb1) {
+  // Components: A_x, A_y, A_z, and phi under space charge (the same stencil; its E/B terms are
+  // the gradient of phi^n, as in eval_lds_body).
+  constexpr int NC = SC ? 4 : 3;
   constexpr int W = kTileJ + 2, P = (TI + 2) * W;
-  __shared__ double L1[9 * P];  // A^{n+1}: [component][slot][P]
-  __shared__ double L0[3 * P];  // A^n, plane k: [component][P]
+  __shared__ double L1[3 * NC * P];  // A^{n+1}: [component][slot][P]
+  __shared__ double L0[NC * P];      // A^n, plane k: [component][P]
   int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tj = lb % tilesJ;
   lb /= tilesJ;
@@ -988,9 +990,14 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
   const int j = j0 + lane, i = i0 + w;
   const bool active = i <= N0 - 2 && j >= 1 && j <= N1 - 2;
   const long N1N0 = (long)N1 * N0;
-  const gptr<const R> p[3] = {G((const R*)An1.c[0]), G((const R*)An1.c[1]), G((const R*)An1.c[2])};
-  const gptr<const R> n[3] = {G((const R*)An.c[0]), G((const R*)An.c[1]), G((const R*)An.c[2])};
-  const gptr<R> x[3] = {G((R*)X.c[0]), G((R*)X.c[1]), G((R*)X.c[2])};
+  gptr<const R> p[NC], n[NC];
+  gptr<R> x[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    p[c] = G((const R*)component(An1, c));
+    n[c] = G((const R*)component(An, c));
+    x[c] = G((R*)component(X, c));
+  }
   // staged per plane by this thread: its own node, and (as in k_interior_lds) a halo row node
   // (waves 0 and 1) or a halo column node (lanes < 2 TI of wave 0)
   const bool own_ok = i <= N0 - 1 && j <= N1 - 1;
@@ -1009,11 +1016,11 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
   const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
   const long hcol_off = (long)N1 * hcol_i + hcol_j;
   // A^{n+1} of the plane to stage next
-  double p_own[3] = {}, p_row[3] = {}, p_col[3] = {};
+  double p_own[NC] = {}, p_row[NC] = {}, p_col[NC] = {};
   auto fetch_p = [&](int k) {
     const long b = N1N0 * k;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if (own_ok) p_own[c] = (double)p[c][b + own_off];
       if (hrow_ok) p_row[c] = (double)p[c][b + hrow_off];
       if (hcol_ok) p_col[c] = (double)p[c][b + hcol_off];
@@ -1021,21 +1028,21 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
   };
   auto stage_p = [&](int slot) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       double* d = L1 + (c * 3 + slot) * P;
       if (own_ok) d[own_slot] = p_own[c];
       if (hrow_ok) d[hrow_slot] = p_row[c];
       if (hcol_ok) d[hcol_slot] = p_col[c];
     }
   };
-  // A^n: own A_x, A_y at planes k-1 .. k+2 (qx, qy), own A_z at k and k+1, halo of the next plane.
-  // The curl reads A^n_x only at y +- 1 and A^n_y only at x +- 1, so A^n_x needs no halo rows and
-  // A^n_y no halo columns.
-  double qx[4] = {}, qy[4] = {}, qz = 0.0, qz_next = 0.0, n_row[3] = {}, n_col[3] = {};
+  // A^n: own A_x, A_y (and phi) at planes k-1 .. k+2 (qx, qy, qf), own A_z at k and k+1, halo
+  // of the next plane.  The curl reads A^n_x only at y +- 1 and A^n_y only at x +- 1, so A^n_x
+  // needs no halo rows and A^n_y no halo columns; the gradient of phi needs both.
+  double qx[4] = {}, qy[4] = {}, qf[4] = {}, qz = 0.0, qz_next = 0.0, n_row[NC] = {}, n_col[NC] = {};
   auto fetch_n_halo = [&](int k) {
     const long b = N1N0 * k;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if (c != 0 && hrow_ok) n_row[c] = (double)n[c][b + hrow_off];
       if (c != 1 && hcol_ok) n_col[c] = (double)n[c][b + hcol_off];
     }
@@ -1050,6 +1057,7 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
     for (int t = 0; t < 3; ++t) {
       qx[t] = (double)n[0][N1N0 * (kb - 1 + t) + own_off];
       qy[t] = (double)n[1][N1N0 * (kb - 1 + t) + own_off];
+      if (SC) qf[t] = (double)n[NC - 1][N1N0 * (kb - 1 + t) + own_off];
     }
     qz = (double)n[2][N1N0 * kb + own_off];
   }
@@ -1064,9 +1072,9 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
     const int s0 = (k - kb + 1) % 3, sp = (k - kb + 2) % 3, sm = (k - kb) % 3;
     stage_p(sp);
     {
-      const double own[3] = {qx[1], qy[1], qz};
+      const double own[4] = {qx[1], qy[1], qz, qf[1]};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < NC; ++c) {
         double* d = L0 + c * P;
         if (own_ok) d[own_slot] = own[c];
         if (c != 0 && hrow_ok) d[hrow_slot] = n_row[c];
@@ -1080,13 +1088,14 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
       if (own_ok) {
         qx[3] = (double)n[0][m + 2 * N1N0];
         qy[3] = (double)n[1][m + 2 * N1N0];
+        if (SC) qf[3] = (double)n[NC - 1][m + 2 * N1N0];
         qz_next = (double)n[2][m + N1N0];
       }
     }
     if (active) {
-      const double v1[3] = {qx[1], qy[1], qz};
+      const double v1[4] = {qx[1], qy[1], qz, qf[1]};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const double* c0 = L1 + (c * 3 + s0) * P + own_slot;
         const double* cp = L1 + (c * 3 + sp) * P + own_slot;
         const double* cm = L1 + (c * 3 + sm) * P + own_slot;
@@ -1109,10 +1118,16 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
         const double* p2l = L1 + (2 * 3 + s0) * P + own_slot;
         const double p0p = L1[(0 * 3 + sp) * P + own_slot], p0m = L1[(0 * 3 + sm) * P + own_slot];
         const double p1p = L1[(1 * 3 + sp) * P + own_slot], p1m = L1[(1 * 3 + sm) * P + own_slot];
-        const double ex = dt_(p0l[0]) - dt_(qx[1]);
-        const double ey = dt_(p1l[0]) - dt_(qy[1]);
+        double ex = dt_(p0l[0]) - dt_(qx[1]);
+        double ey = dt_(p1l[0]) - dt_(qy[1]);
         float ez = (float)dt_(p2l[0]);
         ez = (float)((double)ez - dt_(n2l[0]));
+        if (SC) {  // minus the gradient of phi^n (eval_lds_body's expressions)
+          const double* fl = L0 + (NC - 1) * P + own_slot;
+          ex = ex - dx_(fl[W] - fl[-W]);
+          ey = ey - dy_(fl[1] - fl[-1]);
+          ez = (float)((double)ez - dz_(qf[2] - qf[0]));
+        }
         const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_(qy[2] - qy[0]) +
                                         dy_(p2l[1] - p2l[-1]) - dz_(p1p - p1m)));
         const float by = (float)(0.5 * (dz_(qx[2] - qx[0]) - dx_(n2l[W] - n2l[-W]) +
@@ -1133,6 +1148,7 @@ __global__ __launch_bounds__(kTileJ* TI, kStencilWaves) void k_stencil_eb(FieldS
     __syncthreads();
     qx[0] = qx[1]; qx[1] = qx[2]; qx[2] = qx[3];
     qy[0] = qy[1]; qy[1] = qy[2]; qy[2] = qy[3];
+    qf[0] = qf[1]; qf[1] = qf[2]; qf[2] = qf[3];
     qz = qz_next;
   }
 }
@@ -1331,7 +1347,7 @@ hipError_t field_variant(Domain* d, int kernel, int v, int klo, int khi, const i
   return hipErrorInvalidValue;
 }
 
-template <typename R, bool NSFD, int TI, int K>
+template <typename R, bool NSFD, int TI, int K, bool SC>
 hipError_t stencil_eb_launch(Domain* d, const int* zr, int pb0, int pb1) {
   const mithra_grid& g = d->g;
   const int tilesJ = (g.N1 + kTileJ - 1) / kTileJ;
@@ -1343,20 +1359,24 @@ hipError_t stencil_eb_launch(Domain* d, const int* zr, int pb0, int pb1) {
   EvalConst q;
   q.mdt = -g.dt; q.dx2 = 2.0 * g.dx; q.dy2 = 2.0 * g.dy; q.dz2 = 2.0 * g.dz;
   q.rmdt = 1.0 / q.mdt; q.rdx2 = 1.0 / q.dx2; q.rdy2 = 1.0 / q.dy2; q.rdz2 = 1.0 / q.dz2;
-  hipLaunchKernelGGL((k_stencil_eb<R, NSFD, TI, K>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
+  hipLaunchKernelGGL((k_stencil_eb<R, NSFD, TI, K, SC>), dim3((unsigned)nblk), dim3(kTileJ * TI), 0, d->stream,
                      d->lvl[d->i_np1], d->lvl[d->i_n], d->lvl[d->i_nm1], d->eb, g.N0, g.N1, g.N2, tilesJ, tilesI, s,
                      q, zr, pb0, pb1);
   return hipGetLastError();
 }
 
-template <typename R, bool NSFD>
+template <typename R, bool NSFD, bool SC = false>
 hipError_t stencil_eb_variant(Domain* d, const int* zr, int pb0, int pb1) {
   static_assert(kFusedVariants == 4, "variant table and switch disagree");
   switch (d->tile_fused) {
-    case 0: return stencil_eb_launch<R, NSFD, 4, 64>(d, zr, pb0, pb1);
-    case 1: return stencil_eb_launch<R, NSFD, 8, 64>(d, zr, pb0, pb1);
-    case 2: return stencil_eb_launch<R, NSFD, 16, 64>(d, zr, pb0, pb1);
-    case 3: return stencil_eb_launch<R, NSFD, 8, 128>(d, zr, pb0, pb1);
+    case 0: return stencil_eb_launch<R, NSFD, 4, 64, SC>(d, zr, pb0, pb1);
+    case 1: return stencil_eb_launch<R, NSFD, 8, 64, SC>(d, zr, pb0, pb1);
+    // under space charge 16-row tiles need more than the 128 registers a 1024-thread
+    // workgroup allows (they spill): that variant runs the 8-row form
+    case 2:
+      if constexpr (SC) return stencil_eb_launch<R, NSFD, 8, 64, SC>(d, zr, pb0, pb1);
+      else return stencil_eb_launch<R, NSFD, 16, 64, SC>(d, zr, pb0, pb1);
+    case 3: return stencil_eb_launch<R, NSFD, 8, 128, SC>(d, zr, pb0, pb1);
   }
   return hipErrorInvalidValue;
 }
@@ -1369,6 +1389,9 @@ hipError_t stencil_eb_impl(Domain* d) {
     zr = d->zrange;
     lazy_planes(d, &pb0, &pb1);
   }
+  if (d->ncomp == 4)
+    return d->g.solver == MITHRA_SOLVER_NSFD ? stencil_eb_variant<R, true, true>(d, zr, pb0, pb1)
+                                             : stencil_eb_variant<R, false, true>(d, zr, pb0, pb1);
   return d->g.solver == MITHRA_SOLVER_NSFD ? stencil_eb_variant<R, true>(d, zr, pb0, pb1)
                                            : stencil_eb_variant<R, false>(d, zr, pb0, pb1);
 }
@@ -1488,7 +1511,8 @@ hipError_t launch_interior(Domain* d, bool fuse_yfaces) {
 extern const int kFusedShapes[kFusedVariants][2] = {{4, 64}, {8, 64}, {16, 64}, {8, 128}};
 
 hipError_t launch_stencil_eb(Domain* d) {
-  if (d->ncomp != 3 || d->tile_fused < 0 || d->tile_fused >= kFusedVariants) return hipErrorInvalidValue;
+  if ((d->ncomp != 3 && d->ncomp != 4) || d->tile_fused < 0 || d->tile_fused >= kFusedVariants)
+    return hipErrorInvalidValue;
   return d->precision == MITHRA_PRECISION_FP64 ? stencil_eb_impl<double>(d) : stencil_eb_impl<float>(d);
 }
 

@@ -804,7 +804,7 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
                                          const unsigned* __restrict__ p_row, const unsigned* __restrict__ p_idx,
                                          const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
                                          int TI, int tilesI, int KC, int accumulate, double src_coef,
-                                         int skip_empty, int block) {
+                                         double src_rho, int skip_empty, int block) {
   extern __shared__ double tile[];  // [2][ncomp][TI][N1]
   constexpr int ncomp = SC ? 4 : 3;
   const int chunk = block / tilesI;
@@ -859,8 +859,9 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
     }
     __syncthreads();
     // Node plane kc takes halves of cell planes kc-1 and kc only.  When neither lists any, its J
-    // is +0, and in the pipelined form X + a4 J = X bit for bit when a4 carries a minus sign
-    // (a4 (+0) = -0, and x + -0 = x for every x, -0 included), so that plane is left untouched.
+    // (and rho) is +0, and in the pipelined form X + a4 J = X bit for bit when a4 (and a5, the
+    // coefficient of rho) carries a minus sign (a4 (+0) = -0, and x + -0 = x for every x, -0
+    // included), so that plane is left untouched.
     const bool untouched = SRC && skip_empty && prev_total + total == 0;
     prev_total = total;
     if (kc >= kb) {  // node plane kc is complete: write it, and its slot becomes plane kc+2
@@ -879,10 +880,12 @@ __device__ inline void deposit_rows_body(const double* __restrict__ part, long c
         for (int comp = 0; comp < ncomp; ++comp) {
           const gptr<R> out = G((R*)component(J, comp));
           if (SRC) {
-            // the pipelined step: X + a4 J at interior nodes, J rounded as the buffer would hold it
+            // the pipelined step: X + a4 J (and X + a5 rho for phi) at interior nodes, J rounded as
+            // the buffer would hold it
             if (inner) {
               const double xo = (double)ld_stream((gptr<const R>)out, base + o);  // read once, written once
-              st_stream(out, base + o, (R)(xo + src_coef * (double)(R)p[comp * tsz + o]));
+              const double coef = comp < 3 ? src_coef : src_rho;
+              st_stream(out, base + o, (R)(xo + coef * (double)(R)p[comp * tsz + o]));
             }
           } else {
             out[base + o] = (R)p[comp * tsz + o];
@@ -899,9 +902,9 @@ template <typename R, bool SC, bool SRC = false>
 __global__ __launch_bounds__(kDepositThreads, SC ? 4 : 8) void k_deposit_rows(
     const double* __restrict__ part, long cap, FieldSet J, DepositConst c, const unsigned* __restrict__ p_row,
     const unsigned* __restrict__ p_idx, const unsigned* __restrict__ m_row, const unsigned* __restrict__ m_idx,
-    int TI, int tilesI, int KC, int accumulate, double src_coef, int skip_empty) {
+    int TI, int tilesI, int KC, int accumulate, double src_coef, double src_rho, int skip_empty) {
   deposit_rows_body<R, SC, SRC>(part, cap, J, c, p_row, p_idx, m_row, m_idx, TI, tilesI, KC, accumulate, src_coef,
-                                skip_empty, blockIdx.x);
+                                src_rho, skip_empty, blockIdx.x);
 }
 
 // Bunching factor partial sums: per workgroup sum of q cos(k z), q sin(k z), q over its particles.
@@ -1122,10 +1125,10 @@ __global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* _
       const DepositConst c = v.dep;  // a local copy: the LDS atomics cannot alias it
       if (v.ncomp == 4)  // space charge: rho rides as the fourth component
         deposit_rows_body<R, true>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
-                                   v.dtilesI, kDepositChunk, 0, 0.0, 0, e.y);
+                                   v.dtilesI, kDepositChunk, 0, 0.0, 0.0, 0, e.y);
       else
         deposit_rows_body<R, false>(v.part, v.cap, v.lvl[v.i_np1], c, po.row, po.idx, mo.row, mo.idx, v.dTI,
-                                    v.dtilesI, kDepositChunk, 0, 0.0, 0, e.y);
+                                    v.dtilesI, kDepositChunk, 0, 0.0, 0.0, 0, e.y);
     }
     __syncthreads();  // the LDS tile is reused by the next entry
   }
@@ -1407,9 +1410,9 @@ hipError_t finish_deposit(Domain* d);
 
 hipError_t launch_deposit(Domain* d, int accumulate) {
   const mithra_grid& g = d->g;
-  if (accumulate == kDepositSource && (!deposit_tiled(d) || d->np == 0 || d->ncomp != 3 || d->deterministic))
+  if (accumulate == kDepositSource && (!deposit_tiled(d) || d->np == 0 || d->deterministic))
     return hipErrorInvalidValue;
-  const double src_coef = d->co.a[4];
+  const double src_coef = d->co.a[4], src_rho = d->co.a[5];
   if (d->deterministic && d->np > 0) {
     const unsigned rows = (unsigned)((g.N2 - 1) * (long)g.N0);
     hipError_t e;
@@ -1452,25 +1455,30 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   const dim3 grid((unsigned)deposit_blocks(g.N2, tilesI, KC));
   const unsigned *pr = d->ord[0].row, *pi = d->ord[0].idx, *mr = d->ord[1].row, *mi = d->ord[1].idx;
   const bool f64 = d->precision == MITHRA_PRECISION_FP64;
-  const int skip = std::signbit(src_coef) ? 1 : 0;  // see deposit_rows_body: exact only then
-  if (accumulate == kDepositSource && f64)
-    hipLaunchKernelGGL((k_deposit_rows<double, false, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part,
-                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
-  else if (accumulate == kDepositSource)
-    hipLaunchKernelGGL((k_deposit_rows<float, false, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part,
-                       d->cap, J, c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
-  else if (f64 && c.sc)
-    hipLaunchKernelGGL((k_deposit_rows<double, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
-  else if (f64)
-    hipLaunchKernelGGL((k_deposit_rows<double, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
-  else if (c.sc)
-    hipLaunchKernelGGL((k_deposit_rows<float, true>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
-  else
-    hipLaunchKernelGGL((k_deposit_rows<float, false>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, c, pr, pi,
-                       mr, mi, TI, tilesI, KC, accumulate, src_coef, skip);
+  // see deposit_rows_body: exact only when every source coefficient carries a minus sign
+  const int skip = std::signbit(src_coef) && (!c.sc || std::signbit(src_rho)) ? 1 : 0;
+#define MITHRA_DEPOSIT(R_, SC_, SRC_)                                                                          \
+  hipLaunchKernelGGL((k_deposit_rows<R_, SC_, SRC_>), grid, dim3(kDepositThreads), lds, d->stream, d->part, d->cap, J, \
+                     c, pr, pi, mr, mi, TI, tilesI, KC, accumulate, src_coef, src_rho, skip)
+  const bool src = accumulate == kDepositSource;
+  if (f64) {
+    if (c.sc) {
+      if (src) MITHRA_DEPOSIT(double, true, true);
+      else MITHRA_DEPOSIT(double, true, false);
+    } else {
+      if (src) MITHRA_DEPOSIT(double, false, true);
+      else MITHRA_DEPOSIT(double, false, false);
+    }
+  } else {
+    if (c.sc) {
+      if (src) MITHRA_DEPOSIT(float, true, true);
+      else MITHRA_DEPOSIT(float, true, false);
+    } else {
+      if (src) MITHRA_DEPOSIT(float, false, true);
+      else MITHRA_DEPOSIT(float, false, false);
+    }
+  }
+#undef MITHRA_DEPOSIT
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return finish_deposit(d);
 }

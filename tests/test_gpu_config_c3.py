@@ -3,11 +3,11 @@ a 30 MeV, 500 A Gaussian beam (sigma_r 200 um, sigma_z 100 um, 0.2 % spread), K 
 a 1 MW Gaussian-beam seed of 2 mm waist at the resonant wavelength (bench.py's C3 workload, through
 the drop-in host set-up).
 
-1. FP64 against the oracle: the plain GPU step (space charge takes the plain step) and the oracle's
-   call-by-call march of the reference's loop body for 3 steps from the same set-up.  Bar: every
-   level of A and phi, J, rho and the particles within 1e-12 relative L2, P(z) within 1e-10 of its
-   peak.  The seed's analytic field uses the device's libm and the deposit sums in another order,
-   both far inside the bar.
+1. FP64 against the oracle: the plain GPU step, the pipelined GPU step (phi's X in the fused
+   kernel, X + a5 rho in the deposit) and the oracle's call-by-call march of the reference's loop
+   body for 3 steps from the same set-up.  Bar: every level of A and phi, J, rho and the particles
+   within 1e-12 relative L2, P(z) within 1e-10 of its peak.  The seed's analytic field uses the
+   device's libm and the deposit sums in another order, both far inside the bar.
 2. FP32-bulk against FP64 on the GPU over 200 steps: P(z) relative L1 error (the reference's metric,
    doc/MITHRA_EXAMPLES/MITHRA_EXAMPLES.tex:182-185) and the bunching factor's error, printed and held
    to the 2 % P(z) budget of the scan path with a wide margin."""
@@ -52,6 +52,35 @@ def plain_march(dom, m, n, k=None):
     return np.array(b)
 
 
+def pipelined_march(dom, m, n):
+    """One pipelined run of n steps (more = 1 on all but the last)."""
+    for s in range(n):
+        dom.step_pipelined(m.time, m.seed, m.bp, m.und, m.tb, m.t["bunch_time_step"], m.t["n_sub"], m.n, m.n,
+                           more=s < n - 1)
+        if s < n - 1:
+            assert dom.pipeline_pending()
+        m.advance()
+    assert not dom.pipeline_pending()
+
+
+def compare(gpu, ora, label):
+    out = {}
+    for lv, name in ((0, "n"), (1, "n-1"), (2, "J/rho")):
+        (ag, fg), (ao, fo) = gpu.get_potentials(lv), ora.get_potentials(lv)
+        out[f"A {name}"] = rel_l2(ag, ao)
+        out[f"phi {name}"] = rel_l2(fg, fo)
+    pg, po = gpu.get_particles(), ora.get_particles()
+    out["particles"] = rel_l2(pg[:, 1:10], po[:, 1:10])
+    P_g = gpu.power_fetch(0, ORACLE_STEPS, 1, 1)[:, 0, 0]
+    P_o = ora.power_fetch(0, ORACLE_STEPS, 1, 1)[:, 0, 0]
+    out["P"] = float(np.max(np.abs(P_g - P_o)) / max(np.max(np.abs(P_o)), 1e-300))
+    print(f"C3 FP64, {label} GPU step vs oracle after {ORACLE_STEPS} steps: {out}; P {P_o}", flush=True)
+    assert np.max(np.abs(P_o)) > 0.0
+    assert np.array_equal(pg[:, 10], po[:, 10])
+    for key, v in out.items():
+        assert v <= (1e-10 if key == "P" else 1e-12), f"{label}: {key}"
+
+
 def test_c3_fp64_against_oracle(c3):
     import bench
     from mithra_amd import abi
@@ -59,10 +88,12 @@ def test_c3_fp64_against_oracle(c3):
     lib = load_oracle()
     lib.oracle_set_threads(min(16, os.cpu_count() or 1))
     gpu = bench.open_domain(c3, abi.PRECISION_FP64, rows=ORACLE_STEPS)
+    pipe = bench.open_domain(c3, abi.PRECISION_FP64, rows=ORACLE_STEPS)
     ora = bench.open_domain(c3, abi.PRECISION_FP64, lib=lib, prefix="oracle_", rows=ORACLE_STEPS)
-    mg, mo = bench.March(c3), bench.March(c3)
+    mg, mp, mo = bench.March(c3), bench.March(c3), bench.March(c3)
     assert abs(mg.seed.amplitude) > 1e-50 and mg.seed.seed_type == 2
     plain_march(gpu, mg, ORACLE_STEPS)
+    pipelined_march(pipe, mp, ORACLE_STEPS)
     for s in range(ORACLE_STEPS):
         ora.field_update(mo.time, mo.seed)
         ora.bunch_begin()
@@ -76,22 +107,10 @@ def test_c3_fp64_against_oracle(c3):
         ora.current_update()
         mo.advance()
         print(f"oracle step {s} done", flush=True)
-    out = {}
-    for lv, name in ((0, "n"), (1, "n-1"), (2, "J/rho")):
-        (ag, fg), (ao, fo) = gpu.get_potentials(lv), ora.get_potentials(lv)
-        out[f"A {name}"] = rel_l2(ag, ao)
-        out[f"phi {name}"] = rel_l2(fg, fo)
-    pg, po = gpu.get_particles(), ora.get_particles()
-    out["particles"] = rel_l2(pg[:, 1:10], po[:, 1:10])
-    P_g = gpu.power_fetch(0, ORACLE_STEPS, 1, 1)[:, 0, 0]
-    P_o = ora.power_fetch(0, ORACLE_STEPS, 1, 1)[:, 0, 0]
-    out["P"] = float(np.max(np.abs(P_g - P_o)) / max(np.max(np.abs(P_o)), 1e-300))
-    print(f"C3 FP64, GPU vs oracle after {ORACLE_STEPS} steps: {out}; P {P_o}", flush=True)
-    assert np.max(np.abs(P_o)) > 0.0
-    assert np.array_equal(pg[:, 10], po[:, 10])
-    for key, v in out.items():
-        assert v <= (1e-10 if key == "P" else 1e-12), key
+    compare(gpu, ora, "plain")
+    compare(pipe, ora, "pipelined")
     gpu.close()
+    pipe.close()
     ora.close()
 
 

@@ -2,16 +2,17 @@
 
 The pipelined step reorders Solver::solve()'s loop body (src/solver.cpp:1300-1399): the E/B
 evaluation of step n is fused with the next interior update without its source term, X, and the
-deposit writes X + a4 J instead of J.  The interior update sums its terms left to right with the
-source last (src/database.cpp:44-134), so in FP64 the split changes no bit.
+deposit writes X + a4 J instead of J (and X + a5 rho for phi under space charge, whose E takes the
+gradient of phi^n in the same kernel).  The interior update sums its terms left to right with
+the source last (src/database.cpp:44-134), so in FP64 the split changes no bit.
 
 Bit-exact checks need a deterministic deposit.  The row-tiled deposit adds into LDS with
 atomics, so a node that receives three or more contributions may round differently from run to
 run.  The exact tests therefore march a sparse subset of the reference's particles, at least
 four z-cells apart, so that no node receives more than two contributions (and a + b = b + a).
 The full bunch is compared to the rounding of the summation order, as the other multi-step tests
-do.  Cases with space charge, and FP32-bulk domains, run the plain step whatever `more` asks
-(the pipelined form covers A only, in FP64), which is checked too."""
+do.  FP32-bulk domains, and domains without particles, run the plain step whatever `more`
+asks, which is checked too."""
 import numpy as np
 import pytest
 
@@ -121,17 +122,15 @@ def test_pending_state_guards_the_potentials(case):
     time, tb = case.time0, case.time_bunch0
     dom2 = open_sparse(case)
     dom2.step_pipelined(time, case.seed, case.bunch, case.undulators, tb, case.dtb, case.n_sub, 0, 0, more=True)
-    pipelined = not case.grid.space_charge
-    assert dom2.pipeline_pending() == pipelined
-    if pipelined:
-        with pytest.raises(RuntimeError):
-            dom2.get_potentials(0)
-        with pytest.raises(RuntimeError):
-            dom2.field_update(time, case.seed)
-        with pytest.raises(RuntimeError):
-            dom2.current_update()
-        dom2.get_particles()
-        dom2.get_fields()
+    assert dom2.pipeline_pending()
+    with pytest.raises(RuntimeError):
+        dom2.get_potentials(0)
+    with pytest.raises(RuntimeError):
+        dom2.field_update(time, case.seed)
+    with pytest.raises(RuntimeError):
+        dom2.current_update()
+    dom2.get_particles()
+    dom2.get_fields()
     for _ in range(case.n_sub):
         tb += case.dtb
     time += case.dt
@@ -143,21 +142,8 @@ def test_pending_state_guards_the_potentials(case):
 
 
 def test_fused_tile_shapes_identical(case):
-    """Every tile shape of the fused kernel gives the same bits; autotune_fused keeps the state.
-    A space-charge domain takes the plain step whatever the fused tile: with every tile shape a
-    3-step pipelined request leaves nothing pending and equals the plain march bit for bit."""
-    if case.grid.space_charge:
-        plain = open_sparse(case)
-        run(case, plain, 3, False)
-        want = list(flatten(state(case, plain, 3)))
-        for v in range(abi.FUSED_TILE_VARIANTS):
-            dom = open_sparse(case)
-            dom.set_fused_tile(v)
-            run(case, dom, 3, True)
-            assert not dom.pipeline_pending()
-            for (k, x), (_, y) in zip(want, flatten(state(case, dom, 3))):
-                assert np.array_equal(x, y), f"fused shape {v}: {k}"
-        return
+    """Every tile shape of the fused kernel gives the same bits, with and without space charge;
+    autotune_fused keeps the state."""
     ref = None
     for v in range(abi.FUSED_TILE_VARIANTS):
         dom = open_sparse(case)
@@ -198,8 +184,7 @@ def test_timed_phases(case):
     ms = run(case, dom, 3, True, timed=True)
     assert all(len(m) == len(abi.PHASES) for m in ms)
     assert all(x >= 0 for m in ms for x in m)
-    if not case.grid.space_charge:
-        assert ms[1][0] < ms[0][0]
+    assert ms[1][0] < ms[0][0]
 
 
 def test_no_particles_runs_plain(case):
