@@ -155,8 +155,8 @@ int run_step(mithra_dev* d, double time, const mithra_seed* seed, const mithra_b
   if (rc) return rc;
   const bool pipe_in = d->pipe != 0;
   if (!pipe_in && (rc = settle(d))) return rc;
-  // FP64 only: in FP32 the LDS-staged field kernels run at half the speed of the register
-  // forms, and the fused kernel stages in LDS
+  // FP64 only: in FP32-bulk the fused kernel saves no time overall (the separate Mur y faces and
+  // the deposit's X reads cost what it saves), and X would be rounded to FP32 before a4 J
   const bool pipe_out = more && d->precision == MITHRA_PRECISION_FP64 && d->np > 0 &&
                         deposit_tiled(d) && !d->deterministic;
   if (ms && !d->ev_ready) {

@@ -190,41 +190,43 @@ __device__ inline void interior_lds_body(FieldSet an_set, FieldSet anm1_set, Fie
   const int fs = j == 0 ? 1 : -1;
   const bool face = FACES && i <= N0 - 2 && (((yf.mask & 1) && j == 0) || ((yf.mask & 2) && j == N1 - 1));
   const int nb_lane = max(lane + fs, 0);
-  double r_own = 0.0, r_row = 0.0, r_col = 0.0;
+  // Prefetched values stay in the storage type until they are staged or used: a conversion
+  // right after a load would make the wave wait for that load there (FP32-bulk).
+  R r_own = 0, r_row = 0, r_col = 0;
   auto fetch = [&](int k) {
     const long b = N1N0 * k;
-    if (own_ok) r_own = (double)an[b + own_off];
-    if (hrow_ok) r_row = (double)an[b + hrow_off];
-    if (hcol_ok) r_col = (double)an[b + hcol_off];
+    if (own_ok) r_own = an[b + own_off];
+    if (hrow_ok) r_row = an[b + hrow_off];
+    if (hcol_ok) r_col = an[b + hcol_off];
   };
   auto stage = [&](int slot) {
     double* p = L + slot * P;
-    if (own_ok) p[own_slot] = r_own;
-    if (hrow_ok) p[hrow_slot] = r_row;
-    if (hcol_ok) p[hcol_slot] = r_col;
+    if (own_ok) p[own_slot] = (double)r_own;
+    if (hrow_ok) p[hrow_slot] = (double)r_row;
+    if (hcol_ok) p[hcol_slot] = (double)r_col;
   };
   // A^{n-1} and J of the next plane are loaded one plane ahead, like the A^n planes, so their
   // latency overlaps the current plane's work.
   const gptr<const R> jsrc = G((const R*)component(anp1_set, comp));
   const bool need_m1 = active || face;
-  double v1_next = 0.0, src_next = 0.0;
+  R v1_next = 0, src_next = 0;
   fetch(kb - 1);
   stage(0);
   fetch(kb);
   stage(1);
   fetch(kb + 1);
   long m = N1N0 * kb + own_off;
-  if (need_m1) v1_next = (double)anm1[m];
-  if (active) src_next = (double)jsrc[m];
+  if (need_m1) v1_next = anm1[m];
+  if (active) src_next = jsrc[m];
   for (int k = kb; k < ke; ++k, m += N1N0) {
     const int s0 = (k - kb + 1) % 3, sp = (k - kb + 2) % 3, sm = (k - kb) % 3;
-    const double v1 = v1_next, src = src_next;
+    const double v1 = (double)v1_next, src = (double)src_next;
     stage(sp);
     __syncthreads();
     if (k + 2 <= ke) fetch(k + 2);
     if (k + 1 < ke) {
-      if (need_m1) v1_next = (double)anm1[m + N1N0];
-      if (active) src_next = (double)jsrc[m + N1N0];
+      if (need_m1) v1_next = anm1[m + N1N0];
+      if (active) src_next = jsrc[m + N1N0];
     }
     const double* c0 = L + s0 * P + own_slot;
     const double* cp = L + sp * P + own_slot;
@@ -836,12 +838,14 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
   }
   const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
   const long hcol_off = (long)N1 * hcol_i + hcol_j;
-  // qz[a][s]: own value of z-array a at planes k-1, k, k+1, k+2 (s = 0..3) of iteration k
-  double qz[NZ][4];
-  double pre_n2 = 0.0, pre_p2 = 0.0, pre_row[NA], pre_col[NA];
+  // qz[a][s]: own value of z-array a at planes k-1, k, k+1, k+2 (s = 0..3) of iteration k.
+  // Prefetched values stay in the storage type until they are staged or used: a conversion
+  // right after a load would make the wave wait for that load there (FP32-bulk).
+  R qz[NZ][4];
+  R pre_n2 = 0, pre_p2 = 0, pre_row[NA], pre_col[NA];
   auto fetch_z = [&](int k, int s) {
 #pragma unroll
-    for (int a = 0; a < NZ; ++a) qz[a][s] = own_ok ? (double)A[zarr[a]][N1N0 * k + own_off] : 0.0;
+    for (int a = 0; a < NZ; ++a) qz[a][s] = own_ok ? A[zarr[a]][N1N0 * k + own_off] : (R)0;
   };
   // The curl reads A_x (arrays 0, 3) only at y +- 1 and A_y (arrays 1, 4) only at x +- 1: they
   // need no halo rows and no halo columns respectively.
@@ -850,13 +854,13 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
   auto fetch_plane = [&](int k) {  // what plane k stages besides the queued own values
     const long b = N1N0 * k;
     if (own_ok) {
-      pre_n2 = (double)A[2][b + own_off];
-      pre_p2 = (double)A[5][b + own_off];
+      pre_n2 = A[2][b + own_off];
+      pre_p2 = A[5][b + own_off];
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      if (need_row(a) && hrow_ok) pre_row[a] = (double)A[a][b + hrow_off];
-      if (need_col(a) && hcol_ok) pre_col[a] = (double)A[a][b + hcol_off];
+      if (need_row(a) && hrow_ok) pre_row[a] = A[a][b + hrow_off];
+      if (need_col(a) && hcol_ok) pre_col[a] = A[a][b + hcol_off];
     }
   };
   fetch_z(kb - 1, 0);
@@ -872,18 +876,18 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
   for (int k = kb; k < ke; ++k, m += N1N0) {
     // stage plane k
     if (own_ok) {
-      L[0 * P + own_slot] = qz[0][1];
-      L[1 * P + own_slot] = qz[1][1];
-      L[2 * P + own_slot] = pre_n2;
-      L[3 * P + own_slot] = qz[2][1];
-      L[4 * P + own_slot] = qz[3][1];
-      L[5 * P + own_slot] = pre_p2;
-      if (SC) L[6 * P + own_slot] = qz[NZ - 1][1];
+      L[0 * P + own_slot] = (double)qz[0][1];
+      L[1 * P + own_slot] = (double)qz[1][1];
+      L[2 * P + own_slot] = (double)pre_n2;
+      L[3 * P + own_slot] = (double)qz[2][1];
+      L[4 * P + own_slot] = (double)qz[3][1];
+      L[5 * P + own_slot] = (double)pre_p2;
+      if (SC) L[6 * P + own_slot] = (double)qz[NZ - 1][1];
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      if (need_row(a) && hrow_ok) L[a * P + hrow_slot] = pre_row[a];
-      if (need_col(a) && hcol_ok) L[a * P + hcol_slot] = pre_col[a];
+      if (need_row(a) && hrow_ok) L[a * P + hrow_slot] = (double)pre_row[a];
+      if (need_col(a) && hcol_ok) L[a * P + hcol_slot] = (double)pre_col[a];
     }
     __syncthreads();
     if (k + 1 < ke) fetch_plane(k + 1);
@@ -895,20 +899,20 @@ __device__ inline void eval_lds_body(FieldSet An, FieldSet Ap, float* __restrict
       const double* p2l = L + 5 * P + own_slot;
       const double* n0l = L + 0 * P + own_slot;
       const double* p0l = L + 3 * P + own_slot;
-      double ex = dt_(qz[2][1]) - dt_(qz[0][1]);
-      double ey = dt_(qz[3][1]) - dt_(qz[1][1]);
+      double ex = dt_((double)qz[2][1]) - dt_((double)qz[0][1]);
+      double ey = dt_((double)qz[3][1]) - dt_((double)qz[1][1]);
       float ez = (float)dt_(p2l[0]);
       ez = (float)((double)ez - dt_(n2l[0]));
       if (SC) {
         const double* fl = L + 6 * P + own_slot;
         ex = ex - dx_(fl[W] - fl[-W]);
         ey = ey - dy_(fl[1] - fl[-1]);
-        ez = (float)((double)ez - dz_(qz[NZ - 1][2] - qz[NZ - 1][0]));
+        ez = (float)((double)ez - dz_((double)qz[NZ - 1][2] - (double)qz[NZ - 1][0]));
       }
-      const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_(qz[1][2] - qz[1][0]) +
-                                      dy_(p2l[1] - p2l[-1]) - dz_(qz[3][2] - qz[3][0])));
-      const float by = (float)(0.5 * (dz_(qz[0][2] - qz[0][0]) - dx_(n2l[W] - n2l[-W]) +
-                                      dz_(qz[2][2] - qz[2][0]) - dx_(p2l[W] - p2l[-W])));
+      const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_((double)qz[1][2] - (double)qz[1][0]) +
+                                      dy_(p2l[1] - p2l[-1]) - dz_((double)qz[3][2] - (double)qz[3][0])));
+      const float by = (float)(0.5 * (dz_((double)qz[0][2] - (double)qz[0][0]) - dx_(n2l[W] - n2l[-W]) +
+                                      dz_((double)qz[2][2] - (double)qz[2][0]) - dx_(p2l[W] - p2l[-W])));
       const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
                                       dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
       const f2 v0 = {(float)ex, (float)ey}, v1 = {ez, bx}, v2 = {by, bz};
@@ -1015,36 +1019,38 @@ __global__ __launch_bounds__(kTileJ* TI, SC ? 3 : kStencilWaves) void k_stencil_
   }
   const bool hcol_ok = hcol_j >= 0 && hcol_j <= N1 - 1 && hcol_i <= N0 - 1;
   const long hcol_off = (long)N1 * hcol_i + hcol_j;
-  // A^{n+1} of the plane to stage next
-  double p_own[NC] = {}, p_row[NC] = {}, p_col[NC] = {};
+  // A^{n+1} of the plane to stage next.  Prefetched values stay in the storage type until they
+  // are used: a conversion right after a load would make the wave wait for that load there,
+  // and the prefetch would overlap nothing (FP32-bulk).
+  R p_own[NC] = {}, p_row[NC] = {}, p_col[NC] = {};
   auto fetch_p = [&](int k) {
     const long b = N1N0 * k;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      if (own_ok) p_own[c] = (double)p[c][b + own_off];
-      if (hrow_ok) p_row[c] = (double)p[c][b + hrow_off];
-      if (hcol_ok) p_col[c] = (double)p[c][b + hcol_off];
+      if (own_ok) p_own[c] = p[c][b + own_off];
+      if (hrow_ok) p_row[c] = p[c][b + hrow_off];
+      if (hcol_ok) p_col[c] = p[c][b + hcol_off];
     }
   };
   auto stage_p = [&](int slot) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       double* d = L1 + (c * 3 + slot) * P;
-      if (own_ok) d[own_slot] = p_own[c];
-      if (hrow_ok) d[hrow_slot] = p_row[c];
-      if (hcol_ok) d[hcol_slot] = p_col[c];
+      if (own_ok) d[own_slot] = (double)p_own[c];
+      if (hrow_ok) d[hrow_slot] = (double)p_row[c];
+      if (hcol_ok) d[hcol_slot] = (double)p_col[c];
     }
   };
   // A^n: own A_x, A_y (and phi) at planes k-1 .. k+2 (qx, qy, qf), own A_z at k and k+1, halo
   // of the next plane.  The curl reads A^n_x only at y +- 1 and A^n_y only at x +- 1, so A^n_x
   // needs no halo rows and A^n_y no halo columns; the gradient of phi needs both.
-  double qx[4] = {}, qy[4] = {}, qf[4] = {}, qz = 0.0, qz_next = 0.0, n_row[NC] = {}, n_col[NC] = {};
+  R qx[4] = {}, qy[4] = {}, qf[4] = {}, qz = 0, qz_next = 0, n_row[NC] = {}, n_col[NC] = {};
   auto fetch_n_halo = [&](int k) {
     const long b = N1N0 * k;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      if (c != 0 && hrow_ok) n_row[c] = (double)n[c][b + hrow_off];
-      if (c != 1 && hcol_ok) n_col[c] = (double)n[c][b + hcol_off];
+      if (c != 0 && hrow_ok) n_row[c] = n[c][b + hrow_off];
+      if (c != 1 && hcol_ok) n_col[c] = n[c][b + hcol_off];
     }
   };
   fetch_p(kb - 1);
@@ -1055,11 +1061,11 @@ __global__ __launch_bounds__(kTileJ* TI, SC ? 3 : kStencilWaves) void k_stencil_
   if (own_ok) {
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-      qx[t] = (double)n[0][N1N0 * (kb - 1 + t) + own_off];
-      qy[t] = (double)n[1][N1N0 * (kb - 1 + t) + own_off];
-      if (SC) qf[t] = (double)n[NC - 1][N1N0 * (kb - 1 + t) + own_off];
+      qx[t] = n[0][N1N0 * (kb - 1 + t) + own_off];
+      qy[t] = n[1][N1N0 * (kb - 1 + t) + own_off];
+      if (SC) qf[t] = n[NC - 1][N1N0 * (kb - 1 + t) + own_off];
     }
-    qz = (double)n[2][N1N0 * kb + own_off];
+    qz = n[2][N1N0 * kb + own_off];
   }
   fetch_n_halo(kb);
   const double mdt = q.mdt, dx2 = q.dx2, dy2 = q.dy2, dz2 = q.dz2;
@@ -1072,13 +1078,13 @@ __global__ __launch_bounds__(kTileJ* TI, SC ? 3 : kStencilWaves) void k_stencil_
     const int s0 = (k - kb + 1) % 3, sp = (k - kb + 2) % 3, sm = (k - kb) % 3;
     stage_p(sp);
     {
-      const double own[4] = {qx[1], qy[1], qz, qf[1]};
+      const double own[4] = {(double)qx[1], (double)qy[1], (double)qz, (double)qf[1]};
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         double* d = L0 + c * P;
         if (own_ok) d[own_slot] = own[c];
-        if (c != 0 && hrow_ok) d[hrow_slot] = n_row[c];
-        if (c != 1 && hcol_ok) d[hcol_slot] = n_col[c];
+        if (c != 0 && hrow_ok) d[hrow_slot] = (double)n_row[c];
+        if (c != 1 && hcol_ok) d[hcol_slot] = (double)n_col[c];
       }
     }
     __syncthreads();
@@ -1086,14 +1092,14 @@ __global__ __launch_bounds__(kTileJ* TI, SC ? 3 : kStencilWaves) void k_stencil_
     if (k + 1 < ke) {
       fetch_n_halo(k + 1);
       if (own_ok) {
-        qx[3] = (double)n[0][m + 2 * N1N0];
-        qy[3] = (double)n[1][m + 2 * N1N0];
-        if (SC) qf[3] = (double)n[NC - 1][m + 2 * N1N0];
-        qz_next = (double)n[2][m + N1N0];
+        qx[3] = n[0][m + 2 * N1N0];
+        qy[3] = n[1][m + 2 * N1N0];
+        if (SC) qf[3] = n[NC - 1][m + 2 * N1N0];
+        qz_next = n[2][m + N1N0];
       }
     }
     if (active) {
-      const double v1[4] = {qx[1], qy[1], qz, qf[1]};
+      const double v1[4] = {(double)qx[1], (double)qy[1], (double)qz, (double)qf[1]};
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const double* c0 = L1 + (c * 3 + s0) * P + own_slot;
@@ -1118,19 +1124,19 @@ __global__ __launch_bounds__(kTileJ* TI, SC ? 3 : kStencilWaves) void k_stencil_
         const double* p2l = L1 + (2 * 3 + s0) * P + own_slot;
         const double p0p = L1[(0 * 3 + sp) * P + own_slot], p0m = L1[(0 * 3 + sm) * P + own_slot];
         const double p1p = L1[(1 * 3 + sp) * P + own_slot], p1m = L1[(1 * 3 + sm) * P + own_slot];
-        double ex = dt_(p0l[0]) - dt_(qx[1]);
-        double ey = dt_(p1l[0]) - dt_(qy[1]);
+        double ex = dt_(p0l[0]) - dt_((double)qx[1]);
+        double ey = dt_(p1l[0]) - dt_((double)qy[1]);
         float ez = (float)dt_(p2l[0]);
         ez = (float)((double)ez - dt_(n2l[0]));
         if (SC) {  // minus the gradient of phi^n (eval_lds_body's expressions)
           const double* fl = L0 + (NC - 1) * P + own_slot;
           ex = ex - dx_(fl[W] - fl[-W]);
           ey = ey - dy_(fl[1] - fl[-1]);
-          ez = (float)((double)ez - dz_(qf[2] - qf[0]));
+          ez = (float)((double)ez - dz_((double)qf[2] - (double)qf[0]));
         }
-        const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_(qy[2] - qy[0]) +
+        const float bx = (float)(0.5 * (dy_(n2l[1] - n2l[-1]) - dz_((double)qy[2] - (double)qy[0]) +
                                         dy_(p2l[1] - p2l[-1]) - dz_(p1p - p1m)));
-        const float by = (float)(0.5 * (dz_(qx[2] - qx[0]) - dx_(n2l[W] - n2l[-W]) +
+        const float by = (float)(0.5 * (dz_((double)qx[2] - (double)qx[0]) - dx_(n2l[W] - n2l[-W]) +
                                         dz_(p0p - p0m) - dx_(p2l[W] - p2l[-W])));
         const float bz = (float)(0.5 * (dx_(n1l[W] - n1l[-W]) - dy_(n0l[1] - n0l[-1]) +
                                         dx_(p1l[W] - p1l[-W]) - dy_(p0l[1] - p0l[-1])));
