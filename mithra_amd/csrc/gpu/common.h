@@ -195,8 +195,9 @@ struct Domain {
   // cell-row orders of the particle halves (sort.hip); ord[0]: rnp halves of the next deposit,
   // ord[1]: rnm halves
   CellOrder ord[2];
-  unsigned* key_alt;        // radix-sort double buffers
+  unsigned* key_alt;        // radix-sort double buffers (key_alt: the counting order's keys)
   unsigned* idx_alt;
+  unsigned* row_cnt;        // per-row counts of the counting order (rows + 1), zero between orders
   unsigned* sort_block;     // one allocation holding all of the above
   void* sort_tmp;
   size_t sort_tmp_bytes;
@@ -382,6 +383,7 @@ hipError_t launch_particle_ids(Domain* d);
 void free_sort(Domain* d);
 hipError_t ensure_orders(Domain* d, unsigned rows);
 hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows, unsigned span);
+hipError_t scan_row_counts(Domain* d, const unsigned* cnt, unsigned* row, unsigned rows);
 hipError_t reorder_particles(Domain* d, CellOrder& o);
 
 // Batched forms of the kernels (persistent workgroups walking a table of (instance, local tile)

@@ -1112,6 +1112,54 @@ __global__ __launch_bounds__(256) void kb_sort_scatter(BatchView* __restrict__ V
   }
 }
 
+// The single domain's row order by counting.  The particles are walked in a guide list, the
+// previous step's row order, in which the new rows are nearly sorted (a particle crosses a cell
+// row now and then), so runs of equal keys in a wavefront are long and its atomics few.  Pass
+// 1: the key of each listed particle (rnp, or rnm with use_rm) and the row counts.
+__global__ __launch_bounds__(256) void k_guided_keys(const double* __restrict__ part, long cap, long n, DepositConst c,
+                                                     int use_rm, unsigned rows, const unsigned* __restrict__ guide,
+                                                     unsigned* __restrict__ key, unsigned* __restrict__ cnt) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned k = kNoKey;
+  if (s < n) {
+    const long p = guide ? (long)guide[s] : s;
+    const int o = use_rm ? PXM : PX;
+    const double x = part[(o + 0) * cap + p], y = part[(o + 1) * cap + p], z = part[(o + 2) * cap + p];
+    k = rows;  // the bucket of halves that deposit nothing, last
+    if (inside(c, x, y, z)) {
+      const int i = (int)floor(div_rn(x - c.xmin, c.dx, c.ix)), j = (int)floor(div_rn(y - c.ymin, c.dy, c.iy)),
+                kk = (int)floor(div_rn(z - c.zmin, c.dz, c.iz));
+      if (cell_ok(c, i, j, kk)) k = (unsigned)kk * (unsigned)c.N0 + (unsigned)i;
+    }
+    key[s] = k;
+  }
+  const KeyRun run = key_run(k);
+  if (k != kNoKey && run.head) atomicAdd(&cnt[k], run.len);
+}
+
+// Pass 3 (after the counts are scanned into the row starts): each listed particle to its place
+// in its row.  The counts are taken back down to zero, ready for the next order.
+__global__ __launch_bounds__(256) void k_guided_scatter(const unsigned* __restrict__ key_in,
+                                                        const unsigned* __restrict__ guide, long n,
+                                                        const unsigned* __restrict__ row, unsigned* __restrict__ cnt,
+                                                        unsigned* __restrict__ key_out, unsigned* __restrict__ idx_out) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned k = s < n ? key_in[s] : kNoKey;
+  const KeyRun run = key_run(k);
+  unsigned end = 0;
+  if (k != kNoKey && run.head) end = atomicSub(&cnt[k], run.len);
+  // the run's first lane holds the end of the run's range; lanes of one run are consecutive
+  const int lane = __lane_id();
+  end = __shfl(end, lane - (int)run.o);
+  if (k != kNoKey) {
+    const unsigned pos = row[k] + end - run.len + run.o;  // < n whenever the counts are whole
+    if (pos < (unsigned)n) {
+      idx_out[pos] = guide ? guide[s] : (unsigned)s;
+      key_out[pos] = k;
+    }
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(kDepositThreads) void kb_deposit(const BatchView* __restrict__ V, BatchTiles T) {
   for (int r = 0;; ++r) {
@@ -1290,14 +1338,30 @@ DepositConst deposit_const(const Domain* d) {
   return c;
 }
 
-hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows) {
+// guide: a list of all particles in which the keys are nearly sorted (the other order), or
+// nullptr for array order.  The fixed-order deposit sorts by cell with the stable radix sort;
+// the default deposit counts rows in the guide's order.
+hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows, const unsigned* guide) {
   const DepositConst c = deposit_const(d);
-  const unsigned span = d->deterministic ? (unsigned)(c.N1 - 1) : 1u;  // cells, or rows
-  hipLaunchKernelGGL(k_row_keys, dim3((unsigned)((d->np + 255) / 256)), dim3(256), 0, d->stream, d->part, d->cap,
-                     d->np, c, use_rm, rows, span, o.key, o.idx);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  return sort_order(d, o, d->np, rows, span);
+  const unsigned grid = (unsigned)((d->np + 255) / 256);
+  hipError_t e;
+  if (d->deterministic) {
+    const unsigned span = (unsigned)(c.N1 - 1);  // cells
+    hipLaunchKernelGGL(k_row_keys, dim3(grid), dim3(256), 0, d->stream, d->part, d->cap, d->np, c, use_rm, rows,
+                       span, o.key, o.idx);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return sort_order(d, o, d->np, rows, span);
+  }
+  hipLaunchKernelGGL(k_guided_keys, dim3(grid), dim3(256), 0, d->stream, d->part, d->cap, d->np, c, use_rm, rows,
+                     guide, d->key_alt, d->row_cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_row_counts(d, d->row_cnt, o.row, rows)) != hipSuccess) {
+    (void)launch_fill_zero(d, d->row_cnt, sizeof(unsigned) * ((size_t)rows + 1));  // counts zero again
+    return e;
+  }
+  hipLaunchKernelGGL(k_guided_scatter, dim3(grid), dim3(256), 0, d->stream, d->key_alt, guide, d->np, o.row,
+                     d->row_cnt, o.key, o.idx);
+  return hipGetLastError();
 }
 
 // The fixed-order deposit's buffers: both orders' cell starts and both halves' weights.
@@ -1330,8 +1394,8 @@ hipError_t deposit_fixed_order(Domain* d, int accumulate, unsigned rows) {
   const unsigned cells = (unsigned)fixed_order_cells(d);
   hipError_t e;
   if ((e = ensure_fixed_order(d)) != hipSuccess) return e;
-  if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows)) != hipSuccess) return e;
-  if ((e = build_order(d, d->ord[0], 0, rows)) != hipSuccess) return e;
+  if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows, nullptr)) != hipSuccess) return e;
+  if ((e = build_order(d, d->ord[0], 0, rows, d->ord[1].idx)) != hipSuccess) return e;
   const DepositConst c = deposit_const(d);
   const dim3 gp((unsigned)((d->np + 255) / 256));
   for (int h = 0; h < 2; ++h) {
@@ -1438,8 +1502,8 @@ hipError_t launch_deposit(Domain* d, int accumulate) {
   }
   const unsigned rows = (unsigned)((g.N2 - 1) * (long)g.N0);
   if ((e = ensure_orders(d, rows)) != hipSuccess) return e;
-  if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows)) != hipSuccess) return e;
-  if ((e = build_order(d, d->ord[0], 0, rows)) != hipSuccess) return e;
+  if (!d->rm_order_valid && (e = build_order(d, d->ord[1], 1, rows, nullptr)) != hipSuccess) return e;
+  if ((e = build_order(d, d->ord[0], 0, rows, d->ord[1].idx)) != hipSuccess) return e;
   // A single domain must fill the GPU by itself: shorter chunks, then fewer rows per tile,
   // until there are some 2048 workgroups (each costs a little redundant work at its edges).
   int TI = tile_rows(d), KC = kDepositChunk;

@@ -4,10 +4,12 @@
 // few x-rows of the mesh, and needs every particle half whose cell lies in the adjacent cell
 // rows.  An "order" is the list of particle indices sorted by the row (k, i) of one half's cell,
 // plus the start of each row in that list; a workgroup then reads two contiguous ranges per
-// half.  Sorting is rocPRIM's LSD radix sort of 32-bit (row, index) pairs over the key's
-// significant bits only; it is stable, so the halves of one row keep their particle-array order.
-// The fixed-order deposit keys by cell instead ((k N0 + i)(N1 - 1) + j) and lists the start of
-// every cell.
+// half.  The default order is a counting sort that walks the particles in the previous step's
+// row order, where the new rows are nearly sorted (particles.hip: count, scan, scatter; the
+// order within a row is arbitrary, as the deposit's LDS additions allow).  The fixed-order
+// deposit keys by cell instead ((k N0 + i)(N1 - 1) + j) with rocPRIM's LSD radix sort of 32-bit
+// (cell, index) pairs over the key's significant bits, which is stable, so the halves of one
+// cell keep their particle-array order, and lists the start of every cell.
 //
 // The particle arrays themselves are permuted into row order now and then, so the sorted
 // indices walk memory almost sequentially; the original index travels with each particle in
@@ -15,6 +17,7 @@
 #include "common.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace mithra {
 namespace {
@@ -95,7 +98,7 @@ void free_sort(Domain* d) {
   for (int s = 0; s < 2; ++s) {
     d->ord[s].key = d->ord[s].idx = d->ord[s].row = d->ord[s].cell = nullptr;
   }
-  d->key_alt = d->idx_alt = nullptr;
+  d->key_alt = d->idx_alt = d->row_cnt = nullptr;
   d->rm_order_valid = 0;
   d->order1_is_rnp = 0;
 }
@@ -105,7 +108,7 @@ hipError_t ensure_orders(Domain* d, unsigned rows) {
   free_sort(d);
   const size_t cap = (size_t)d->cap;
   const size_t per = (cap + 63) / 64 * 64, rper = ((size_t)rows + 1 + 63) / 64 * 64;
-  hipError_t e = hipMalloc(&d->sort_block, sizeof(unsigned) * (6 * per + 2 * rper));
+  hipError_t e = hipMalloc(&d->sort_block, sizeof(unsigned) * (6 * per + 3 * rper));
   if (e != hipSuccess) return e;
   unsigned* b = d->sort_block;
   for (int s = 0; s < 2; ++s) {
@@ -114,10 +117,11 @@ hipError_t ensure_orders(Domain* d, unsigned rows) {
     d->ord[s].row = b; b += rper;
   }
   d->key_alt = b; b += per;
-  d->idx_alt = b;
+  d->idx_alt = b; b += per;
+  d->row_cnt = b;
   d->sort_cap = d->cap;
   d->sort_rows = rows;
-  return hipSuccess;
+  return launch_fill_zero(d, d->row_cnt, sizeof(unsigned) * rper);
 }
 
 // Sorts the keys of order o (key units: rows, or cells when span > 1 with `rows` * span cells)
@@ -155,6 +159,25 @@ hipError_t sort_order(Domain* d, CellOrder& o, long n, unsigned rows, unsigned s
     hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((units + 1 + 255) / 256)), dim3(256), 0, d->stream, o.key, o.row,
                        rows, span, o.cell);
   return hipGetLastError();
+}
+
+// row[r] = cnt[0] + ... + cnt[r-1] for r in [0, rows]: the start of every row of a counting
+// order (cnt has rows + 1 entries, the last one the bucket of halves that deposit nothing).
+hipError_t scan_row_counts(Domain* d, const unsigned* cnt, unsigned* row, unsigned rows) {
+  size_t bytes = 0;
+  hipError_t e = rocprim::exclusive_scan(nullptr, bytes, cnt, row, 0u, (size_t)rows + 1, rocprim::plus<unsigned>(),
+                                         d->stream);
+  if (e != hipSuccess) return e;
+  if (bytes > d->sort_tmp_bytes || !d->sort_tmp) {
+    (void)hipFree(d->sort_tmp);
+    d->sort_tmp = nullptr;
+    d->sort_tmp_bytes = 0;
+    if ((e = hipMalloc(&d->sort_tmp, bytes ? bytes : 1)) != hipSuccess) return e;
+    d->sort_tmp_bytes = bytes;
+  }
+  bytes = d->sort_tmp_bytes;
+  return rocprim::exclusive_scan(d->sort_tmp, bytes, cnt, row, 0u, (size_t)rows + 1, rocprim::plus<unsigned>(),
+                                 d->stream);
 }
 
 hipError_t reorder_particles(Domain* d, CellOrder& o) {
