@@ -1361,7 +1361,9 @@ hipError_t build_order(Domain* d, CellOrder& o, int use_rm, unsigned rows, const
   }
   hipLaunchKernelGGL(k_guided_scatter, dim3(grid), dim3(256), 0, d->stream, d->key_alt, guide, d->np, o.row,
                      d->row_cnt, o.key, o.idx);
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess)
+    (void)launch_fill_zero(d, d->row_cnt, sizeof(unsigned) * ((size_t)rows + 1));  // counts zero again
+  return e;
 }
 
 // The fixed-order deposit's buffers: both orders' cell starts and both halves' weights.
